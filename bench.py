@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the headline workload: FP64-accurate solves/s at n=16384 with the
+mixed-precision LU + iterative refinement, next to the plain FP64 GPU LU, plus FP64 GEMM
+fraction of the matrix-core peak.
+
+One step = one complete FP64-accurate solve of A x = b (A resident in HBM): FP16 (or BF16)
+matrix-core LU with FP32 panels, then refinement with FP64 residuals until the normwise
+backward error is <= 1e-14.  A = U diag(s) V^T, U and V from QR of N(0,1) matrices,
+s geometric from 1 to 1/kappa; b = A x_true with x_true uniform[-1,1].
+
+Prints one JSON line (see README of the driver contract): value = solves/s on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "f16": 2.5e15, "bf16": 2.5e15, "hbm": 8.0e12}
+METRIC = "FP64-accurate solves/s, n=16384: mixed-prec LU+IR vs FP64 LU; GEMM %peak"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_system(n, kappa, nrhs, seed, torch):
+    """A = U diag(s) V^T (QR of Gaussians), x_true uniform[-1,1], b = A x_true (FP64)."""
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    t0 = time.time()
+    U, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    log(f"QR 1 done {time.time() - t0:.1f}s")
+    V, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    log(f"QR 2 done {time.time() - t0:.1f}s")
+    s = kappa ** (-torch.arange(n, dtype=torch.float64, device=dev) / (n - 1))
+    A = (U * s) @ V.T.contiguous()
+    del U, V
+    xt = torch.rand(n, nrhs, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    b = A @ xt
+    torch.cuda.synchronize()
+    log(f"system ready {time.time() - t0:.1f}s")
+    return A.contiguous(), b.contiguous(), xt
+
+
+def backward_error(torch, A, x, b):
+    r = b - A @ x
+    rn = r.abs().amax(dim=0)
+    xn = x.abs().amax(dim=0)
+    an = A.abs().sum(dim=1).amax()
+    return float((rn / (an * xn)).max())
+
+
+def time_gemm(ddla, n, reps):
+    A = ddla.DeviceMatrix.random(n, n, 0xDD1A0001)
+    B = ddla.DeviceMatrix.random(n, n, 0xDD1A0002)
+    C = ddla.DeviceMatrix(n, n)
+    ddla.BlasProvider.GeMM(1.0, A, B, 0.0, C)
+    ddla.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ddla.BlasProvider.GeMM(1.0, A, B, 0.0, C)
+    ddla.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    tf = 2.0 * n ** 3 / dt
+    return {"n": n, "ms": dt * 1e3, "tflops": tf / 1e12, "frac_fp64_peak": tf / PEAK["f64"]}
+
+
+def roofline_of(name, e):
+    """Roofline object for one profiled kernel (totals over the timed region)."""
+    if e["ms"] <= 0 or e["count"] == 0:
+        return None
+    sec = e["ms"] / 1e3
+    prec = "f16" if "f16" in name else "bf16" if "bf16" in name else "f32" if "f32" in name else "f64"
+    peak_f = PEAK[prec]
+    ai = e["flops"] / e["bytes"] if e["bytes"] > 0 else float("inf")
+    ridge = peak_f / PEAK["hbm"]
+    per_launch_ms = e["ms"] / e["count"]
+    if e["flops"] > 0 and ai >= ridge:
+        ach = e["flops"] / sec
+        return {"kernel": name, "bound": "mfma", "achieved": ach / 1e12, "peak": peak_f / 1e12,
+                "unit": "TFLOP/s", "frac": ach / peak_f, "traffic": None,
+                "avg_launch_ms": per_launch_ms, "launches": e["count"],
+                "flops_per_launch": e["flops"] / e["count"]}
+    ach = e["bytes"] / sec
+    return {"kernel": name, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK["hbm"] / 1e9,
+            "unit": "GB/s", "frac": ach / PEAK["hbm"], "traffic": None,
+            "avg_launch_ms": per_launch_ms, "launches": e["count"],
+            "bytes_per_launch": e["bytes"] / e["count"]}
+
+
+def cpu_baseline(n_target, n_sample, nrhs):
+    """The oracle (CPU restatement of DDLA's managed FP64 LU + solve) on a bounded sample."""
+    from oracle import oracle as O
+    nproc = os.cpu_count() or 2
+    threads = max(1, min(nproc // 2, 16))  # reference policy ProcessorCount/2, box share 16
+    O.set_threads(threads)
+    A = O.uniform_matrix(n_sample, n_sample, 0xDD1A0002)
+    B = O.uniform_matrix(n_sample, nrhs, 0xDD1A0003)
+    t0 = time.perf_counter()
+    info, piv = O.getrf(A)
+    O.getrs(A, piv, B)
+    dt = time.perf_counter() - t0
+    scale = (n_target / n_sample) ** 3
+    return {"value": 1.0 / (dt * scale), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle FP64 blocked LU (block 128) + solve at n={n_sample} took "
+                       f"{dt:.2f}s on {threads} threads; scaled by (n/{n_sample})^3={scale:.0f} "
+                       f"to n={n_target}")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--kappa", type=float, default=1e6)
+    ap.add_argument("--prec", choices=["fp16", "bf16", "fp32"], default="fp16")
+    ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0xDD1A0003)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this backend has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ddla_amd as ddla
+
+    ddla.init(0)
+    n, nrhs = args.n, args.nrhs
+    prec = {"fp16": ddla.PREC_FP16, "bf16": ddla.PREC_BF16, "fp32": ddla.PREC_FP32}[args.prec]
+    log(f"n={n} kappa={args.kappa:g} prec={args.prec} nrhs={nrhs}")
+    A_t, b_t, xt = make_system(n, args.kappa, nrhs, args.seed, torch)
+    A = ddla.DeviceMatrix.wrap(A_t.data_ptr(), n, n, n, A_t)
+    B = ddla.DeviceMatrix.wrap(b_t.data_ptr(), n, nrhs, nrhs, b_t)
+    X_t = torch.zeros(n, nrhs, dtype=torch.float64, device="cuda")
+    X = ddla.DeviceMatrix.wrap(X_t.data_ptr(), n, nrhs, nrhs, X_t)
+    torch.cuda.synchronize()
+
+    # ---- mixed precision LU + IR (the measured step)
+    for w in range(args.warmup):
+        info = ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14)
+        log(f"warmup {w}: status={info.status} iters={info.iters} berr={info.backward_error:.2e} "
+            f"factor={info.factor_ms:.1f}ms refine={info.refine_ms:.1f}ms")
+    ddla.profile_reset()
+    ddla.profile_enable(True)
+    ddla.synchronize()
+    t0 = time.perf_counter()
+    infos = []
+    for k in range(args.steps):
+        infos.append(ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14))
+    ddla.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    ddla.profile_enable(False)
+    prof = ddla.profile_read()
+    last = infos[-1]
+    log(f"mixed: {t_mixed * 1e3:.1f} ms/solve status={last.status} iters={last.iters} "
+        f"berr={last.backward_error:.2e}")
+    torch.cuda.synchronize()
+    be_mixed = backward_error(torch, A_t, X_t, b_t)
+    fe_mixed = float(((X_t - xt).abs().amax() / xt.abs().amax()))
+
+    # ---- plain FP64 GPU LU + solve on the same system
+    W_t = torch.empty_like(A_t)
+    Xf_t = torch.empty_like(b_t)
+    W = ddla.DeviceMatrix.wrap(W_t.data_ptr(), n, n, n, W_t)
+    Xf = ddla.DeviceMatrix.wrap(Xf_t.data_ptr(), n, nrhs, nrhs, Xf_t)
+    t_fp64 = []
+    for k in range(args.warmup + max(1, min(args.steps, 2))):
+        W_t.copy_(A_t)
+        Xf_t.copy_(b_t)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        piv = ddla.PLUDecompose(W)
+        ddla.PLUSolve(W, piv, Xf)
+        ddla.synchronize()
+        if k >= args.warmup:
+            t_fp64.append(time.perf_counter() - t0)
+    t_fp64 = sum(t_fp64) / len(t_fp64)
+    be_fp64 = backward_error(torch, A_t, Xf_t, b_t)
+    fe_vs_fp64 = float(((X_t - Xf_t).abs().amax() / Xf_t.abs().amax()))
+    log(f"fp64 LU: {t_fp64 * 1e3:.1f} ms/solve berr={be_fp64:.2e}")
+    del W, W_t
+
+    # ---- FP64 GEMM (config: M=N=K=1024) and a large square GEMM
+    gemm = [time_gemm(ddla, 1024, 50), time_gemm(ddla, 8192, 5)]
+    log("gemm", gemm)
+
+    # ---- roofline of the dominant kernel of the timed region
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])[0] if prof else None
+    roof = roofline_of(dom, prof[dom]) if dom else None
+    breakdown = {k: {"launches": v["count"], "ms_per_step": v["ms"] / args.steps}
+                 for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(n, min(args.cpu_sample, n), nrhs)
+        log("cpu", cpu)
+
+    out = {
+        "metric": METRIC,
+        "value": 1.0 / t_mixed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{args.prec}-mfma/fp32/fp64",
+        "data": "synthetic",
+        "config": {"workload": f"mixed-precision LU + iterative refinement, n={n}, kappa={args.kappa:g}, "
+                               f"{args.prec} trailing updates, nrhs={nrhs}",
+                   "n": n, "kappa": args.kappa, "factor_prec": args.prec, "nrhs": nrhs,
+                   "tol": 1e-14},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "mixed": {"iters": last.iters, "status": last.status,
+                  "backward_error": be_mixed, "forward_error_vs_xtrue": fe_mixed,
+                  "forward_error_vs_fp64_lu": fe_vs_fp64,
+                  "factor_ms": last.factor_ms, "refine_ms": last.refine_ms},
+        "fp64_lu": {"ms_per_solve": t_fp64 * 1e3, "solves_per_s": 1.0 / t_fp64,
+                    "backward_error": be_fp64,
+                    "gflops": (2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs) / t_fp64 / 1e9},
+        "speedup_vs_fp64_lu": t_fp64 / t_mixed,
+        "gemm_fp64": gemm,
+        "kernel_breakdown": breakdown,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

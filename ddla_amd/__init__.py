@@ -1,0 +1,417 @@
+"""DDLA's dense hot path on MI355X: Python mirror of DDLA's GEMM / LU / Cholesky API over
+the C ABI of libddla_hip.so (hand-written HIP kernels for gfx950).
+
+Names, argument meaning and error behaviour follow the reference C# API:
+  BlasProvider.GeMM / TrSM / SyRk / AMax   DDLA/BLAS/3m.cs:10-314, DDLA/BLAS/1v.cs:30-41
+  LU, PLUDecompose, PLUSolve, ApplyPiv     DDLA/Factorizations/LU.cs:13-465
+  Cholesky, CholDecompose, CholeskySolve   DDLA/Factorizations/Cholesky.cs:10-236
+Matrices are numpy arrays on the host (any strides, like MatrixView) or DeviceMatrix
+objects that stay resident in HBM.  There is no CPU fallback: every call runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native
+from .native import (LEFT, LOWER, NO_TRANSPOSE, NONUNIT_DIAG, RIGHT, TRANSPOSE, UNIT_DIAG, UPPER,
+                     PREC_BF16, PREC_FP16, PREC_FP32, PREC_FP64, DDLAError, MpirInfo, MpirOpts,
+                     check)
+
+__all__ = [
+    "DeviceMatrix", "BlasProvider", "LU", "Cholesky", "PLUDecompose", "PLUSolve", "ApplyPiv",
+    "CholDecompose", "CholeskySolve", "LinalgException", "gesv_mpir", "posv_mpir", "init",
+    "synchronize", "NO_TRANSPOSE", "TRANSPOSE", "LOWER", "UPPER", "UNIT_DIAG", "NONUNIT_DIAG",
+    "LEFT", "RIGHT", "PREC_FP64", "PREC_FP32", "PREC_FP16", "PREC_BF16", "DDLAError",
+]
+
+
+class LinalgException(ArithmeticError):
+    """Non-positive Cholesky pivot (DDLA/Core/LinalgException.cs, Cholesky.cs:228-236)."""
+
+
+def _lib():
+    return native.lib()
+
+
+def init(device: int = 0):
+    check(_lib().ddla_init(device))
+
+
+def synchronize():
+    check(_lib().ddla_sync())
+
+
+def stream_handle() -> int:
+    """The library's hipStream_t as an integer (for torch.cuda.ExternalStream)."""
+    p = ctypes.c_void_p()
+    check(_lib().ddla_get_stream(ctypes.byref(p)))
+    return p.value or 0
+
+
+def profile_enable(on: bool = True):
+    check(_lib().ddla_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(_lib().ddla_profile_reset())
+
+
+def profile_read() -> dict:
+    """{kernel: {count, ms, flops, bytes}} accumulated while profiling was enabled."""
+    L = _lib()
+    n = L.ddla_profile_read(-1, None, 0, None, None, None, None)
+    out = {}
+    for i in range(n):
+        name = ctypes.create_string_buffer(128)
+        cnt, ms, fl, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(L.ddla_profile_read(i, name, 128, ctypes.byref(cnt), ctypes.byref(ms),
+                                  ctypes.byref(fl), ctypes.byref(by)))
+        out[name.value.decode()] = {"count": cnt.value, "ms": ms.value, "flops": fl.value,
+                                    "bytes": by.value}
+    return out
+
+
+def _as_f64(a):
+    a = np.asarray(a)
+    if a.dtype != np.float64:
+        raise TypeError("DDLA matrices are float64")
+    return a
+
+
+def _strides(a):
+    return a.strides[0] // 8, a.strides[1] // 8
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class DeviceMatrix:
+    """Row-major FP64 matrix resident in device memory (element (i,j) at ptr[i*ld + j])."""
+
+    def __init__(self, rows: int, cols: int, ld: int | None = None):
+        self.rows, self.cols = int(rows), int(cols)
+        self.ld = int(ld) if ld is not None else max(self.cols, 1)
+        p = ctypes.c_void_p()
+        check(_lib().ddla_malloc(ctypes.byref(p), max(self.rows * self.ld, 1) * 8))
+        self._ptr = p
+        self._owner = True
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._ptr
+
+    @property
+    def shape(self):
+        return (self.rows, self.cols)
+
+    @classmethod
+    def from_host(cls, a) -> "DeviceMatrix":
+        a = _as_f64(a)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        d = cls(a.shape[0], a.shape[1])
+        d.set(a)
+        return d
+
+    @classmethod
+    def wrap(cls, ptr: int, rows: int, cols: int, ld: int, keepalive=None) -> "DeviceMatrix":
+        """A non-owning view of device memory allocated elsewhere (e.g. a torch tensor)."""
+        v = object.__new__(cls)
+        v.rows, v.cols, v.ld = int(rows), int(cols), int(ld)
+        v._ptr = ctypes.c_void_p(int(ptr))
+        v._owner = False
+        v._base = keepalive
+        return v
+
+    @classmethod
+    def random(cls, rows: int, cols: int, seed: int) -> "DeviceMatrix":
+        """SplitMix64 uniform[-1,1) fill on the device (same stream as the CPU oracle)."""
+        d = cls(rows, cols)
+        check(_lib().ddla_drandm_dev(d.rows, d.cols, d.ptr, d.ld, ctypes.c_uint64(seed)))
+        return d
+
+    def set(self, a):
+        a = _as_f64(a)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        assert a.shape == self.shape
+        check(_lib().ddla_dmat_set(self.rows, self.cols, _ptr(a), *_strides(a), self.ptr, self.ld))
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty((self.rows, self.cols), dtype=np.float64)
+        check(_lib().ddla_dmat_get(self.rows, self.cols, self.ptr, self.ld, _ptr(out), *_strides(out)))
+        return out
+
+    def sub(self, i: int, m: int, j: int, n: int) -> "DeviceMatrix":
+        """A non-owning view of rows [i, i+m) and columns [j, j+n)."""
+        v = object.__new__(DeviceMatrix)
+        v.rows, v.cols, v.ld = m, n, self.ld
+        v._ptr = ctypes.c_void_p(self._ptr.value + (i * self.ld + j) * 8)
+        v._owner = False
+        v._base = self
+        return v
+
+    def free(self):
+        if getattr(self, "_owner", False) and self._ptr:
+            _lib().ddla_free(self._ptr)
+            self._ptr = ctypes.c_void_p()
+            self._owner = False
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BlasProvider:
+    """GPU replacements for the BLAS entry points on DDLA's dense path."""
+
+    @staticmethod
+    def GeMM(alpha, A, B, beta, C, transA=NO_TRANSPOSE, transB=NO_TRANSPOSE):
+        """C = alpha*op(A)*op(B) + beta*C (DDLA/BLAS/3m.cs:10-42 -> bli_dgemm)."""
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        if isinstance(C, DeviceMatrix):
+            m, n = C.shape
+            k = A.rows if transA == TRANSPOSE else A.cols
+            return check(_lib().ddla_dgemm_dev(transA, transB, m, n, k, ctypes.byref(al), A.ptr,
+                                               A.ld, 1, B.ptr, B.ld, 1, ctypes.byref(be), C.ptr,
+                                               C.ld, 1))
+        A, B, C = _as_f64(A), _as_f64(B), _as_f64(C)
+        m, n = C.shape
+        k = A.shape[0] if transA == TRANSPOSE else A.shape[1]
+        ka, kb = (A.shape[1] if transA == TRANSPOSE else A.shape[0]), \
+            (B.shape[0] if transB == TRANSPOSE else B.shape[1])
+        if ka != m or kb != n or (B.shape[1] if transB == TRANSPOSE else B.shape[0]) != k:
+            raise ValueError("Dimensions of matrices A, B and C must match")
+        return check(_lib().ddla_dgemm(transA, transB, m, n, k, ctypes.byref(al), _ptr(A),
+                                       *_strides(A), _ptr(B), *_strides(B), ctypes.byref(be),
+                                       _ptr(C), *_strides(C)))
+
+    @staticmethod
+    def TrSM(side, uplo, trans, diag, alpha, A: DeviceMatrix, B: DeviceMatrix):
+        """B = alpha*op(A)^-1*B or alpha*B*op(A)^-1 (DDLA/BLAS/3m.cs:287-314)."""
+        al = ctypes.c_double(alpha)
+        m, n = B.shape
+        return check(_lib().ddla_dtrsm_dev(side, uplo, trans, diag, m, n, ctypes.byref(al), A.ptr,
+                                           A.ld, B.ptr, B.ld))
+
+    @staticmethod
+    def SyRk(uplo, trans, alpha, A: DeviceMatrix, beta, C: DeviceMatrix):
+        """uplo(C) = beta*C + alpha*op(A)*op(A)^T (DDLA/BLAS/3m.cs:118-147)."""
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        k = A.rows if trans == TRANSPOSE else A.cols
+        return check(_lib().ddla_dsyrk_dev(uplo, trans, C.rows, k, ctypes.byref(al), A.ptr, A.ld,
+                                           ctypes.byref(be), C.ptr, C.ld))
+
+    @staticmethod
+    def AMax(x: DeviceMatrix, inc: int = 1) -> int:
+        """Index of the first max |x_i| of a device vector (DDLA/BLAS/1v.cs:30-41)."""
+        n = x.rows * x.cols if inc == 1 else x.rows
+        out = ctypes.c_int64()
+        check(_lib().ddla_damaxv_dev(n, x.ptr, inc if inc != 1 else 1, ctypes.byref(out)))
+        return int(out.value)
+
+
+def _zero_pivot(info: int):
+    # reference: InvScal throws ArgumentOutOfRangeException for a zero pivot (1v.cs:99)
+    raise ZeroDivisionError(f"zero pivot at step {info - 1}")
+
+
+def PLUDecompose(A, pivs=None):
+    """Blocked PLU in place (LU.cs:251-264). A: numpy (any strides) or DeviceMatrix.
+
+    Returns the relative pivots (int32[n]); raises on a zero pivot like the reference."""
+    if isinstance(A, DeviceMatrix):
+        n = A.rows
+        if A.cols != n:
+            raise ValueError("Matrix must be square for LU factorization.")
+        dpiv = _DevInts(n)
+        info = check(_lib().ddla_dgetrf_dev(n, A.ptr, A.ld, dpiv.ptr))
+        piv = dpiv.to_host()
+        if pivs is not None:
+            pivs[:] = piv
+        if info:
+            _zero_pivot(info)
+        return piv
+    A = _as_f64(A)
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise ValueError("Matrix must be square for LU factorization.")
+    piv = np.zeros(n, dtype=np.int32) if pivs is None else pivs
+    if piv.shape != (n,) or piv.dtype != np.int32:
+        raise ValueError("The length of pivs must match the number of rows in the matrix.")
+    info = check(_lib().ddla_dgetrf(n, _ptr(A), *_strides(A), _ptr(piv)))
+    if info:
+        _zero_pivot(info)
+    return piv
+
+
+def PLUSolve(A, pivs, B, trans=False):
+    """B <- A^-1 B (or A^-T B) from the packed factors (LU.cs:266-287)."""
+    if isinstance(A, DeviceMatrix):
+        dpiv = pivs if isinstance(pivs, _DevInts) else _DevInts.from_host(pivs)
+        return check(_lib().ddla_dgetrs_dev(TRANSPOSE if trans else NO_TRANSPOSE, A.rows, B.cols,
+                                            A.ptr, A.ld, dpiv.ptr, B.ptr, B.ld))
+    A, B = _as_f64(A), _as_f64(B)
+    if B.ndim == 1:
+        B2 = B.reshape(-1, 1)
+    else:
+        B2 = B
+    if A.shape[0] != B2.shape[0]:
+        raise ValueError("RHS matrix size mismatch")
+    piv = np.ascontiguousarray(pivs, dtype=np.int32)
+    return check(_lib().ddla_dgetrs(TRANSPOSE if trans else NO_TRANSPOSE, A.shape[0], B2.shape[1],
+                                    _ptr(A), *_strides(A), _ptr(piv), _ptr(B2), *_strides(B2)))
+
+
+def ApplyPiv(B: DeviceMatrix, pivs, side=LEFT, forward=True):
+    """Row (left) or column (right) interchanges with relative pivots (LU.cs:237-249)."""
+    npiv = len(pivs) if not isinstance(pivs, _DevInts) else pivs.n
+    dpiv = pivs if isinstance(pivs, _DevInts) else _DevInts.from_host(pivs)
+    return check(_lib().ddla_dlaswp_dev(side, 1 if forward else 0, B.rows, B.cols, B.ptr, B.ld,
+                                        dpiv.ptr, npiv))
+
+
+class _DevInts:
+    def __init__(self, n):
+        self.n = int(n)
+        p = ctypes.c_void_p()
+        check(_lib().ddla_malloc(ctypes.byref(p), max(self.n, 1) * 4))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        d = cls(a.size)
+        check(_lib().ddla_memcpy_h2d(d.ptr, _ptr(a), a.size * 4))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.n, dtype=np.int32)
+        check(_lib().ddla_memcpy_d2h(_ptr(out), self.ptr, self.n * 4))
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                _lib().ddla_free(self.ptr)
+        except Exception:
+            pass
+
+
+class LU:
+    """LU with partial pivoting (LU.cs:13-128); the factorization runs once, lazily."""
+
+    def __init__(self, A, inplace=False, usePiv=True):
+        if not usePiv:
+            raise NotImplementedError("LU without pivoting is not on the GPU path")
+        A = _as_f64(A)
+        n = A.shape[0]
+        if A.shape[1] != n:
+            raise ValueError("A must be square")
+        self.n = n
+        self.matrix = A if inplace else np.array(A, order="C", copy=True)
+        self.pivs = np.zeros(n, dtype=np.int32)
+        self._computed = False
+
+    def _compute_once(self):
+        if not self._computed:
+            PLUDecompose(self.matrix, self.pivs)
+            self._computed = True
+
+    def Solve(self, b, trans=False, inplace=False):
+        """Solves A x = b (the reference's vector overload ignores trans, LU.cs:77; this
+        mirror honours it)."""
+        self._compute_once()
+        b = _as_f64(b)
+        if b.shape[0] != self.n:
+            raise ValueError("RHS vector size mismatch")
+        x = b if inplace else np.array(b, copy=True)
+        PLUSolve(self.matrix, self.pivs, x, trans)
+        return x
+
+    def Deconstruct(self):
+        self._compute_once()
+        n = self.n
+        P = np.eye(n)
+        d = DeviceMatrix.from_host(P)
+        ApplyPiv(d, self.pivs)
+        P = d.to_host()
+        L = np.tril(self.matrix, -1) + np.eye(n)
+        U = np.triu(self.matrix)
+        return P, L, U
+
+
+def CholDecompose(uplo, A):
+    """Blocked Cholesky in place (Cholesky.cs:52-69); raises LinalgException if not SPD."""
+    if isinstance(A, DeviceMatrix):
+        info = check(_lib().ddla_dpotrf_dev(uplo, A.rows, A.ptr, A.ld))
+    else:
+        A = _as_f64(A)
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("Matrix must be square")
+        info = check(_lib().ddla_dpotrf(uplo, A.shape[0], _ptr(A), *_strides(A)))
+    if info:
+        raise LinalgException(f"Cholesky factorization encountered a non-positive pivot at {info - 1}")
+    return 0
+
+
+def CholeskySolve(uplo, A, B):
+    """B <- A^-1 B with a Cholesky factor (Cholesky.cs:102-123)."""
+    if isinstance(A, DeviceMatrix):
+        return check(_lib().ddla_dpotrs_dev(uplo, A.rows, B.cols, A.ptr, A.ld, B.ptr, B.ld))
+    A, B = _as_f64(A), _as_f64(B)
+    B2 = B.reshape(-1, 1) if B.ndim == 1 else B
+    if B2.shape[0] != A.shape[0]:
+        raise ValueError("Matrix dimensions do not match for Cholesky solve.")
+    return check(_lib().ddla_dpotrs(uplo, A.shape[0], B2.shape[1], _ptr(A), *_strides(A),
+                                    _ptr(B2), *_strides(B2)))
+
+
+class Cholesky:
+    """Cholesky factorization (Cholesky.cs:10-88)."""
+
+    def __init__(self, A, uplo=LOWER, inplace=False):
+        A = _as_f64(A)
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("Matrix must be square")
+        self.uplo = uplo
+        self.matrix = A if inplace else np.array(A, order="C", copy=True)
+        self._computed = False
+
+    def ComputeOnce(self):
+        if not self._computed:
+            self._computed = True
+            CholDecompose(self.uplo, self.matrix)
+
+    def Solve(self, b, inplace=False):
+        self.ComputeOnce()
+        x = b if inplace else np.array(_as_f64(b), copy=True)
+        CholeskySolve(self.uplo, self.matrix, x)
+        return x
+
+
+def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_iters, gmres):
+    opts = MpirOpts(prec, max_iters, tol, gmres, 0)
+    info = MpirInfo()
+    st = fn(A.rows, B.cols, A.ptr, A.ld, B.ptr, B.ld, X.ptr, X.ld, ctypes.byref(opts),
+            ctypes.byref(info))
+    if st < 0 and st != native.ERR_NOT_CONVERGED:
+        check(st)
+    return info
+
+
+def gesv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16, tol=1e-14,
+              max_iters=50, gmres=1) -> MpirInfo:
+    """Mixed-precision LU + iterative refinement to FP64 accuracy (no reference counterpart)."""
+    return _mpir(_lib().ddla_dgesv_mpir_dev, A, B, X, prec, tol, max_iters, gmres)
+
+
+def posv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16, tol=1e-14,
+              max_iters=50, gmres=1) -> MpirInfo:
+    """Mixed-precision Cholesky + iterative refinement (no reference counterpart)."""
+    return _mpir(_lib().ddla_dposv_mpir_dev, A, B, X, prec, tol, max_iters, gmres)

@@ -1,0 +1,185 @@
+// Internal declarations shared by the HIP translation units of libddla_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ddla_native.h"
+
+namespace ddla {
+
+// ------------------------------------------------------------------ errors ---------------
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define DDLA_HIP_CHECK(expr)                                                      \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      ::ddla::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),    \
+                        __FILE__, __LINE__);                                      \
+      return DDLA_ERR_HIP;                                                        \
+    }                                                                             \
+  } while (0)
+
+#define DDLA_ARG_CHECK(cond, ...)                 \
+  do {                                            \
+    if (!(cond)) {                                \
+      ::ddla::set_error(__VA_ARGS__);             \
+      return DDLA_ERR_INVALID_ARG;                \
+    }                                             \
+  } while (0)
+
+#define DDLA_RETURN_IF(st)       \
+  do {                           \
+    int _s = (st);               \
+    if (_s != 0) return _s;      \
+  } while (0)
+
+// ---------------------------------------------------------------- context ----------------
+struct Context {
+  bool ready = false;
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  // growable scratch buffers (device memory kept for the life of the context)
+  void* ws[8] = {};
+  size_t ws_bytes[8] = {};
+};
+
+Context& ctx();
+int ensure_init();
+hipStream_t stream();
+// Returns a device scratch buffer `slot` of at least `bytes` (contents undefined).
+int workspace(int slot, size_t bytes, void** out);
+enum WorkspaceSlot {
+  WS_PANEL = 0,   // panel-kernel exchange slots
+  WS_PIVOT = 1,   // pivots / permutations / info words
+  WS_TEMP = 2,    // general temporaries (packed operands, transposes)
+  WS_MIXED = 3,   // low-precision working copies
+  WS_MIXED2 = 4,
+  WS_REFINE = 5,  // refinement vectors
+  WS_GMRES = 6,
+  WS_SMALL = 7,   // small reduction outputs
+};
+
+// Optional per-kernel timing (ddla_profile_*): while enabled, a ProfScope around a launch
+// records HIP events on the launch stream and accumulates the kernel's time together with
+// its algorithmic flops and bytes under `name`.
+bool profiling();
+class ProfScope {
+ public:
+  ProfScope(const char* name, double flops, double bytes, hipStream_t s);
+  ~ProfScope();
+  ProfScope(const ProfScope&) = delete;
+  ProfScope& operator=(const ProfScope&) = delete;
+
+ private:
+  hipStream_t s_;
+  int entry_ = -1;
+  double flops_ = 0.0, bytes_ = 0.0;
+  void* a_ = nullptr;
+  void* b_ = nullptr;
+};
+void profile_enable(bool on);
+void profile_reset();
+int profile_read(int idx, char* name, size_t len, int64_t* count, double* ms, double* flops,
+                 double* bytes);
+
+inline int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
+
+// Maps a linear workgroup id to a (tile_m, tile_n) pair of a GEMM: workgroup ids are
+// dispatched round-robin over the 8 XCDs, so ids are first regrouped into XCD-contiguous
+// bands; each band walks a grouped order (8 tile rows at a time) so the A row panels and
+// B column panels it reuses stay in that XCD's L2.
+__device__ __forceinline__ void gemm_tile_of(int pid, int tiles_m, int tiles_n, int& tm, int& tn) {
+  const int T = tiles_m * tiles_n;
+  const int T8 = (T / 8) * 8;
+  if (pid < T8) pid = (pid % 8) * (T8 / 8) + pid / 8;
+  constexpr int G = 8;
+  const int per_group = G * tiles_n;
+  const int group = pid / per_group;
+  const int first_m = group * G;
+  const int gm = min(tiles_m - first_m, G);
+  const int in_group = pid % per_group;
+  tm = first_m + in_group % gm;
+  tn = in_group / gm;
+}
+
+// ------------------------------------------------------------ kernels (launchers) --------
+// FP64 GEMM on row-major C: C = alpha*A*B + beta*C, A m x k, B k x n.
+//   a_kcontig: A(i,k) at A[i*lda + k], else A[i + k*lda]
+//   b_ncontig: B(k,j) at B[k*ldb + j], else B[k + j*ldb]
+//   tri: 0 full, 1 lower triangle of C only (i >= j), 2 upper (i <= j)
+int launch_dgemm(int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                 bool a_kcontig, const double* B, int64_t ldb, bool b_ncontig, double beta,
+                 double* C, int64_t ldc, int tri, hipStream_t s);
+
+int launch_sgemm(int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                 bool a_kcontig, const float* B, int64_t ldb, bool b_ncontig, float beta, float* C,
+                 int64_t ldc, int tri, hipStream_t s);
+
+// C += alpha*A*B on the FP16 (bf16=false) or BF16 matrix cores, FP32 C, FP32 operands
+// rounded per 128-block with power-of-two scales; scale_ws holds >= tiles_m + tiles_n floats
+int launch_hgemm(bool bf16, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k, float alpha,
+                 const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                 float* scale_ws, hipStream_t s);
+
+// general strided copy / transpose helpers
+int launch_dcopy2d(int64_t m, int64_t n, const double* src, int64_t rs, int64_t cs, double* dst,
+                   int64_t ld, hipStream_t s);
+// dst(i,j) at dst[i*rs + j*cs] = src[i*ld + j]
+int launch_dscatter2d(int64_t m, int64_t n, const double* src, int64_t ld, double* dst, int64_t rs,
+                      int64_t cs, hipStream_t s);
+int launch_dscale2d(int64_t m, int64_t n, double beta, double* c, int64_t rs, int64_t cs,
+                    int tri, hipStream_t s);
+
+// LU pieces (lu_kernels.hip).  The panel kernel writes relative pivots to piv[0..nb),
+// sets *info = info_base + i + 1 at the first zero pivot (if *info was 0) and emits the
+// list of moved rows as (dst, src) pairs relative to the panel's first row.
+template <typename T>
+int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
+                          int info_base, int* list_out, int* count_out, hipStream_t s);
+template <typename T>
+int launch_laswp_list(T* A, int64_t ld, int64_t row0, int64_t ncols, int64_t skip_lo,
+                      int64_t skip_hi, const int* list, const int* count, int max_count,
+                      hipStream_t s);
+int launch_perm_compose(int n, int npanels, int nb, const int* lists, const int* counts, int* perm,
+                        hipStream_t s);
+
+// triangular block solves (trsm.hip): op(T) X = B for one bs x bs triangle (bs <= 128)
+template <typename T>
+int launch_trsm_slab(bool lower, bool unit, int bs, int64_t ncols, const T* A, int64_t rs,
+                     int64_t cs, T* B, int64_t ldb, hipStream_t s);
+// op(A) X = B, A n x n row-major; lower/upper name the stored triangle of A
+template <typename T>
+int trsm_left(bool lower, bool unit, bool transA, int64_t n, int64_t nrhs, const T* A,
+              int64_t lda, T* B, int64_t ldb, hipStream_t s);
+
+// Cholesky pieces (chol_kernels.hip)
+template <typename T>
+int launch_potf2_ex(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s);
+template <typename T>
+int launch_trsm_rlt_panel(int nb, int64_t nrows, const T* L, int64_t ldl, T* B, int64_t ldb,
+                          hipStream_t s);
+int launch_dtranspose_inplace(int64_t n, double* a, int64_t lda, hipStream_t s);
+
+// pivots -> permutation (perm[i] = source row of row i after all interchanges)
+int launch_piv_to_perm(int64_t n, const int32_t* piv, int32_t* perm, hipStream_t s);
+template <typename T>
+int launch_gather_rows(int64_t n, int64_t ncols, const T* src, int64_t lds,
+                       const int32_t* perm, T* dst, int64_t ldd, bool inverse, hipStream_t s);
+
+// internal FP64 drivers
+int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* info_host);
+int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_t lda,
+                    const int32_t* dpiv, double* b, int64_t ldb);
+int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host);
+int dpotrs_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda, double* b,
+                    int64_t ldb);
+
+}  // namespace ddla

@@ -1,0 +1,138 @@
+// FP64 factorization drivers (host side): blocked PLU, solves, blocked Cholesky.
+//
+//   dgetrf_internal  PLUDecBlock (DDLA/Factorizations/LU.cs:289-342): per 128-column panel
+//                    panel factorization with pivoting, interchanges on the columns left
+//                    and right of the panel, A12 <- L11^-1 A12, A22 <- A22 - A21 A12.
+//   dgetrs_internal  PLUSolveInternel (LU.cs:431-465).
+//   dpotrf_internal  CholeskyLowerBlock (DDLA/Factorizations/Cholesky.cs:142-160); the upper
+//                    form factors the transposed storage with the same kernels.
+//   dpotrs_internal  CholeskySolveUncheck (Cholesky.cs:210-226).
+#include <algorithm>
+
+#include "ddla_internal.hpp"
+
+namespace ddla {
+
+namespace {
+constexpr int NBLK = 128;  // LU.PLUBlockSize (LU.cs:235), Cholesky.BlockSize (Cholesky.cs:125)
+
+int read_info(int32_t* dinfo, int32_t* info_host) {
+  DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
+  DDLA_HIP_CHECK(hipStreamSynchronize(stream()));
+  return 0;
+}
+}  // namespace
+
+// Workspace layout of the pivot slot: [info word][counts: npanels][lists: npanels*4*NBLK]
+int lu_bookkeeping(int64_t n, int32_t** dinfo, int** counts, int** lists) {
+  const int64_t np = (n + NBLK - 1) / NBLK;
+  void* ws = nullptr;
+  const size_t bytes = 64 + np * sizeof(int) + np * 4 * NBLK * sizeof(int) + (size_t)n * sizeof(int);
+  DDLA_RETURN_IF(workspace(WS_PIVOT, bytes, &ws));
+  char* p = static_cast<char*>(ws);
+  *dinfo = reinterpret_cast<int32_t*>(p);
+  *counts = reinterpret_cast<int*>(p + 64);
+  *lists = *counts + np;
+  return 0;
+}
+
+int* lu_perm_area(int64_t n) {
+  const int64_t np = (n + NBLK - 1) / NBLK;
+  char* p = static_cast<char*>(ctx().ws[WS_PIVOT]);
+  return reinterpret_cast<int*>(p + 64 + np * sizeof(int) + np * 4 * NBLK * sizeof(int));
+}
+
+int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* info_host) {
+  hipStream_t s = stream();
+  int32_t* dinfo;
+  int *counts, *lists;
+  DDLA_RETURN_IF(lu_bookkeeping(n, &dinfo, &counts, &lists));
+  DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
+  int k = 0;
+  for (int64_t j = 0; j < n; j += NBLK, ++k) {
+    const int b = (int)std::min<int64_t>(NBLK, n - j);
+    const int64_t m = n - j;
+    double* panel = a + j * lda + j;
+    int* list_k = lists + (size_t)k * 4 * NBLK;
+    DDLA_RETURN_IF(launch_getf2_panel_ex<double>(m, b, panel, lda, dpiv + j, dinfo, (int)j, list_k,
+                                                 counts + k, s));
+    // interchanges on columns [0, j) and [j+b, n) of rows j..n-1 (LU.cs:316-329)
+    DDLA_RETURN_IF(launch_laswp_list<double>(a + j * lda, lda, 0, n, j, j + b, list_k, counts + k,
+                                             2 * NBLK, s));
+    const int64_t rest = n - j - b;
+    if (rest > 0) {
+      // A12 = trilu(A11) \ A12 (LU.cs:332)
+      DDLA_RETURN_IF(launch_trsm_slab<double>(true, true, b, rest, panel, lda, 1, panel + b, lda, s));
+      // A22 -= A21 * A12 (LU.cs:334)
+      DDLA_RETURN_IF(launch_dgemm(rest, rest, b, -1.0, panel + b * lda, lda, true, panel + b, lda,
+                                  true, 1.0, panel + b * lda + b, lda, 0, s));
+    }
+  }
+  return read_info(dinfo, info_host);
+}
+
+int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_t lda,
+                    const int32_t* dpiv, double* b, int64_t ldb) {
+  hipStream_t s = stream();
+  if (n <= 0 || nrhs <= 0) return 0;
+  void* ws = nullptr;
+  const size_t perm_bytes = ((size_t)n * sizeof(int) + 255) / 256 * 256;
+  DDLA_RETURN_IF(workspace(WS_REFINE, perm_bytes + (size_t)n * nrhs * sizeof(double), &ws));
+  int* perm = static_cast<int*>(ws);
+  double* tmp = reinterpret_cast<double*>(static_cast<char*>(ws) + perm_bytes);
+  DDLA_RETURN_IF(launch_piv_to_perm(n, dpiv, perm, s));
+  if (!trans) {
+    // X = U^-1 L^-1 P B
+    DDLA_RETURN_IF(launch_gather_rows<double>(n, nrhs, b, ldb, perm, tmp, nrhs, false, s));
+    DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, tmp, nrhs, 1, b, ldb, s));
+    DDLA_RETURN_IF(trsm_left<double>(true, true, false, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(trsm_left<double>(false, false, false, n, nrhs, a, lda, b, ldb, s));
+  } else {
+    // A^T X = B: U^T V = B, L^T Y = V, X = P^-1 Y
+    DDLA_RETURN_IF(trsm_left<double>(false, false, true, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(trsm_left<double>(true, true, true, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(launch_gather_rows<double>(n, nrhs, b, ldb, perm, tmp, nrhs, true, s));
+    DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, tmp, nrhs, 1, b, ldb, s));
+  }
+  return 0;
+}
+
+int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host) {
+  hipStream_t s = stream();
+  int32_t* dinfo;
+  int *counts, *lists;
+  DDLA_RETURN_IF(lu_bookkeeping(1, &dinfo, &counts, &lists));
+  DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
+  if (!lower) DDLA_RETURN_IF(launch_dtranspose_inplace(n, a, lda, s));
+  for (int64_t j = 0; j < n; j += NBLK) {
+    const int b = (int)std::min<int64_t>(NBLK, n - j);
+    double* a11 = a + j * lda + j;
+    DDLA_RETURN_IF(launch_potf2_ex<double>(b, a11, lda, dinfo, (int)j, s));
+    const int64_t rest = n - j - b;
+    if (rest > 0) {
+      double* a21 = a11 + b * lda;
+      // A21 = A21 * L11^-T (Cholesky.cs:156-157)
+      DDLA_RETURN_IF(launch_trsm_rlt_panel<double>(b, rest, a11, lda, a21, lda, s));
+      // lower(A22) -= A21 * A21^T (Cholesky.cs:158)
+      DDLA_RETURN_IF(launch_dgemm(rest, rest, b, -1.0, a21, lda, true, a21, lda, false, 1.0,
+                                  a21 + b, lda, 1, s));
+    }
+  }
+  if (!lower) DDLA_RETURN_IF(launch_dtranspose_inplace(n, a, lda, s));
+  return read_info(dinfo, info_host);
+}
+
+int dpotrs_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda, double* b,
+                    int64_t ldb) {
+  hipStream_t s = stream();
+  if (lower) {
+    DDLA_RETURN_IF(trsm_left<double>(true, false, false, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(trsm_left<double>(true, false, true, n, nrhs, a, lda, b, ldb, s));
+  } else {
+    DDLA_RETURN_IF(trsm_left<double>(false, false, true, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(trsm_left<double>(false, false, false, n, nrhs, a, lda, b, ldb, s));
+  }
+  return 0;
+}
+
+}  // namespace ddla

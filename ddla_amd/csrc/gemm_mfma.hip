@@ -1,0 +1,403 @@
+// FP64 / FP32 GEMM on the CDNA4 matrix cores (v_mfma_f64_16x16x4_f64,
+// v_mfma_f32_16x16x4_f32).
+//
+// Replaces the managed BLIS-style GEMM (DDLA/BLAS/Managed/3m/GeMM.cs:14-73, packing
+// Packing.cs:30-101, macro-kernel MacroKernel.cs:24-97, micro-kernel
+// DDLA/Einsum/GEMMKernel.cs:36-240) and the BLIS dgemm it stands in for
+// (DDLA/BLAS/Native/3m.cs:12-24).  Design:
+//   * one workgroup (256 lanes = 4 wave64) computes a BM x BN tile of row-major C;
+//     the 4 waves form a 2 x 2 grid and each wave owns (BM/2) x (BN/2) = TM x TN MFMA
+//     16x16 tiles held in accumulator registers;
+//   * K advances in steps of 16: the next A/B slices are fetched from HBM into
+//     registers while the matrix cores work on the current slices in LDS (double
+//     buffered LDS, one barrier per step) -- this replaces the CPU path's explicit
+//     packing pass;
+//   * LDS rows are padded by two elements so the MFMA operand reads (16 lanes = 16 rows
+//     of A at one k, or 16 columns of B at one k) hit distinct banks;
+//   * workgroup ids are remapped so every XCD (own L2) walks a contiguous band of a
+//     grouped tile order, keeping the A row panels and B column panels it reuses in its
+//     own L2;
+//   * epilogue C = alpha*acc + beta*C in registers (beta == 0 never reads C, matching
+//     Managed/1m.cs:265-269), optional lower/upper triangle mask for SYRK.
+// The 16x16x4 FP64 and FP32 instructions share one operand layout: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16] and receives D[4*(l/16)+r][l%16], r = 0..3.
+#include "ddla_internal.hpp"
+
+namespace ddla {
+namespace {
+
+template <typename T>
+struct Vec;
+template <>
+struct Vec<double> {
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Vec<float> {
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T, int BM_, int BN_>
+struct Cfg {
+  typedef T E;
+  typedef typename Vec<T>::v2 v2;
+  typedef typename Vec<T>::v4 v4;
+  static constexpr int BM = BM_, BN = BN_, BK = 16;
+  static constexpr int WM = BM / 2, WN = BN / 2;
+  static constexpr int TM = WM / 16, TN = WN / 16;
+  static constexpr int PA = BK + 2;  // LDS row stride of A tile ([m][k])
+  static constexpr int PB = BN + 2;  // LDS row stride of B tile ([k][n])
+  static constexpr int ACH = BM * BK / 2 / 256;  // 2-element chunks per lane
+  static constexpr int BCH = BK * BN / 2 / 256;
+  static constexpr int AS = BM * PA, BS = BK * PB;
+};
+
+template <class C, bool AK>
+__device__ __forceinline__ void load_a(typename C::v2 (&r)[C::ACH],
+                                       const typename C::E* __restrict__ A, int64_t lda, int M,
+                                       int K, int m0, int k0, bool vec) {
+  typedef typename C::E E;
+  typedef typename C::v2 v2;
+#pragma unroll
+  for (int q = 0; q < C::ACH; ++q) {
+    const int v = threadIdx.x + 256 * q;
+    int gi, gk;
+    if (AK) {
+      gi = m0 + v / (C::BK / 2);
+      gk = k0 + (v % (C::BK / 2)) * 2;
+      const E* p = A + (int64_t)gi * lda + gk;
+      if (vec && gi < M && gk + 1 < K) {
+        r[q] = *reinterpret_cast<const v2*>(p);
+      } else {
+        r[q].x = (gi < M && gk < K) ? p[0] : E(0);
+        r[q].y = (gi < M && gk + 1 < K) ? p[1] : E(0);
+      }
+    } else {
+      gk = k0 + v / (C::BM / 2);
+      gi = m0 + (v % (C::BM / 2)) * 2;
+      const E* p = A + gi + (int64_t)gk * lda;
+      if (vec && gk < K && gi + 1 < M) {
+        r[q] = *reinterpret_cast<const v2*>(p);
+      } else {
+        r[q].x = (gk < K && gi < M) ? p[0] : E(0);
+        r[q].y = (gk < K && gi + 1 < M) ? p[1] : E(0);
+      }
+    }
+  }
+}
+
+template <class C, bool AK>
+__device__ __forceinline__ void store_a(const typename C::v2 (&r)[C::ACH], typename C::E* As) {
+#pragma unroll
+  for (int q = 0; q < C::ACH; ++q) {
+    const int v = threadIdx.x + 256 * q;
+    if (AK) {
+      const int row = v / (C::BK / 2), kc = (v % (C::BK / 2)) * 2;
+      *reinterpret_cast<typename C::v2*>(As + row * C::PA + kc) = r[q];
+    } else {
+      const int k = v / (C::BM / 2), row = (v % (C::BM / 2)) * 2;
+      As[row * C::PA + k] = r[q].x;
+      As[(row + 1) * C::PA + k] = r[q].y;
+    }
+  }
+}
+
+template <class C, bool BNC>
+__device__ __forceinline__ void load_b(typename C::v2 (&r)[C::BCH],
+                                       const typename C::E* __restrict__ B, int64_t ldb, int N,
+                                       int K, int n0, int k0, bool vec) {
+  typedef typename C::E E;
+  typedef typename C::v2 v2;
+#pragma unroll
+  for (int q = 0; q < C::BCH; ++q) {
+    const int v = threadIdx.x + 256 * q;
+    int gj, gk;
+    if (BNC) {
+      gk = k0 + v / (C::BN / 2);
+      gj = n0 + (v % (C::BN / 2)) * 2;
+      const E* p = B + (int64_t)gk * ldb + gj;
+      if (vec && gk < K && gj + 1 < N) {
+        r[q] = *reinterpret_cast<const v2*>(p);
+      } else {
+        r[q].x = (gk < K && gj < N) ? p[0] : E(0);
+        r[q].y = (gk < K && gj + 1 < N) ? p[1] : E(0);
+      }
+    } else {
+      gj = n0 + v / (C::BK / 2);
+      gk = k0 + (v % (C::BK / 2)) * 2;
+      const E* p = B + gk + (int64_t)gj * ldb;
+      if (vec && gj < N && gk + 1 < K) {
+        r[q] = *reinterpret_cast<const v2*>(p);
+      } else {
+        r[q].x = (gj < N && gk < K) ? p[0] : E(0);
+        r[q].y = (gj < N && gk + 1 < K) ? p[1] : E(0);
+      }
+    }
+  }
+}
+
+template <class C, bool BNC>
+__device__ __forceinline__ void store_b(const typename C::v2 (&r)[C::BCH], typename C::E* Bs) {
+#pragma unroll
+  for (int q = 0; q < C::BCH; ++q) {
+    const int v = threadIdx.x + 256 * q;
+    if (BNC) {
+      const int k = v / (C::BN / 2), col = (v % (C::BN / 2)) * 2;
+      *reinterpret_cast<typename C::v2*>(Bs + k * C::PB + col) = r[q];
+    } else {
+      const int col = v / (C::BK / 2), kc = (v % (C::BK / 2)) * 2;
+      Bs[kc * C::PB + col] = r[q].x;
+      Bs[(kc + 1) * C::PB + col] = r[q].y;
+    }
+  }
+}
+
+}  // namespace
+
+namespace {
+
+template <class C, bool AK, bool BNC, bool BETA0, int TRI>
+__global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, typename C::E alpha,
+                                                      const typename C::E* __restrict__ A,
+                                                      int64_t lda,
+                                                      const typename C::E* __restrict__ B,
+                                                      int64_t ldb, typename C::E beta,
+                                                      typename C::E* __restrict__ Cm, int64_t ldc,
+                                                      int tiles_m, int tiles_n, bool veca,
+                                                      bool vecb) {
+  typedef typename C::E E;
+  typedef typename C::v4 v4;
+  __shared__ __attribute__((aligned(16))) E As[2][C::AS];
+  __shared__ __attribute__((aligned(16))) E Bs[2][C::BS];
+
+  int tmi, tni;
+  gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
+  const int m0 = tmi * C::BM, n0 = tni * C::BN;
+  if (TRI == 1 && n0 > m0 + C::BM - 1) return;  // tile strictly above the diagonal
+  if (TRI == 2 && m0 > n0 + C::BN - 1) return;  // tile strictly below the diagonal
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  v4 acc[C::TM][C::TN];
+#pragma unroll
+  for (int i = 0; i < C::TM; ++i)
+#pragma unroll
+    for (int j = 0; j < C::TN; ++j) acc[i][j] = v4{E(0), E(0), E(0), E(0)};
+
+  typename C::v2 ra[C::ACH], rb[C::BCH];
+  const int nk = (K + C::BK - 1) / C::BK;
+  load_a<C, AK>(ra, A, lda, M, K, m0, 0, veca);
+  load_b<C, BNC>(rb, B, ldb, N, K, n0, 0, vecb);
+  store_a<C, AK>(ra, As[0]);
+  store_b<C, BNC>(rb, Bs[0]);
+  __syncthreads();
+
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) {
+      load_a<C, AK>(ra, A, lda, M, K, m0, (kt + 1) * C::BK, veca);
+      load_b<C, BNC>(rb, B, ldb, N, K, n0, (kt + 1) * C::BK, vecb);
+    }
+    const E* as = As[cur] + (wm * C::WM + l16) * C::PA + lk;
+    const E* bs = Bs[cur] + lk * C::PB + wn * C::WN + l16;
+#pragma unroll
+    for (int kk = 0; kk < C::BK / 4; ++kk) {
+      E a[C::TM], b[C::TN];
+#pragma unroll
+      for (int i = 0; i < C::TM; ++i) a[i] = as[i * 16 * C::PA + kk * 4];
+#pragma unroll
+      for (int j = 0; j < C::TN; ++j) b[j] = bs[kk * 4 * C::PB + j * 16];
+#pragma unroll
+      for (int i = 0; i < C::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < C::TN; ++j) acc[i][j] = Vec<E>::mfma(a[i], b[j], acc[i][j]);
+    }
+    if (kt + 1 < nk) {
+      store_a<C, AK>(ra, As[cur ^ 1]);
+      store_b<C, BNC>(rb, Bs[cur ^ 1]);
+    }
+    __syncthreads();
+  }
+
+  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of every 16x16 tile
+#pragma unroll
+  for (int i = 0; i < C::TM; ++i)
+#pragma unroll
+    for (int j = 0; j < C::TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = m0 + wm * C::WM + i * 16 + lk * 4 + r;
+        const int gj = n0 + wn * C::WN + j * 16 + l16;
+        bool ok = gi < M && gj < N;
+        if (TRI == 1) ok = ok && gj <= gi;
+        if (TRI == 2) ok = ok && gj >= gi;
+        if (ok) {
+          E* p = Cm + (int64_t)gi * ldc + gj;
+          const E v = acc[i][j][r];
+          *p = BETA0 ? alpha * v : fma(alpha, v, beta * *p);
+        }
+      }
+}
+
+template <class C, bool AK, bool BNC>
+int launch_cfg(int M, int N, int K, typename C::E alpha, const typename C::E* A, int64_t lda,
+               const typename C::E* B, int64_t ldb, typename C::E beta, typename C::E* Cm,
+               int64_t ldc, int tri, hipStream_t s) {
+  const int tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
+  const int64_t tiles = (int64_t)tm * tn;
+  if (tiles > INT32_MAX) {
+    set_error("gemm: problem too large (%lld tiles)", (long long)tiles);
+    return DDLA_ERR_INVALID_ARG;
+  }
+  const bool veca = (reinterpret_cast<uintptr_t>(A) % (2 * sizeof(typename C::E)) == 0) && (lda % 2 == 0);
+  const bool vecb = (reinterpret_cast<uintptr_t>(B) % (2 * sizeof(typename C::E)) == 0) && (ldb % 2 == 0);
+  dim3 grid(static_cast<unsigned>(tiles)), block(256);
+#define DDLA_GEMM_GO(B0, T)                                                                 \
+  hipLaunchKernelGGL((gemm_kernel<C, AK, BNC, B0, T>), grid, block, 0, s, M, N, K, alpha, A, \
+                     lda, B, ldb, beta, Cm, ldc, tm, tn, veca, vecb)
+  const bool b0 = (beta == typename C::E(0));
+  if (tri == 0) {
+    if (b0) DDLA_GEMM_GO(true, 0); else DDLA_GEMM_GO(false, 0);
+  } else if (tri == 1) {
+    if (b0) DDLA_GEMM_GO(true, 1); else DDLA_GEMM_GO(false, 1);
+  } else {
+    if (b0) DDLA_GEMM_GO(true, 2); else DDLA_GEMM_GO(false, 2);
+  }
+#undef DDLA_GEMM_GO
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <class C>
+int launch_layout(int M, int N, int K, typename C::E alpha, const typename C::E* A, int64_t lda,
+                  bool ak, const typename C::E* B, int64_t ldb, bool bn, typename C::E beta,
+                  typename C::E* Cm, int64_t ldc, int tri, hipStream_t s) {
+  if (ak && bn) return launch_cfg<C, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s);
+  if (ak && !bn) return launch_cfg<C, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s);
+  if (!ak && bn) return launch_cfg<C, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s);
+  return launch_cfg<C, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s);
+}
+
+// C = beta*C over a strided m x n view (tri: 0 all, 1 lower, 2 upper); beta == 0 stores 0.
+template <typename T>
+__global__ void scale2d_kernel(int64_t m, int64_t n, T beta, T* c, int64_t rs, int64_t cs,
+                               int tri) {
+  const int64_t total = m * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / n, j = t % n;
+    if (tri == 1 && j > i) continue;
+    if (tri == 2 && j < i) continue;
+    T* p = c + i * rs + j * cs;
+    *p = (beta == T(0)) ? T(0) : beta * *p;
+  }
+}
+
+__global__ void dcopy2d_kernel(int64_t m, int64_t n, const double* __restrict__ src, int64_t rs,
+                               int64_t cs, double* __restrict__ dst, int64_t ld) {
+  const int64_t total = m * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / n, j = t % n;
+    dst[i * ld + j] = src[i * rs + j * cs];
+  }
+}
+
+__global__ void dscatter2d_kernel(int64_t m, int64_t n, const double* __restrict__ src, int64_t ld,
+                                  double* __restrict__ dst, int64_t rs, int64_t cs) {
+  const int64_t total = m * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / n, j = t % n;
+    dst[i * rs + j * cs] = src[i * ld + j];
+  }
+}
+
+int grid_for(int64_t total) {
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return static_cast<int>(g);
+}
+
+template <typename T>
+int gemm_any(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, bool a_kcontig,
+             const T* B, int64_t ldb, bool b_ncontig, T beta, T* C, int64_t ldc, int tri,
+             hipStream_t s) {
+  if (m <= 0 || n <= 0) return 0;
+  if (k <= 0 || alpha == T(0)) {
+    if (beta == T(1)) return 0;
+    hipLaunchKernelGGL(scale2d_kernel<T>, dim3(grid_for(m * n)), dim3(256), 0, s, m, n, beta, C,
+                       ldc, (int64_t)1, tri);
+    DDLA_HIP_CHECK(hipGetLastError());
+    return 0;
+  }
+  if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) {
+    set_error("gemm: dimension exceeds 2^31-1");
+    return DDLA_ERR_INVALID_ARG;
+  }
+  const int M = (int)m, N = (int)n, K = (int)k;
+  const int64_t big_tiles = (int64_t)ceil_div(M, 128) * ceil_div(N, 128);
+  const int cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
+  const double mn = (tri ? 0.5 : 1.0) * (double)m * n;
+  ProfScope prof(sizeof(T) == 8 ? "dgemm_mfma_f64" : "sgemm_mfma_f32", 2.0 * mn * k,
+                 sizeof(T) * ((double)m * k + (double)k * n + (beta == T(0) ? 1.0 : 2.0) * mn), s);
+  if (big_tiles >= 2 * cus)
+    return launch_layout<Cfg<T, 128, 128>>(M, N, K, alpha, A, lda, a_kcontig, B, ldb, b_ncontig,
+                                           beta, C, ldc, tri, s);
+  return launch_layout<Cfg<T, 64, 64>>(M, N, K, alpha, A, lda, a_kcontig, B, ldb, b_ncontig, beta,
+                                       C, ldc, tri, s);
+}
+
+}  // namespace
+
+int launch_dscale2d(int64_t m, int64_t n, double beta, double* c, int64_t rs, int64_t cs, int tri,
+                    hipStream_t s) {
+  if (m <= 0 || n <= 0 || beta == 1.0) return 0;
+  hipLaunchKernelGGL(scale2d_kernel<double>, dim3(grid_for(m * n)), dim3(256), 0, s, m, n, beta, c,
+                     rs, cs, tri);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_dcopy2d(int64_t m, int64_t n, const double* src, int64_t rs, int64_t cs, double* dst,
+                   int64_t ld, hipStream_t s) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(dcopy2d_kernel, dim3(grid_for(m * n)), dim3(256), 0, s, m, n, src, rs, cs,
+                     dst, ld);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_dscatter2d(int64_t m, int64_t n, const double* src, int64_t ld, double* dst, int64_t rs,
+                      int64_t cs, hipStream_t s) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(dscatter2d_kernel, dim3(grid_for(m * n)), dim3(256), 0, s, m, n, src, ld, dst,
+                     rs, cs);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_dgemm(int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                 bool a_kcontig, const double* B, int64_t ldb, bool b_ncontig, double beta,
+                 double* C, int64_t ldc, int tri, hipStream_t s) {
+  return gemm_any<double>(m, n, k, alpha, A, lda, a_kcontig, B, ldb, b_ncontig, beta, C, ldc, tri, s);
+}
+
+int launch_sgemm(int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                 bool a_kcontig, const float* B, int64_t ldb, bool b_ncontig, float beta, float* C,
+                 int64_t ldc, int tri, hipStream_t s) {
+  return gemm_any<float>(m, n, k, alpha, A, lda, a_kcontig, B, ldb, b_ncontig, beta, C, ldc, tri, s);
+}
+
+}  // namespace ddla

@@ -1,0 +1,650 @@
+// Mixed-precision LU and Cholesky with iterative refinement to FP64 accuracy.
+//
+// No counterpart in the reference (DDLA is FP64-only).  The factorizations follow the same
+// blocked algorithms as the FP64 path (DDLA/Factorizations/LU.cs:289-342,
+// DDLA/Factorizations/Cholesky.cs:142-160) on an FP32 working copy:
+//   * panel / diagonal block and TRSM in FP32 (partial pivoting, same kernels as FP64);
+//   * the trailing update C <- C - A B runs on the FP16 or BF16 matrix cores
+//     (v_mfma_f32_16x16x32_{f16,bf16}) with FP32 accumulation; operands are rounded
+//     while they are staged from HBM into LDS, after multiplication by a per-128-block
+//     power-of-two scale (max |x| of the block mapped to (2^13, 2^14], so FP16 neither
+//     overflows nor loses small entries to subnormals); the epilogue divides the scales
+//     out exactly;
+//   * refinement: X += (LU)^-1 P R with R = B - A X in FP64 (GEMM on the FP64 matrix
+//     cores), stopping when max_j ||r_j||_inf / (||A||_inf ||x_j||_inf) <= tol.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "ddla_internal.hpp"
+
+namespace ddla {
+
+int lu_bookkeeping(int64_t n, int32_t** dinfo, int** counts, int** lists);
+
+namespace {
+
+constexpr int NBLK = 128;
+constexpr int HB = 128;  // scale block (rows of A / columns of B) = GEMM tile
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <bool BF16>
+struct LowP;
+template <>
+struct LowP<false> {
+  typedef half8 v8;
+  typedef _Float16 e;
+  static __device__ __forceinline__ f4 mfma(v8 a, v8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct LowP<true> {
+  typedef bf16x8 v8;
+  typedef __bf16 e;
+  static __device__ __forceinline__ f4 mfma(v8 a, v8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+// power-of-two scale of a block: 1 for bf16 (FP32 exponent range) or an empty block
+__device__ __forceinline__ float block_scale(float amax, bool bf16) {
+  if (bf16 || !(amax > 0.0f) || !isfinite(amax)) return 1.0f;
+  const int e = (int)ceilf(log2f(amax));
+  return exp2f((float)(14 - e));
+}
+
+// scales[b] for row blocks (rows [b*HB, b*HB+HB) of an m x k row-major matrix)
+__global__ __launch_bounds__(256) void rowblock_scale_kernel(int64_t m, int64_t k,
+                                                             const float* __restrict__ X,
+                                                             int64_t ld, float* __restrict__ out,
+                                                             bool bf16) {
+  __shared__ float red[4];
+  const int64_t r0 = (int64_t)blockIdx.x * HB;
+  const int64_t rows = min<int64_t>(HB, m - r0);
+  float mx = 0.0f;
+  for (int64_t t = threadIdx.x; t < rows * k; t += 256) {
+    const int64_t r = t / k, c = t % k;
+    mx = fmaxf(mx, fabsf(X[(r0 + r) * ld + c]));
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    out[blockIdx.x] = block_scale(mx, bf16);
+  }
+}
+
+// scales[b] for column blocks (columns [b*HB, b*HB+HB) of a k x n row-major matrix)
+__global__ __launch_bounds__(256) void colblock_scale_kernel(int64_t k, int64_t n,
+                                                             const float* __restrict__ X,
+                                                             int64_t ld, float* __restrict__ out,
+                                                             bool bf16) {
+  __shared__ float red[4];
+  const int64_t c0 = (int64_t)blockIdx.x * HB;
+  const int64_t cols = min<int64_t>(HB, n - c0);
+  float mx = 0.0f;
+  for (int64_t t = threadIdx.x; t < cols * k; t += 256) {
+    const int64_t r = t / cols, c = t % cols;
+    mx = fmaxf(mx, fabsf(X[r * ld + c0 + c]));
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    out[blockIdx.x] = block_scale(mx, bf16);
+  }
+}
+
+// C (FP32, m x n) = C + alpha * (A * B); A m x k FP32 (k contiguous); B k x n FP32,
+// n contiguous (B_KC = false) or k contiguous, B(k,j) at B[j*ldb + k] (B_KC = true).
+// 128 x 128 tile per workgroup, K step 32, operands rounded to FP16/BF16 in LDS.
+template <bool BF16, bool B_KC, int TRI>
+__global__ __launch_bounds__(256, 2) void hgemm_kernel(int M, int N, int K, float alpha,
+                                                       const float* __restrict__ A, int64_t lda,
+                                                       const float* __restrict__ B, int64_t ldb,
+                                                       float* __restrict__ C, int64_t ldc,
+                                                       const float* __restrict__ sa,
+                                                       const float* __restrict__ sb, int tiles_m,
+                                                       int tiles_n) {
+  typedef LowP<BF16> LP;
+  typedef typename LP::v8 v8;
+  typedef typename LP::e E;
+  constexpr int BM = 128, BN = 128, BK = 32, PK = BK + 8;  // padded rows: conflict-free b128 reads
+  __shared__ __attribute__((aligned(16))) E As[2][BM * PK];
+  __shared__ __attribute__((aligned(16))) E Bs[2][BN * PK];
+
+  int tmi, tni;
+  gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
+  const int m0 = tmi * BM, n0 = tni * BN;
+  if (TRI == 1 && n0 > m0 + BM - 1) return;
+  const float scA = sa[tmi], scB = sb[tni];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // staging: 2 items per lane for A (row, k-group of 8) and for B (column, k-group of 8)
+  float ra[2][8], rb[2][8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int v = tid + 256 * q;
+      {
+        const int row = v >> 2, kg = (v & 3) * 8;
+        const int gi = m0 + row;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int gk = k0 + kg + e;
+          ra[q][e] = (gi < M && gk < K) ? A[(int64_t)gi * lda + gk] : 0.f;
+        }
+      }
+      if (B_KC) {
+        const int col = v >> 2, kg = (v & 3) * 8;
+        const int gj = n0 + col;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int gk = k0 + kg + e;
+          rb[q][e] = (gj < N && gk < K) ? B[(int64_t)gj * ldb + gk] : 0.f;
+        }
+      } else {
+        const int col = v & 127, kg = (v >> 7) * 8;
+        const int gj = n0 + col;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int gk = k0 + kg + e;
+          rb[q][e] = (gj < N && gk < K) ? B[(int64_t)gk * ldb + gj] : 0.f;
+        }
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int v = tid + 256 * q;
+      v8 pa, pb;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        pa[e] = static_cast<E>(ra[q][e] * scA);
+        pb[e] = static_cast<E>(rb[q][e] * scB);
+      }
+      const int arow = v >> 2, akg = (v & 3) * 8;
+      *reinterpret_cast<v8*>(&As[buf][arow * PK + akg]) = pa;
+      int bcol, bkg;
+      if (B_KC) {
+        bcol = v >> 2;
+        bkg = (v & 3) * 8;
+      } else {
+        bcol = v & 127;
+        bkg = (v >> 7) * 8;
+      }
+      *reinterpret_cast<v8*>(&Bs[buf][bcol * PK + bkg]) = pb;
+    }
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    v8 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      a[i] = *reinterpret_cast<const v8*>(&As[cur][(wm * 64 + i * 16 + l16) * PK + lk * 8]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      b[j] = *reinterpret_cast<const v8*>(&Bs[cur][(wn * 64 + j * 16 + l16) * PK + lk * 8]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = LP::mfma(a[i], b[j], acc[i][j]);
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+  const float f = alpha / (scA * scB);  // exact: the scales are powers of two
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = m0 + wm * 64 + i * 16 + lk * 4 + r;
+        const int gj = n0 + wn * 64 + j * 16 + l16;
+        bool ok = gi < M && gj < N;
+        if (TRI == 1) ok = ok && gj <= gi;
+        if (ok) {
+          float* p = C + (int64_t)gi * ldc + gj;
+          *p = fmaf(f, acc[i][j][r], *p);
+        }
+      }
+}
+
+__global__ void d2s_kernel(int64_t m, int64_t n, const double* __restrict__ src, int64_t lds,
+                           float* __restrict__ dst, int64_t ldd) {
+  const int64_t total = m * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / n, j = t % n;
+    dst[i * ldd + j] = static_cast<float>(src[i * lds + j]);
+  }
+}
+
+// dst (float, n x nrhs contiguous) = (float) src[perm[i]]  (perm == nullptr: identity)
+__global__ void gather_d2s_kernel(int64_t n, int64_t nrhs, const double* __restrict__ src,
+                                  int64_t lds, const int* __restrict__ perm,
+                                  float* __restrict__ dst) {
+  const int64_t total = n * nrhs;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / nrhs, j = t % nrhs;
+    const int64_t si = perm ? perm[i] : i;
+    dst[t] = static_cast<float>(src[si * lds + j]);
+  }
+}
+
+// X += (double) D ; R = B
+__global__ void update_x_kernel(int64_t n, int64_t nrhs, const float* __restrict__ D,
+                                double* __restrict__ X, int64_t ldx, const double* __restrict__ B,
+                                int64_t ldb, double* __restrict__ R) {
+  const int64_t total = n * nrhs;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / nrhs, j = t % nrhs;
+    X[i * ldx + j] += static_cast<double>(D[t]);
+    R[t] = B[i * ldb + j];
+  }
+}
+
+// out[i] = sum_j |A[i][j]| (one wave per row)
+__global__ __launch_bounds__(256) void row_abs_sum_kernel(int64_t n, const double* __restrict__ A,
+                                                          int64_t lda, double* __restrict__ out) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  const double* a = A + row * lda;
+  for (int64_t j = lane; j < n; j += 64) s += fabs(a[j]);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) out[row] = s;
+}
+
+// out[0] = max_i v[i] over n values (single workgroup)
+__global__ __launch_bounds__(1024) void max_kernel(int64_t n, const double* __restrict__ v,
+                                                   double* __restrict__ out) {
+  __shared__ double red[16];
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, v[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; ++w) m = fmax(m, red[w]);
+    *out = m;
+  }
+}
+
+// per column j of the n x nrhs matrices: out[2j] = ||R_j||_inf, out[2j+1] = ||X_j||_inf
+__global__ __launch_bounds__(256) void col_inf_norms_kernel(int64_t n, int64_t nrhs,
+                                                            const double* __restrict__ R,
+                                                            const double* __restrict__ X,
+                                                            int64_t ldx,
+                                                            unsigned long long* __restrict__ out) {
+  // non-negative doubles order like their bit patterns: atomicMax on the bits
+  const int64_t total = n * nrhs;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / nrhs, j = t % nrhs;
+    double r = fabs(R[t]), x = fabs(X[i * ldx + j]);
+    if (r != r) r = INFINITY;
+    if (x != x) x = INFINITY;
+    atomicMax(out + 2 * j, __double_as_longlong(r));
+    atomicMax(out + 2 * j + 1, __double_as_longlong(x));
+  }
+}
+
+__global__ void zero_u64_kernel(int64_t n, unsigned long long* p) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+int grid_for(int64_t total) {
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return static_cast<int>(g);
+}
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  Timer() {
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+  }
+  ~Timer() {
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+  }
+  void start(hipStream_t s) { hipEventRecord(a, s); }
+  double stop_ms(hipStream_t s) {
+    hipEventRecord(b, s);
+    hipEventSynchronize(b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+  }
+};
+
+}  // namespace
+
+// C += alpha * A * B with FP16/BF16 operands and FP32 accumulation (see hgemm_kernel).
+int launch_hgemm(bool bf16, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k, float alpha,
+                 const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                 float* scale_ws, hipStream_t s) {
+  if (m <= 0 || n <= 0 || k <= 0) return 0;
+  const int tm = ceil_div(m, 128), tn = ceil_div(n, 128);
+  float* sa = scale_ws;
+  float* sb = scale_ws + tm;
+  hipLaunchKernelGGL(rowblock_scale_kernel, dim3(tm), dim3(256), 0, s, m, k, A, lda, sa, bf16);
+  if (b_kcontig)
+    hipLaunchKernelGGL(rowblock_scale_kernel, dim3(tn), dim3(256), 0, s, n, k, B, ldb, sb, bf16);
+  else
+    hipLaunchKernelGGL(colblock_scale_kernel, dim3(tn), dim3(256), 0, s, k, n, B, ldb, sb, bf16);
+  DDLA_HIP_CHECK(hipGetLastError());
+  dim3 grid((unsigned)((int64_t)tm * tn)), block(256);
+  const int M = (int)m, N = (int)n, K = (int)k;
+  const double mn = (tri ? 0.5 : 1.0) * (double)m * n;
+  ProfScope prof(bf16 ? "hgemm_mfma_bf16" : "hgemm_mfma_f16", 2.0 * mn * k,
+                 4.0 * ((double)m * k + (double)k * n + 2.0 * mn), s);
+#define HG(BF, KC, T)                                                                        \
+  hipLaunchKernelGGL((hgemm_kernel<BF, KC, T>), grid, block, 0, s, M, N, K, alpha, A, lda, B, \
+                     ldb, C, ldc, sa, sb, tm, tn)
+  if (bf16) {
+    if (b_kcontig) {
+      if (tri) HG(true, true, 1); else HG(true, true, 0);
+    } else {
+      if (tri) HG(true, false, 1); else HG(true, false, 0);
+    }
+  } else {
+    if (b_kcontig) {
+      if (tri) HG(false, true, 1); else HG(false, true, 0);
+    } else {
+      if (tri) HG(false, false, 1); else HG(false, false, 0);
+    }
+  }
+#undef HG
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+namespace {
+
+// Workspace (WS_MIXED): FP32 working matrix n x n
+// Workspace (WS_MIXED2): scales, perm, pivots, D (n x nrhs float), R (n x nrhs double),
+//                        row sums, norm words
+struct MixedWs {
+  float* W;
+  float* scales;
+  int* perm;
+  int32_t* piv;
+  float* D;
+  double* R;
+  double* rowsum;
+  double* normA;
+  unsigned long long* norms;
+};
+
+int mixed_ws(int64_t n, int64_t nrhs, MixedWs* w) {
+  void* p1 = nullptr;
+  DDLA_RETURN_IF(workspace(WS_MIXED, (size_t)n * n * sizeof(float), &p1));
+  w->W = static_cast<float*>(p1);
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const int64_t nb = (n + 127) / 128 + 1;
+  const size_t s_scales = al(2 * nb * sizeof(float));
+  const size_t s_perm = al(n * sizeof(int));
+  const size_t s_piv = al(n * sizeof(int32_t));
+  const size_t s_D = al((size_t)n * nrhs * sizeof(float));
+  const size_t s_R = al((size_t)n * nrhs * sizeof(double));
+  const size_t s_rs = al(n * sizeof(double));
+  const size_t s_na = al(sizeof(double));
+  const size_t s_no = al(2 * nrhs * sizeof(unsigned long long));
+  void* p2 = nullptr;
+  DDLA_RETURN_IF(workspace(WS_MIXED2, s_scales + s_perm + s_piv + s_D + s_R + s_rs + s_na + s_no, &p2));
+  char* c = static_cast<char*>(p2);
+  w->scales = reinterpret_cast<float*>(c); c += s_scales;
+  w->perm = reinterpret_cast<int*>(c); c += s_perm;
+  w->piv = reinterpret_cast<int32_t*>(c); c += s_piv;
+  w->D = reinterpret_cast<float*>(c); c += s_D;
+  w->R = reinterpret_cast<double*>(c); c += s_R;
+  w->rowsum = reinterpret_cast<double*>(c); c += s_rs;
+  w->normA = reinterpret_cast<double*>(c); c += s_na;
+  w->norms = reinterpret_cast<unsigned long long*>(c);
+  return 0;
+}
+
+// FP32 LU with partial pivoting; trailing updates in prec (FP16/BF16 matrix cores, or
+// FP32 matrix cores for DDLA_PREC_FP32).  Fills w.perm.  Returns info via *info_host.
+int sgetrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
+  hipStream_t s = stream();
+  int32_t* dinfo;
+  int *counts, *lists;
+  DDLA_RETURN_IF(lu_bookkeeping(n, &dinfo, &counts, &lists));
+  DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
+  float* a = w.W;
+  const int64_t lda = n;
+  int k = 0;
+  for (int64_t j = 0; j < n; j += NBLK, ++k) {
+    const int b = (int)std::min<int64_t>(NBLK, n - j);
+    float* panel = a + j * lda + j;
+    int* list_k = lists + (size_t)k * 4 * NBLK;
+    DDLA_RETURN_IF(launch_getf2_panel_ex<float>(n - j, b, panel, lda, w.piv + j, dinfo, (int)j,
+                                                list_k, counts + k, s));
+    DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, n, j, j + b, list_k, counts + k,
+                                            2 * NBLK, s));
+    const int64_t rest = n - j - b;
+    if (rest > 0) {
+      DDLA_RETURN_IF(launch_trsm_slab<float>(true, true, b, rest, panel, lda, 1, panel + b, lda, s));
+      if (prec == DDLA_PREC_FP32) {
+        DDLA_RETURN_IF(launch_sgemm(rest, rest, b, -1.f, panel + b * lda, lda, true, panel + b, lda,
+                                    true, 1.f, panel + b * lda + b, lda, 0, s));
+      } else {
+        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, false, 0, rest, rest, b, -1.f,
+                                    panel + b * lda, lda, panel + b, lda, panel + b * lda + b, lda,
+                                    w.scales, s));
+      }
+    }
+  }
+  DDLA_RETURN_IF(launch_perm_compose((int)n, k, NBLK, lists, counts, w.perm, s));
+  DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DDLA_HIP_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// FP32 lower Cholesky; trailing SYRK in prec.
+int spotrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
+  hipStream_t s = stream();
+  int32_t* dinfo;
+  int *counts, *lists;
+  DDLA_RETURN_IF(lu_bookkeeping(1, &dinfo, &counts, &lists));
+  DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
+  float* a = w.W;
+  const int64_t lda = n;
+  for (int64_t j = 0; j < n; j += NBLK) {
+    const int b = (int)std::min<int64_t>(NBLK, n - j);
+    float* a11 = a + j * lda + j;
+    DDLA_RETURN_IF(launch_potf2_ex<float>(b, a11, lda, dinfo, (int)j, s));
+    const int64_t rest = n - j - b;
+    if (rest > 0) {
+      float* a21 = a11 + b * lda;
+      DDLA_RETURN_IF(launch_trsm_rlt_panel<float>(b, rest, a11, lda, a21, lda, s));
+      if (prec == DDLA_PREC_FP32) {
+        DDLA_RETURN_IF(launch_sgemm(rest, rest, b, -1.f, a21, lda, true, a21, lda, false, 1.f,
+                                    a21 + b, lda, 1, s));
+      } else {
+        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, true, 1, rest, rest, b, -1.f, a21, lda,
+                                    a21, lda, a21 + b, lda, w.scales, s));
+      }
+    }
+  }
+  DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DDLA_HIP_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// ||A||_inf into w.normA (device) and returned on the host
+int norm_inf(int64_t n, const double* A, int64_t lda, MixedWs& w, double* host) {
+  hipStream_t s = stream();
+  hipLaunchKernelGGL(row_abs_sum_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, s, n, A, lda, w.rowsum);
+  hipLaunchKernelGGL(max_kernel, dim3(1), dim3(1024), 0, s, n, w.rowsum, w.normA);
+  DDLA_HIP_CHECK(hipGetLastError());
+  DDLA_HIP_CHECK(hipMemcpyAsync(host, w.normA, sizeof(double), hipMemcpyDeviceToHost, s));
+  DDLA_HIP_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Classic refinement. lu: solve with P, L (unit) and U; else L L^T.
+int refine(bool lu, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
+           int64_t ldb, double* X, int64_t ldx, MixedWs& w, double normA, double tol, int max_iters,
+           int* iters, double* berr_out) {
+  hipStream_t s = stream();
+  const int64_t tot = n * nrhs;
+  // X = 0, R = B
+  DDLA_RETURN_IF(launch_dscale2d(n, nrhs, 0.0, X, ldx, 1, 0, s));
+  DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, B, ldb, 1, w.R, nrhs, s));
+  std::vector<unsigned long long> nh(2 * nrhs);
+  double berr = INFINITY;
+  int it = 0;
+  for (; it < max_iters; ++it) {
+    // D = (LU)^-1 P R in FP32
+    hipLaunchKernelGGL(gather_d2s_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.R,
+                       nrhs, lu ? w.perm : nullptr, w.D);
+    DDLA_HIP_CHECK(hipGetLastError());
+    if (lu) {
+      DDLA_RETURN_IF(trsm_left<float>(true, true, false, n, nrhs, w.W, n, w.D, nrhs, s));
+      DDLA_RETURN_IF(trsm_left<float>(false, false, false, n, nrhs, w.W, n, w.D, nrhs, s));
+    } else {
+      DDLA_RETURN_IF(trsm_left<float>(true, false, false, n, nrhs, w.W, n, w.D, nrhs, s));
+      DDLA_RETURN_IF(trsm_left<float>(true, false, true, n, nrhs, w.W, n, w.D, nrhs, s));
+    }
+    // X += D (FP64); R = B - A X (FP64)
+    hipLaunchKernelGGL(update_x_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.D, X, ldx,
+                       B, ldb, w.R);
+    DDLA_HIP_CHECK(hipGetLastError());
+    DDLA_RETURN_IF(launch_dgemm(n, nrhs, n, -1.0, A, lda, true, X, ldx, true, 1.0, w.R, nrhs, 0, s));
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, s, 2 * nrhs, w.norms);
+    hipLaunchKernelGGL(col_inf_norms_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.R, X,
+                       ldx, w.norms);
+    DDLA_HIP_CHECK(hipGetLastError());
+    DDLA_HIP_CHECK(hipMemcpyAsync(nh.data(), w.norms, nh.size() * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
+    DDLA_HIP_CHECK(hipStreamSynchronize(s));
+    double worst = 0.0;
+    for (int64_t j = 0; j < nrhs; ++j) {
+      const double r = __builtin_bit_cast(double, nh[2 * j]);
+      const double x = __builtin_bit_cast(double, nh[2 * j + 1]);
+      const double e = (x > 0.0 && normA > 0.0) ? r / (normA * x) : (r == 0.0 ? 0.0 : INFINITY);
+      worst = std::max(worst, e);
+    }
+    berr = worst;
+    if (!(berr > tol)) {
+      ++it;
+      break;
+    }
+  }
+  *iters = it;
+  *berr_out = berr;
+  return 0;
+}
+
+int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const double* b,
+         int64_t ldb, double* x, int64_t ldx, const ddla_mpir_opts* opts, ddla_mpir_info* info) {
+  hipStream_t s = stream();
+  ddla_mpir_info out{};
+  int prec = opts ? opts->factor_prec : DDLA_PREC_FP16;
+  const int max_iters = (opts && opts->max_iters > 0) ? opts->max_iters : 50;
+  const double tol = (opts && opts->tol > 0) ? opts->tol : 1e-14;
+  if (prec != DDLA_PREC_FP16 && prec != DDLA_PREC_BF16 && prec != DDLA_PREC_FP32) {
+    set_error("factor precision %d not supported by the mixed solver", prec);
+    return DDLA_ERR_UNSUPPORTED;
+  }
+  MixedWs w;
+  DDLA_RETURN_IF(mixed_ws(n, nrhs, &w));
+  double normA = 0.0;
+  DDLA_RETURN_IF(norm_inf(n, a, lda, w, &normA));
+  Timer t;
+  t.start(s);
+  int32_t finfo = 0;
+  hipLaunchKernelGGL(d2s_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, n, n, a, lda, w.W, n);
+  DDLA_HIP_CHECK(hipGetLastError());
+  if (lu) {
+    DDLA_RETURN_IF(sgetrf_mixed(n, prec, w, &finfo));
+  } else {
+    DDLA_RETURN_IF(spotrf_mixed(n, prec, w, &finfo));
+    if (finfo > 0 && prec != DDLA_PREC_FP32) {
+      // the low-precision factor lost positive definiteness: FP32 factor instead
+      out.fallback = 1;
+      hipLaunchKernelGGL(d2s_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, n, n, a, lda, w.W, n);
+      DDLA_RETURN_IF(spotrf_mixed(n, DDLA_PREC_FP32, w, &finfo));
+    }
+  }
+  out.factor_ms = t.stop_ms(s);
+  if (finfo > 0) {
+    out.status = finfo;
+    if (info) *info = out;
+    return finfo;
+  }
+  t.start(s);
+  int iters = 0;
+  double berr = INFINITY;
+  DDLA_RETURN_IF(refine(lu, n, nrhs, a, lda, b, ldb, x, ldx, w, normA, tol, max_iters, &iters, &berr));
+  out.refine_ms = t.stop_ms(s);
+  out.iters = iters;
+  out.backward_error = berr;
+  out.status = (berr <= tol) ? 0 : DDLA_ERR_NOT_CONVERGED;
+  if (info) *info = out;
+  if (out.status) set_error("refinement stopped at backward error %.3e after %d steps", berr, iters);
+  return out.status;
+}
+
+}  // namespace
+
+}  // namespace ddla
+
+using namespace ddla;
+
+extern "C" {
+
+int ddla_dgesv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
+                        const double* b, ddla_inc_t ldb, double* x, ddla_inc_t ldx,
+                        const ddla_mpir_opts* opts, ddla_mpir_info* info) {
+  DDLA_RETURN_IF(ensure_init());
+  DDLA_ARG_CHECK(n >= 0 && nrhs >= 0 && lda >= n && ldb >= nrhs && ldx >= nrhs,
+                 "bad gesv arguments");
+  if (n == 0 || nrhs == 0) return 0;
+  DDLA_ARG_CHECK(n <= 32768, "n = %lld above the supported 32768", (long long)n);
+  return mpir(true, n, nrhs, a, lda, b, ldb, x, ldx, opts, info);
+}
+
+int ddla_dposv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
+                        const double* b, ddla_inc_t ldb, double* x, ddla_inc_t ldx,
+                        const ddla_mpir_opts* opts, ddla_mpir_info* info) {
+  DDLA_RETURN_IF(ensure_init());
+  DDLA_ARG_CHECK(n >= 0 && nrhs >= 0 && lda >= n && ldb >= nrhs && ldx >= nrhs,
+                 "bad posv arguments");
+  if (n == 0 || nrhs == 0) return 0;
+  return mpir(false, n, nrhs, a, lda, b, ldb, x, ldx, opts, info);
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+/*
+ * ddla_native.h -- C ABI of the MI355X (gfx950) dense linear-algebra backend for DDLA.
+ *
+ * The library (ddla_amd/libddla_hip.so) replaces, for DDLA's dense hot path, the BLIS
+ * entry points that DDLA reaches through P/Invoke (`static partial class Source`,
+ * DDLA/BLAS/Native/utils.cs:8-17) and the managed blocked factorizations built on them
+ * (DDLA/Factorizations/LU.cs, DDLA/Factorizations/Cholesky.cs).
+ *
+ * Conventions (all taken from the P/Invoke declarations the entry points replace):
+ *   - dim_t / inc_t are 64-bit signed integers (DDLA/Alias.cs:1-2).
+ *   - A matrix argument is (pointer to element (0,0), row stride rs, column stride cs) in
+ *     elements, like MatrixView (DDLA/Core/MatrixView.cs:22-31,483-485).  Host-pointer
+ *     entry points accept any strides; device-pointer (`_dev`) entry points take row-major
+ *     device matrices with a leading dimension ld (element (i,j) at p[i*ld + j]).
+ *   - Flag values are DDLA's BLIS-valued enums (the files in DDLA/Misc/Flags).
+ *   - Scalars are passed by pointer where the replaced BLIS call does (`in double`).
+ *   - Every entry point returns int: 0 success, > 0 LAPACK-style info (1-based index of
+ *     the first zero pivot / non-positive Cholesky pivot), < 0 an error code below; the
+ *     message of the last error is available from ddla_last_error().
+ *   - LU pivots use DDLA's relative encoding: piv[i] = p - i where row p was swapped with
+ *     row i at step i, 0 = no swap (LU.cs:367-374,386-391).
+ *   - Device entry points are ordered on the library stream (ddla_set_stream) and return
+ *     after the work is queued, except those that return an info value or fill a result
+ *     struct, which synchronise the stream before returning.  Host-pointer entry points
+ *     are synchronous and keep no host pointer after they return.
+ *   - One library context per process; calls from several host threads must be
+ *     serialised by the caller.
+ */
+#ifndef DDLA_NATIVE_H
+#define DDLA_NATIVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int64_t ddla_dim_t; /* dim_t, DDLA/Alias.cs:1 */
+typedef int64_t ddla_inc_t; /* inc_t, DDLA/Alias.cs:2 */
+
+/* TransType (DDLA/Misc/Flags/TransType.cs:3-9) */
+#define DDLA_NO_TRANSPOSE 0x00
+#define DDLA_TRANSPOSE 0x08
+#define DDLA_CONJ_NO_TRANSPOSE 0x10
+#define DDLA_CONJ_TRANSPOSE 0x18
+/* UpLo (DDLA/Misc/Flags/UpLo.cs:3-9) */
+#define DDLA_ZEROS 0x00
+#define DDLA_LOWER 0xC0
+#define DDLA_UPPER 0x60
+#define DDLA_DENSE 0xE0
+/* DiagType (DDLA/Misc/Flags/DiagType.cs:3-7) */
+#define DDLA_NONUNIT_DIAG 0x000
+#define DDLA_UNIT_DIAG 0x100
+/* SideType (DDLA/Misc/Flags/SideType.cs:3-7) */
+#define DDLA_LEFT 0
+#define DDLA_RIGHT 1
+
+/* status codes */
+#define DDLA_SUCCESS 0
+#define DDLA_ERR_INVALID_ARG (-1)
+#define DDLA_ERR_HIP (-2)
+#define DDLA_ERR_NO_DEVICE (-3)
+#define DDLA_ERR_ALLOC (-4)
+#define DDLA_ERR_UNSUPPORTED (-5)
+#define DDLA_ERR_NOT_CONVERGED (-6)
+
+/* factor precisions of the mixed-precision solvers */
+#define DDLA_PREC_FP64 0
+#define DDLA_PREC_FP32 1
+#define DDLA_PREC_FP16 2
+#define DDLA_PREC_BF16 3
+
+/* ---------------------------------------------------------------- runtime ---------- */
+/* No counterpart in the reference (BLIS needs no context); explicit set-up so a P/Invoke
+ * caller controls device selection and tear-down. */
+int ddla_init(int device);
+int ddla_finalize(void);
+/* Copies the message of the last error (NUL-terminated, truncated to len). */
+int ddla_last_error(char* buf, size_t len);
+/* Library stream: a hipStream_t cast to void*; NULL selects the library's own stream. */
+int ddla_set_stream(void* stream);
+int ddla_get_stream(void** stream);
+int ddla_sync(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int ddla_version(void);
+
+/* Per-kernel timing, no counterpart in the reference.  While enabled, every launch of the
+ * library's kernels is bracketed by HIP events on its stream; the totals per kernel name
+ * (launch count, GPU milliseconds, algorithmic flops and bytes) are read back with
+ * ddla_profile_read.  idx < 0 returns the number of entries. */
+int ddla_profile_enable(int on);
+int ddla_profile_reset(void);
+int ddla_profile_read(int idx, char* name, size_t len, int64_t* count, double* total_ms,
+                      double* flops, double* bytes);
+
+/* ------------------------------------------------------- device buffers ---------- */
+/* Device-resident matrices (the boundary's replacement for pinned managed arrays). */
+int ddla_malloc(void** dptr, size_t bytes);
+int ddla_free(void* dptr);
+int ddla_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int ddla_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int ddla_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes);
+/* Strided host view (m x n, rs, cs) <-> row-major device matrix with leading dim ld. */
+int ddla_dmat_set(ddla_dim_t m, ddla_dim_t n, const double* host, ddla_inc_t rs, ddla_inc_t cs,
+                  double* dev, ddla_inc_t ld);
+int ddla_dmat_get(ddla_dim_t m, ddla_dim_t n, const double* dev, ddla_inc_t ld, double* host,
+                  ddla_inc_t rs, ddla_inc_t cs);
+/* Fills an m x n row-major device matrix with the SplitMix64 stream mapped to [-1, 1):
+ * element k = i*n + j gets (splitmix64(seed + (k+1)*0x9E3779B97F4A7C15) >> 11)*2^-52 - 1.
+ * Replaces Rand/randm (DDLA/BLAS/Managed/utils.cs:145-233 -> bli_drandm) for device data. */
+int ddla_drandm_dev(ddla_dim_t m, ddla_dim_t n, double* dev, ddla_inc_t ld, uint64_t seed);
+
+/* ------------------------------------------------------------------- GEMM ---------- */
+/* C = alpha*op(A)*op(B) + beta*C with BLIS-style strides; host pointers; synchronous.
+ * Same argument list as bli_dgemm (DDLA/BLAS/Native/3m.cs:12-24), the call behind
+ * BlasProvider.GeMM (DDLA/BLAS/3m.cs:10-42).  beta == 0 overwrites C without reading it
+ * (Managed/3m.cs:32 -> Managed/1m.cs:265-269); k == 0 only scales C. */
+int ddla_dgemm(int transa, int transb, ddla_dim_t m, ddla_dim_t n, ddla_dim_t k,
+               const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa,
+               const double* b, ddla_inc_t rsb, ddla_inc_t csb, const double* beta, double* c,
+               ddla_inc_t rsc, ddla_inc_t csc);
+/* Same operation on device pointers (any strides; unit stride along rows or columns of
+ * every operand takes the fast path); stream-ordered. */
+int ddla_dgemm_dev(int transa, int transb, ddla_dim_t m, ddla_dim_t n, ddla_dim_t k,
+                   const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa,
+                   const double* b, ddla_inc_t rsb, ddla_inc_t csb, const double* beta,
+                   double* c, ddla_inc_t rsc, ddla_inc_t csc);
+
+/* ------------------------------------------------------ LU (partial pivoting) ------ */
+/* Blocked right-looking PLU of a row-major n x n device matrix, block 128, in place.
+ * Replaces LU.PLUDecompose (DDLA/Factorizations/LU.cs:251-264, PLUDecBlock :289-342,
+ * PLUDecUnblock :344-374).  dpiv: device int32[n], relative pivots.  Returns info
+ * (first zero pivot, 1-based; the reference throws from InvScal, DDLA/BLAS/1v.cs:99). */
+int ddla_dgetrf_dev(ddla_dim_t n, double* a, ddla_inc_t lda, int32_t* dpiv);
+/* Solves A X = B (trans == DDLA_NO_TRANSPOSE) or A^T X = B with the factors of
+ * ddla_dgetrf_dev; B (n x nrhs row-major, device) is overwritten with X.
+ * Replaces LU.PLUSolve (LU.cs:266-287, PLUSolveInternel :431-465). */
+int ddla_dgetrs_dev(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
+                    const int32_t* dpiv, double* b, ddla_inc_t ldb);
+/* Host-pointer forms with strided views (synchronous). */
+int ddla_dgetrf(ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t csa, int32_t* piv);
+int ddla_dgetrs(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
+                ddla_inc_t csa, const int32_t* piv, double* b, ddla_inc_t rsb, ddla_inc_t csb);
+/* Row interchanges with relative pivots: side DDLA_LEFT swaps rows, DDLA_RIGHT columns;
+ * forward applies P, backward P^-1.  Replaces LU.ApplyPiv (LU.cs:237-249, :376-429). */
+int ddla_dlaswp_dev(int side, int forward, ddla_dim_t m, ddla_dim_t n, double* x, ddla_inc_t ldx,
+                    const int32_t* dpiv, ddla_dim_t npiv);
+
+/* ----------------------------------------------------------------- Cholesky -------- */
+/* Blocked Cholesky (block 128) of a row-major n x n SPD device matrix in place; only the
+ * uplo triangle is read and written.  Replaces Cholesky.CholDecompose
+ * (DDLA/Factorizations/Cholesky.cs:52-69, lower :127-160, upper :162-208).  Returns info
+ * (1-based index of the first pivot that is not > 0; Cholesky.cs:228-236). */
+int ddla_dpotrf_dev(int uplo, ddla_dim_t n, double* a, ddla_inc_t lda);
+/* Solves A X = B with the factor of ddla_dpotrf_dev (Cholesky.cs:102-123, :210-226). */
+int ddla_dpotrs_dev(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
+                    double* b, ddla_inc_t ldb);
+int ddla_dpotrf(int uplo, ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t csa);
+int ddla_dpotrs(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
+                ddla_inc_t csa, double* b, ddla_inc_t rsb, ddla_inc_t csb);
+
+/* ------------------------------------------------------------ level-3 pieces ------- */
+/* B = alpha * op(A)^-1 B (left) or B op(A)^-1 (right), row-major device matrices.
+ * Replaces bli_dtrsm (DDLA/BLAS/Native/3m.cs:151-162) behind BlasProvider.TrSM
+ * (DDLA/BLAS/3m.cs:287-314). */
+int ddla_dtrsm_dev(int side, int uplo, int transa, int diag, ddla_dim_t m, ddla_dim_t n,
+                   const double* alpha, const double* a, ddla_inc_t lda, double* b,
+                   ddla_inc_t ldb);
+/* uplo triangle of C = beta*C + alpha*op(A) op(A)^T, row-major device matrices.
+ * Replaces bli_dsyrk (DDLA/BLAS/Native/3m.cs:96-106) behind BlasProvider.SyRk
+ * (DDLA/BLAS/3m.cs:118-147). */
+int ddla_dsyrk_dev(int uplo, int transa, ddla_dim_t m, ddla_dim_t k, const double* alpha,
+                   const double* a, ddla_inc_t lda, const double* beta, double* c,
+                   ddla_inc_t ldc);
+/* Index of the first max |x_i| (NaN smallest; empty -> 0), written to *index (host).
+ * Replaces bli_damaxv (DDLA/BLAS/Native/1v.cs:26-31) behind BlasProvider.AMax
+ * (DDLA/BLAS/1v.cs:30-41). */
+int ddla_damaxv_dev(ddla_dim_t n, const double* x, ddla_inc_t incx, ddla_dim_t* index);
+
+/* ------------------------------------------------ mixed precision + refinement ----- */
+/* No counterpart in the reference: factor in a lower precision on the matrix cores, then
+ * recover FP64 accuracy by iterative refinement on the GPU (residual, correction and
+ * convergence test in FP64). */
+typedef struct ddla_mpir_opts {
+  int32_t factor_prec;   /* DDLA_PREC_FP16 / _BF16 / _FP32 / _FP64 */
+  int32_t max_iters;     /* refinement steps before giving up (default 50 if <= 0) */
+  double tol;            /* target normwise backward error (default 1e-14 if <= 0) */
+  int32_t gmres;         /* 0 = classic refinement only, 1 = switch to GMRES-IR if the
+                            classic iteration stalls, 2 = GMRES-IR from the start */
+  int32_t reserved;
+} ddla_mpir_opts;
+
+typedef struct ddla_mpir_info {
+  int32_t status;        /* 0 converged, > 0 factor info, < 0 error / not converged */
+  int32_t iters;         /* refinement steps (GMRES inner steps counted in inner_iters) */
+  int32_t inner_iters;
+  int32_t fallback;      /* 1 if the low-precision factor failed and FP32 was used */
+  double backward_error; /* max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf) */
+  double factor_ms;      /* GPU time of the factorization */
+  double refine_ms;      /* GPU time of the refinement */
+} ddla_mpir_info;
+
+/* Solves A X = B for a row-major n x n FP64 device matrix A (not modified) and n x nrhs
+ * B, writing X (n x nrhs, ldx).  LU with partial pivoting in the factor precision. */
+int ddla_dgesv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
+                        const double* b, ddla_inc_t ldb, double* x, ddla_inc_t ldx,
+                        const ddla_mpir_opts* opts, ddla_mpir_info* info);
+/* Same for SPD A with a lower Cholesky factor; falls back to an FP32 factor when the
+ * low-precision factor is not positive definite. */
+int ddla_dposv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
+                        const double* b, ddla_inc_t ldb, double* x, ddla_inc_t ldx,
+                        const ddla_mpir_opts* opts, ddla_mpir_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDLA_NATIVE_H */

@@ -1,0 +1,200 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: a CPU restatement of DDLA's managed FP64 GEMM, blocked LU with
+partial pivoting and blocked Cholesky, used to check the HIP library and as the timed CPU
+baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import it.
+Every function mirrors the reference routine named in its docstring (paths relative to
+the DDLA repository root); the C source cites the same lines.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+NO_TRANS, TRANS = 0x00, 0x08
+LOWER, UPPER = 0xC0, 0x60
+NON_UNIT, UNIT = 0x000, 0x100
+LEFT, RIGHT = 0, 1
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the Makefile beside this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        i64, dp, ip = ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+        d, i = ctypes.c_double, ctypes.c_int
+        L.orc_set_threads.argtypes = [i]
+        L.orc_set_threads.restype = i
+        L.orc_get_threads.restype = i
+        L.orc_set_block.argtypes = [i, i]
+        L.orc_fill_uniform.argtypes = [dp, i64, ctypes.c_uint64]
+        L.orc_dgemm.argtypes = [i, i, i64, i64, i64, d, dp, i64, i64, dp, i64, i64, d, dp, i64, i64]
+        L.orc_damaxv.argtypes = [i64, dp, i64]
+        L.orc_damaxv.restype = i64
+        L.orc_dtrsm.argtypes = [i, i, i, i, i64, i64, d, dp, i64, i64, dp, i64, i64]
+        L.orc_dsyrk.argtypes = [i, i, i64, i64, d, dp, i64, i64, d, dp, i64, i64]
+        L.orc_dgetrf.argtypes = [i64, dp, i64, i64, ip]
+        L.orc_dgetrf.restype = i64
+        L.orc_dgetf2.argtypes = [i64, i64, dp, i64, i64, ip]
+        L.orc_dgetf2.restype = i64
+        L.orc_dgetrf_nopiv.argtypes = [i64, dp, i64, i64]
+        L.orc_dgetrf_nopiv.restype = i64
+        L.orc_dlaswp.argtypes = [i, i, i64, i64, dp, i64, i64, ip, i64]
+        L.orc_dgetrs.argtypes = [i, i64, i64, dp, i64, i64, ip, dp, i64, i64]
+        L.orc_dpotrf.argtypes = [i, i64, dp, i64, i64]
+        L.orc_dpotrf.restype = i64
+        L.orc_dpotrs.argtypes = [i, i64, i64, dp, i64, i64, dp, i64, i64]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _strides(a):
+    return a.strides[0] // 8, a.strides[1] // 8
+
+
+def set_threads(t: int) -> int:
+    return lib().orc_set_threads(int(t))
+
+
+def threads() -> int:
+    return lib().orc_get_threads()
+
+
+def set_block(piv_block: int = 128, chol_block: int = 128):
+    lib().orc_set_block(int(piv_block), int(chol_block))
+
+
+def uniform(n_elems: int, seed: int) -> np.ndarray:
+    """SplitMix64 counter stream mapped to [-1, 1): x_i = (z_i >> 11) * 2^-52 - 1."""
+    out = np.empty(int(n_elems), dtype=np.float64)
+    lib().orc_fill_uniform(_dp(out), out.size, ctypes.c_uint64(seed & (2**64 - 1)))
+    return out
+
+
+def uniform_matrix(m: int, n: int, seed: int) -> np.ndarray:
+    return uniform(m * n, seed).reshape(m, n)
+
+
+def gemm(alpha, A, B, beta, C, transa=NO_TRANS, transb=NO_TRANS):
+    """Managed BlasProvider.GeMM (DDLA/BLAS/Managed/3m.cs:13-49); C updated in place."""
+    assert A.dtype == B.dtype == C.dtype == np.float64
+    m, n = C.shape
+    k = A.shape[0] if transa == TRANS else A.shape[1]
+    lib().orc_dgemm(transa, transb, m, n, k, alpha, _dp(A), *_strides(A), _dp(B), *_strides(B),
+                    beta, _dp(C), *_strides(C))
+    return C
+
+
+def matmul(A, B):
+    """C = A * B through the managed GEMM (MatrixView.operator*, Core/MatrixView.cs:844-854)."""
+    C = np.empty((A.shape[0], B.shape[1]), dtype=np.float64)
+    return gemm(1.0, A, B, 0.0, C)
+
+
+def amax(x):
+    """AMax (DDLA/BLAS/1v.cs:30-41)."""
+    x = np.asarray(x, dtype=np.float64)
+    return int(lib().orc_damaxv(x.size, _dp(x), x.strides[0] // 8))
+
+
+def trsm(side, uplo, trans, diag, alpha, A, B):
+    """TRSM (facade DDLA/BLAS/3m.cs:287-314); B overwritten with the solution."""
+    m, n = B.shape
+    lib().orc_dtrsm(side, uplo, trans, diag, m, n, alpha, _dp(A), *_strides(A), _dp(B), *_strides(B))
+    return B
+
+
+def syrk(uplo, trans, alpha, A, beta, C):
+    """SYRK on the uplo triangle (facade DDLA/BLAS/3m.cs:118-147)."""
+    m = C.shape[0]
+    k = A.shape[0] if trans == TRANS else A.shape[1]
+    lib().orc_dsyrk(uplo, trans, m, k, alpha, _dp(A), *_strides(A), beta, _dp(C), *_strides(C))
+    return C
+
+
+def getrf(A):
+    """PLUDecompose (LU.cs:251-264) in place. Returns (info, relative pivots int32[n])."""
+    n = A.shape[0]
+    piv = np.zeros(n, dtype=np.int32)
+    info = lib().orc_dgetrf(n, _dp(A), *_strides(A), _ip(piv))
+    return int(info), piv
+
+
+def getf2(P):
+    """PLUDecUnblock (LU.cs:344-374) on an m x nb panel in place."""
+    m, nb = P.shape
+    piv = np.zeros(min(m, nb), dtype=np.int32)
+    info = lib().orc_dgetf2(m, nb, _dp(P), *_strides(P), _ip(piv))
+    return int(info), piv
+
+
+def getrf_nopiv(A):
+    """LUDecompose (LU.cs:133-142) in place."""
+    return int(lib().orc_dgetrf_nopiv(A.shape[0], _dp(A), *_strides(A)))
+
+
+def laswp(X, piv, side=LEFT, forward=True):
+    """ApplyPiv (LU.cs:237-249) in place."""
+    piv = np.ascontiguousarray(piv, dtype=np.int32)
+    m, n = X.shape
+    lib().orc_dlaswp(side, 1 if forward else 0, m, n, _dp(X), *_strides(X), _ip(piv), piv.size)
+    return X
+
+
+def getrs(A, piv, B, trans=False):
+    """PLUSolve (LU.cs:266-287,431-465); B overwritten with X."""
+    piv = np.ascontiguousarray(piv, dtype=np.int32)
+    n, nrhs = B.shape
+    lib().orc_dgetrs(TRANS if trans else NO_TRANS, n, nrhs, _dp(A), *_strides(A), _ip(piv),
+                     _dp(B), *_strides(B))
+    return B
+
+
+def potrf(A, uplo=LOWER):
+    """CholDecompose (Cholesky.cs:52-69) in place; returns info (0 or first bad pivot+1)."""
+    return int(lib().orc_dpotrf(uplo, A.shape[0], _dp(A), *_strides(A)))
+
+
+def potrs(A, B, uplo=LOWER):
+    """CholeskySolve (Cholesky.cs:102-123,210-226); B overwritten with X."""
+    n, nrhs = B.shape
+    lib().orc_dpotrs(uplo, n, nrhs, _dp(A), *_strides(A), _dp(B), *_strides(B))
+    return B
+
+
+def lu_factors(LU):
+    """Split a packed LU into unit-lower L and upper U (LU.Deconstruct, LU.cs:112-128)."""
+    L = np.tril(LU, -1) + np.eye(LU.shape[0])
+    U = np.triu(LU)
+    return L, U
+
+
+def perm_matrix(piv):
+    """P with P*A = L*U, built like LU.Deconstruct: ApplyPiv on the identity."""
+    n = len(piv)
+    P = np.eye(n)
+    laswp(P, piv)
+    return P

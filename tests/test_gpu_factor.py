@@ -1,0 +1,184 @@
+"""GPU parity of the FP64 blocked LU (partial pivoting) and Cholesky against the CPU oracle:
+the pivot sequence must match exactly; factors within 1e-13*n relative Frobenius error."""
+import numpy as np
+import pytest
+
+from conftest import golden, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+SEED_C2 = 0xDD1A0002
+
+
+def backward_error(A, x, b):
+    r = b - A @ x
+    return np.linalg.norm(r, np.inf) / (np.linalg.norm(A, np.inf) * np.linalg.norm(x, np.inf))
+
+
+def test_lu_exact_gepp_fixtures(ddla):
+    e = golden("gepp_exact.npz")
+    for t in range(int(e["count"])):
+        A = e[f"A{t}"].copy()
+        piv = ddla.PLUDecompose(A)
+        np.testing.assert_array_equal(piv, e[f"piv{t}"])
+        np.testing.assert_allclose(A, e[f"LU{t}"], rtol=0, atol=1e-12)
+
+
+def test_lu_designed_known_answers(ddla):
+    d = golden("lu_designed.npz")
+    for t in range(int(d["count"])):
+        A = d[f"A{t}"].copy()
+        piv = ddla.PLUDecompose(A)
+        np.testing.assert_array_equal(piv, d[f"piv{t}"])
+        np.testing.assert_array_equal(np.tril(A, -1) + np.eye(len(A)), d[f"L{t}"])
+        np.testing.assert_array_equal(np.triu(A), d[f"U{t}"])
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 127, 128, 129, 200, 257, 513, 1000, 2048])
+def test_lu_pivots_and_factors_match_oracle(ddla, oracle, n):
+    A0 = oracle.uniform_matrix(n, n, SEED_C2 + n)
+    Ao = A0.copy()
+    info, piv_o = oracle.getrf(Ao)
+    assert info == 0
+    A = A0.copy()
+    piv = ddla.PLUDecompose(A)
+    np.testing.assert_array_equal(piv, piv_o)
+    assert rel_fro(A, Ao) <= 1e-13 * n
+    # reconstruction P A = L U (LU.Deconstruct, TestPLU.cs:79-116)
+    L = np.tril(A, -1) + np.eye(n)
+    U = np.triu(A)
+    P = oracle.perm_matrix(piv)
+    assert rel_fro(P @ A0, L @ U) <= 1e-14 * n
+
+
+def test_lu_device_resident_and_solve(ddla, oracle):
+    n, nrhs = 777, 3
+    A0 = oracle.uniform_matrix(n, n, 42)
+    B0 = oracle.uniform_matrix(n, nrhs, 43)
+    dA = ddla.DeviceMatrix.from_host(A0)
+    piv = ddla.PLUDecompose(dA)
+    Ao = A0.copy()
+    _, piv_o = oracle.getrf(Ao)
+    np.testing.assert_array_equal(piv, piv_o)
+    for trans in (False, True):
+        dB = ddla.DeviceMatrix.from_host(B0)
+        ddla.PLUSolve(dA, piv, dB, trans)
+        X = dB.to_host()
+        Xo = oracle.getrs(Ao, piv_o, B0.copy(), trans=trans)
+        assert rel_fro(X, Xo) <= 1e-11
+        M = A0.T if trans else A0
+        for j in range(nrhs):
+            assert backward_error(M, X[:, j], B0[:, j]) < 1e-15 * 10
+
+
+def test_lu_class_mirror(ddla):
+    n = 300
+    rng = np.random.default_rng(5)
+    A = rng.uniform(-1, 1, (n, n))
+    b = rng.uniform(-1, 1, n)
+    lu = ddla.LU(A)
+    x = lu.Solve(b)
+    assert backward_error(A, x, b) < 1e-14
+    P, L, U = lu.Deconstruct()
+    assert rel_fro(P @ A, L @ U) < 1e-13
+
+
+def test_lu_zero_pivot_raises(ddla):
+    A = np.random.default_rng(1).uniform(-1, 1, (200, 200))
+    A[:, 150] = 0.0
+    with pytest.raises(ZeroDivisionError):
+        ddla.PLUDecompose(A)
+    # info value through the C ABI: first zero pivot, 1-based
+    d = ddla.DeviceMatrix.from_host(np.random.default_rng(1).uniform(-1, 1, (200, 200)) * 0 +
+                                    np.diag(np.r_[np.ones(150), 0.0, np.ones(49)]))
+    dpiv = ddla._DevInts(200)
+    assert ddla.native.lib().ddla_dgetrf_dev(200, d.ptr, d.ld, dpiv.ptr) == 151
+
+
+def test_laswp_device(ddla, oracle):
+    n = 300
+    rng = np.random.default_rng(2)
+    piv = np.array([rng.integers(0, n - i) for i in range(n)], dtype=np.int32)
+    X0 = oracle.uniform_matrix(n, 20, 4)
+    for forward in (True, False):
+        d = ddla.DeviceMatrix.from_host(X0)
+        ddla.ApplyPiv(d, piv, forward=forward)
+        np.testing.assert_array_equal(d.to_host(), oracle.laswp(X0.copy(), piv, forward=forward))
+    Y0 = oracle.uniform_matrix(20, n, 5)
+    d = ddla.DeviceMatrix.from_host(Y0)
+    ddla.ApplyPiv(d, piv, side=ddla.RIGHT)
+    np.testing.assert_array_equal(d.to_host(), oracle.laswp(Y0.copy(), piv, side=oracle.RIGHT))
+
+
+@pytest.mark.slow
+def test_lu_8192_pivot_sequence_and_backward_error(ddla, oracle):
+    # config: n=8192, A uniform[-1,1], b = A x_true; pivot sequence must match exactly
+    n = 8192
+    A0 = oracle.uniform_matrix(n, n, SEED_C2)
+    xt = oracle.uniform(n, SEED_C2 + 1).reshape(-1, 1)
+    b = oracle.matmul(A0, xt)
+    Ao = A0.copy()
+    info, piv_o = oracle.getrf(Ao)
+    assert info == 0
+    dA = ddla.DeviceMatrix.from_host(A0)
+    piv = ddla.PLUDecompose(dA)
+    mism = np.nonzero(piv != piv_o)[0]
+    assert mism.size == 0, f"first divergent pivot at step {mism[0] if mism.size else -1}"
+    db = ddla.DeviceMatrix.from_host(b)
+    ddla.PLUSolve(dA, piv, db)
+    x = db.to_host()
+    assert backward_error(A0, x[:, 0], b[:, 0]) < 1e-15 * 20
+    assert rel_fro(dA.to_host(), Ao) <= 1e-13 * n
+
+
+# ------------------------------------------------------------------------- Cholesky ---
+def test_chol_integer_known_answers(ddla):
+    c = golden("chol_int.npz")
+    for t in range(int(c["count"])):
+        A = c[f"A{t}"].copy()
+        ddla.CholDecompose(ddla.LOWER, A)
+        np.testing.assert_allclose(np.tril(A), c[f"L{t}"], rtol=0, atol=1e-12)
+        np.testing.assert_array_equal(np.triu(A, 1), np.triu(c[f"A{t}"], 1))
+        U = c[f"A{t}"].copy()
+        ddla.CholDecompose(ddla.UPPER, U)
+        np.testing.assert_allclose(np.triu(U), c[f"L{t}"].T, rtol=0, atol=1e-12)
+        np.testing.assert_array_equal(np.tril(U, -1), np.tril(c[f"A{t}"], -1))
+
+
+@pytest.mark.parametrize("n", [1, 7, 128, 129, 300, 1000, 2048])
+@pytest.mark.parametrize("uplo", [0xC0, 0x60])
+def test_chol_matches_oracle(ddla, oracle, n, uplo):
+    G = oracle.uniform_matrix(n, n, 77 + n)
+    S = G @ G.T + n * np.eye(n)
+    So = S.copy()
+    assert oracle.potrf(So, uplo) == 0
+    A = S.copy()
+    ddla.CholDecompose(uplo, A)
+    tri = np.tril if uplo == 0xC0 else np.triu
+    assert rel_fro(tri(A), tri(So)) <= 1e-13 * n
+    B = oracle.uniform_matrix(n, 4, 3)
+    X = B.copy()
+    ddla.CholeskySolve(uplo, A, X)
+    Xo = oracle.potrs(So, B.copy(), uplo)
+    assert rel_fro(X, Xo) <= 1e-12
+
+
+def test_chol_not_spd(ddla):
+    S = np.eye(300)
+    S[200, 200] = -1.0
+    with pytest.raises(ddla.LinalgException):
+        ddla.CholDecompose(ddla.LOWER, S.copy())
+    d = ddla.DeviceMatrix.from_host(S)
+    assert ddla.native.lib().ddla_dpotrf_dev(ddla.LOWER, 300, d.ptr, d.ld) == 201
+    d = ddla.DeviceMatrix.from_host(S)
+    assert ddla.native.lib().ddla_dpotrf_dev(ddla.UPPER, 300, d.ptr, d.ld) == 201
+
+
+def test_chol_class_mirror(ddla):
+    n = 400
+    rng = np.random.default_rng(9)
+    G = rng.standard_normal((n, n))
+    S = G @ G.T + n * np.eye(n)
+    b = rng.uniform(-1, 1, n)
+    x = ddla.Cholesky(S).Solve(b)
+    assert backward_error(S, x, b) < 1e-14

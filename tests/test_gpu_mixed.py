@@ -1,0 +1,83 @@
+"""Mixed-precision LU / Cholesky + iterative refinement: FP64-level backward error and
+forward error against the FP64 oracle solution, on matrices of known condition number."""
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+def conditioned(n, kappa, seed, spd=False):
+    """A = U diag(s) V^T with U, V from QR of Gaussians and s geometric from 1 to 1/kappa
+    (the construction of the mixed-precision benchmark at small n)."""
+    rng = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = kappa ** (-np.arange(n) / (n - 1))
+    if spd:
+        return (U * s) @ U.T
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (U * s) @ V.T
+
+
+def run(ddla, fn, A, nrhs, prec, seed=1, tol=1e-14, gmres=1):
+    n = A.shape[0]
+    B = np.random.default_rng(seed).uniform(-1, 1, (n, nrhs))
+    dA, dB = ddla.DeviceMatrix.from_host(A), ddla.DeviceMatrix.from_host(B)
+    dX = ddla.DeviceMatrix(n, nrhs)
+    info = fn(dA, dB, dX, prec=prec, tol=tol, gmres=gmres)
+    np.testing.assert_array_equal(dA.to_host(), A)  # A is not modified
+    return info, dX.to_host(), B
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3])  # FP32, FP16, BF16
+@pytest.mark.parametrize("kappa", [1e2, 1e4])
+@pytest.mark.parametrize("nrhs", [1, 5])
+def test_gesv_mpir_converges(ddla, oracle, prec, kappa, nrhs):
+    n = 1024
+    A = conditioned(n, kappa, 11)
+    info, X, B = run(ddla, ddla.gesv_mpir, A, nrhs, prec)
+    assert info.status == 0, (info.status, info.iters, info.backward_error)
+    assert info.backward_error <= 1e-14
+    for j in range(nrhs):
+        r = B[:, j] - A @ X[:, j]
+        be = np.linalg.norm(r, np.inf) / (np.linalg.norm(A, np.inf) * np.linalg.norm(X[:, j], np.inf))
+        assert be <= 2e-14
+    # forward error against the FP64 oracle solution, bounded by kappa * u
+    Ao = A.copy()
+    _, piv = oracle.getrf(Ao)
+    Xo = oracle.getrs(Ao, piv, B.copy())
+    assert rel_fro(X, Xo) <= kappa * 1e-14
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3])
+def test_posv_mpir_converges(ddla, oracle, prec):
+    n = 1024
+    A = conditioned(n, 1e4, 12, spd=True)
+    A = (A + A.T) / 2
+    info, X, B = run(ddla, ddla.posv_mpir, A, 2, prec)
+    assert info.status == 0, (info.status, info.iters, info.backward_error)
+    assert info.backward_error <= 1e-14
+    So = A.copy()
+    assert oracle.potrf(So) == 0
+    Xo = oracle.potrs(So, B.copy())
+    assert rel_fro(X, Xo) <= 1e4 * 1e-14
+
+
+def test_posv_mpir_fallback_to_fp32(ddla):
+    # lambda down to 1e-5: a BF16 factor (u = 2^-8) of this SPD matrix loses positive
+    # definiteness, an FP32 factor does not
+    n = 512
+    A = conditioned(n, 1e5, 13, spd=True)
+    A = (A + A.T) / 2
+    info, X, B = run(ddla, ddla.posv_mpir, A, 1, 3)
+    assert info.fallback == 1
+    assert info.status == 0 and info.backward_error <= 1e-14
+
+
+def test_gesv_mpir_rejects_singular(ddla):
+    n = 256
+    A = np.random.default_rng(3).uniform(-1, 1, (n, n))
+    A[:, 100] = 0.0
+    info, _, _ = run(ddla, ddla.gesv_mpir, A, 1, 2)
+    assert info.status == 101
