@@ -393,10 +393,9 @@ class Cholesky:
         x = b if inplace else np.array(_as_f64(b), copy=True)
         CholeskySolve(self.uplo, self.matrix, x)
         return x
-
-
-def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_iters, gmres):
-    opts = MpirOpts(prec, max_iters, tol, gmres, 0)
+def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_iters, gmres,
+          terms):
+    opts = MpirOpts(prec, max_iters, tol, gmres, terms)
     info = MpirInfo()
     st = fn(A.rows, B.cols, A.ptr, A.ld, B.ptr, B.ld, X.ptr, X.ld, ctypes.byref(opts),
             ctypes.byref(info))
@@ -406,12 +405,14 @@ def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_
 
 
 def gesv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16, tol=1e-14,
-              max_iters=50, gmres=1) -> MpirInfo:
-    """Mixed-precision LU + iterative refinement to FP64 accuracy (no reference counterpart)."""
-    return _mpir(_lib().ddla_dgesv_mpir_dev, A, B, X, prec, tol, max_iters, gmres)
+              max_iters=50, gmres=1, terms=0) -> MpirInfo:
+    """Mixed-precision LU + iterative refinement to FP64 accuracy (no reference counterpart).
+
+    terms: low-precision parts per operand of the trailing updates (0 = library default)."""
+    return _mpir(_lib().ddla_dgesv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms)
 
 
 def posv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16, tol=1e-14,
-              max_iters=50, gmres=1) -> MpirInfo:
+              max_iters=50, gmres=1, terms=0) -> MpirInfo:
     """Mixed-precision Cholesky + iterative refinement (no reference counterpart)."""
-    return _mpir(_lib().ddla_dposv_mpir_dev, A, B, X, prec, tol, max_iters, gmres)
+    return _mpir(_lib().ddla_dposv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms)

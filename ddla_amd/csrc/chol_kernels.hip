@@ -70,8 +70,14 @@ __global__ __launch_bounds__(256) void trsm_rlt_kernel(int nb, int64_t nrows,
   const int tid = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * 64;
   const int rh = (int)min<int64_t>(64, nrows - r0);
-  for (int r = 0; r < nb; ++r)
-    for (int c = tid; c <= r; c += 256) Ls[r * (r + 1) / 2 + c] = L[(int64_t)r * ldl + c];
+  const int npk = nb * (nb + 1) / 2;
+  for (int p = tid; p < npk; p += 256) {  // flat packed index: independent loads
+    int r = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
+    while (r * (r + 1) / 2 > p) --r;
+    while ((r + 1) * (r + 2) / 2 <= p) ++r;
+    const int c = p - r * (r + 1) / 2;
+    Ls[p] = L[(int64_t)r * ldl + c];
+  }
   for (int idx = tid; idx < 64 * nb; idx += 256) {
     const int r = idx / nb, c = idx % nb;
     S[c * SL + r] = (r < rh) ? B[(r0 + r) * ldb + c] : static_cast<T>(0);

@@ -125,9 +125,9 @@ int launch_sgemm(int64_t m, int64_t n, int64_t k, float alpha, const float* A, i
 
 // C += alpha*A*B on the FP16 (bf16=false) or BF16 matrix cores, FP32 C, FP32 operands
 // rounded per 128-block with power-of-two scales; scale_ws holds >= tiles_m + tiles_n floats
-int launch_hgemm(bool bf16, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k, float alpha,
-                 const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
-                 float* scale_ws, hipStream_t s);
+int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k,
+                 float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                 int64_t ldc, float* scale_ws, hipStream_t s);
 
 // general strided copy / transpose helpers
 int launch_dcopy2d(int64_t m, int64_t n, const double* src, int64_t rs, int64_t cs, double* dst,

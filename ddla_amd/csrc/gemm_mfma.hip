@@ -19,8 +19,9 @@
 //     own L2;
 //   * epilogue C = alpha*acc + beta*C in registers (beta == 0 never reads C, matching
 //     Managed/1m.cs:265-269), optional lower/upper triangle mask for SYRK.
-// The 16x16x4 FP64 and FP32 instructions share one operand layout: lane l supplies
-// A[l%16][l/16] and B[l/16][l%16] and receives D[4*(l/16)+r][l%16], r = 0..3.
+// Operand layout of both 16x16x4 instructions: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].  The result layouts differ (measured on gfx950): the FP32 form returns
+// D[4*(l/16)+r][l%16] in register r, the FP64 form D[l/16+4*r][l%16].
 #include "ddla_internal.hpp"
 
 namespace ddla {
@@ -35,6 +36,8 @@ struct Vec<double> {
   static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // row within the 16x16 tile of result register r of lane group lk = lane/16
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
 };
 template <>
 struct Vec<float> {
@@ -43,6 +46,7 @@ struct Vec<float> {
   static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
 template <typename T, int BM_, int BN_>
@@ -229,14 +233,14 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, typen
     __syncthreads();
   }
 
-  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of every 16x16 tile
+  // epilogue: register r of lane l holds row Vec<E>::row(l/16, r), column l%16 of each tile
 #pragma unroll
   for (int i = 0; i < C::TM; ++i)
 #pragma unroll
     for (int j = 0; j < C::TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wm * C::WM + i * 16 + lk * 4 + r;
+        const int gi = m0 + wm * C::WM + i * 16 + Vec<E>::row(lk, r);
         const int gj = n0 + wn * C::WN + j * 16 + l16;
         bool ok = gi < M && gj < N;
         if (TRI == 1) ok = ok && gj <= gi;

@@ -8,14 +8,19 @@
 // R rows each, held in LDS for the whole panel.  Partial pivoting needs one global
 // arg-max per column; each column step is one exchange round between the workgroups:
 //   1. every workgroup reduces |a_ri| over its rows (first index wins ties, NaN is the
-//      smallest value, exactly AMax's rule) and publishes (key, row, candidate row
-//      contents) in a double-buffered slot, then raises its step flag;
-//   2. every workgroup waits for all flags, reduces the W candidates to the same winner,
-//      reads the winner's row from its slot, and performs the interchange locally (the
-//      owner of the diagonal row also published that row, so the owner of the pivot row
-//      can take it);
+//      smallest value, exactly AMax's rule) and publishes its candidate (key, row) and the
+//      candidate row's contents; the owner of the diagonal row also publishes that row;
+//   2. every workgroup reads the W candidates, reduces them to the same winner, reads the
+//      winner's row, and performs the interchange locally (the owner of the pivot row takes
+//      the published diagonal row);
 //   3. every workgroup scales its part of the column by 1/pivot (InvScal multiplies by
 //      the reciprocal, DDLA/BLAS/1v.cs:102) and applies the rank-1 update to its rows.
+// The exchange needs no fence or barrier flag: every published 64-bit word carries the
+// 32-bit step tag next to 32 bits of payload (an FP64 value takes two words), is written
+// and read with single-copy-atomic agent-scope accesses, and a reader simply re-reads a
+// word until its tag is the current step's.  Slots alternate between two buffers by step
+// parity; a workgroup can only reach step i+2 after every workgroup published step i+1,
+// i.e. after every reader of step i finished, so a slot is never overwritten while read.
 // Every row carries one extra LDS column: the panel-relative index of the row whose
 // trailing data belongs there.  At the end each workgroup emits (dst, src) pairs for
 // moved rows, so the interchanges for the rest of the matrix are one parallel gather
@@ -27,20 +32,21 @@
 namespace ddla {
 
 struct PanelSync {
-  // double-buffered exchange area, laid out in one workspace allocation
-  unsigned* flags;     // [2][W]
-  double* keys;        // [2][W]
-  int* rows;           // [2][W]
-  void* cand;          // [2][W][NBX] elements of T
-  void* diag;          // [2][NBX]
-  int* list;           // [2*nb][2] (dst, src) panel-relative
-  int* list_count;     // [1]
+  // double-buffered exchange area (one workspace allocation), 64-bit tagged words
+  unsigned long long* keys;  // [2][WMAX][3]: key high, key low, row
+  unsigned long long* cand;  // [2][WMAX][NBX][2]: candidate row (2 words per element)
+  unsigned long long* diag;  // [2][NBX][2]: diagonal row of the step
+  int* list;                 // [2*nb][2] (dst, src) panel-relative
+  int* list_count;           // [1]
 };
 
 namespace {
 
 constexpr int NB_MAX = 128;
 constexpr int NBX = NB_MAX + 1;  // + origin column
+constexpr int WMAX = 256;        // one exchange slot per lane of a workgroup
+
+typedef unsigned long long u64;
 
 __device__ __forceinline__ void better(double& k, int& r, double k2, int r2) {
   if (k2 > k || (k2 == k && r2 < r)) {
@@ -70,28 +76,41 @@ __device__ __forceinline__ void block_argmax(double& key, int& row, double* sk, 
   for (int w = 1; w < 4; ++w) better(key, row, sk[w], sr[w]);
 }
 
-template <typename T>
-__device__ __forceinline__ T ld_agent(const T* p) {
-  if constexpr (sizeof(T) == 8) {
-    unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_bit_cast(T, v);
-  } else {
-    unsigned v = __hip_atomic_load(reinterpret_cast<const unsigned*>(p), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_bit_cast(T, v);
+__device__ __forceinline__ void put(u64* p, unsigned payload, unsigned tag) {
+  __hip_atomic_store(p, (static_cast<u64>(payload) << 32) | tag, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 get(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// waits for the word to carry `tag`; returns its payload
+__device__ __forceinline__ unsigned wait_word(const u64* p, unsigned tag) {
+  u64 v = get(p);
+  while (static_cast<unsigned>(v) != tag) {
+    __builtin_amdgcn_s_sleep(1);
+    v = get(p);
   }
+  return static_cast<unsigned>(v >> 32);
 }
 
 template <typename T>
-__device__ __forceinline__ void st_agent(T* p, T v) {
+__device__ __forceinline__ void put_elem(u64* p, T v, unsigned tag) {
   if constexpr (sizeof(T) == 8) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
-                       __builtin_bit_cast(unsigned long long, v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    const u64 b = __builtin_bit_cast(u64, v);
+    put(p, static_cast<unsigned>(b >> 32), tag);
+    put(p + 1, static_cast<unsigned>(b), tag);
   } else {
-    __hip_atomic_store(reinterpret_cast<unsigned*>(p), __builtin_bit_cast(unsigned, v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    put(p, __builtin_bit_cast(unsigned, v), tag);
+  }
+}
+template <typename T>
+__device__ __forceinline__ T wait_elem(const u64* p, unsigned tag) {
+  if constexpr (sizeof(T) == 8) {
+    const u64 hi = wait_word(p, tag);
+    const u64 lo = wait_word(p + 1, tag);
+    return __builtin_bit_cast(T, (hi << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, wait_word(p, tag));
   }
 }
 
@@ -112,13 +131,9 @@ __global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int nb, T* __re
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = w * R;
   const int nloc = max(0, min(R, m - r0));
-  unsigned* flags = sy.flags;
-  double* keys = sy.keys;
-  int* rowsv = sy.rows;
-  T* cand = static_cast<T*>(sy.cand);
-  T* diag = static_cast<T*>(sy.diag);
 
-  if (w == 0 && tid == 0) st_agent(sy.list_count, 0);
+  if (w == 0 && tid == 0)
+    __hip_atomic_store(sy.list_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // an earlier panel already hit a zero pivot: leave everything as it is (empty list)
   if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
@@ -129,11 +144,10 @@ __global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int nb, T* __re
   for (int r = tid; r < nloc; r += 256) tile[r * NBX + NB_MAX] = static_cast<T>(r0 + r);
   __syncthreads();
 
-  int stop = 0;
   for (int i = 0; i < nb; ++i) {
     const int par = i & 1;
-    const unsigned seq = base + static_cast<unsigned>(i) + 1u;
-    // 1. local candidate
+    const unsigned tag = base + static_cast<unsigned>(i) + 1u;
+    // 1. local candidate, published with the contents of its row
     double key = -2.0;
     int row = INT_MAX;
     if (tid < nloc && r0 + tid >= i) {
@@ -142,75 +156,63 @@ __global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int nb, T* __re
       row = r0 + tid;
     }
     block_argmax(key, row, sk, sr);
-    // 2. publish candidate row, diagonal row, key; raise flag
-    T* my_cand = cand + ((size_t)par * W + w) * NBX;
     if (key > -2.0) {
-      const int lr = row - r0;
-      for (int c = tid; c < NBX; c += 256) st_agent(my_cand + c, tile[lr * NBX + c]);
+      u64* my = sy.cand + ((size_t)par * WMAX + w) * NBX * 2;
+      const T* src = tile + (row - r0) * NBX;
+      for (int c = tid; c < NBX; c += 256) put_elem<T>(my + 2 * c, src[c], tag);
     }
     const bool own_i = (i >= r0 && i < r0 + nloc);
     if (own_i) {
+      u64* dg = sy.diag + (size_t)par * NBX * 2;
       for (int c = tid; c < NBX; c += 256) {
         const T v = tile[(i - r0) * NBX + c];
-        st_agent(diag + (size_t)par * NBX + c, v);
+        put_elem<T>(dg + 2 * c, v, tag);
         drow[c] = v;
       }
     }
     if (tid == 0) {
-      st_agent(keys + (size_t)par * W + w, key);
-      __hip_atomic_store(rowsv + (size_t)par * W + w, row, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      u64* kp = sy.keys + ((size_t)par * WMAX + w) * 3;
+      const u64 kb = __builtin_bit_cast(u64, key);
+      put(kp, static_cast<unsigned>(kb >> 32), tag);
+      put(kp + 1, static_cast<unsigned>(kb), tag);
+      put(kp + 2, static_cast<unsigned>(row), tag);
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0)
-      __hip_atomic_store(flags + (size_t)par * W + w, seq, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    // 3. wait for every workgroup's flag of this step
-    if (wave == 0) {
-      for (;;) {
-        bool ok = true;
-        for (int q = lane; q < W; q += 64)
-          ok = ok && (__hip_atomic_load(flags + (size_t)par * W + q, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT) == seq);
-        if (__all(ok)) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // 4. global winner (identical in every workgroup)
+    // 2. global winner (identical in every workgroup): lane q reads slot q
     key = -2.0;
     row = INT_MAX;
-    for (int q = tid; q < W; q += 256) {
-      const double k2 = ld_agent(keys + (size_t)par * W + q);
-      const int r2 = __hip_atomic_load(rowsv + (size_t)par * W + q, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-      better(key, row, k2, r2);
+    if (tid < W) {
+      const u64* kp = sy.keys + ((size_t)par * WMAX + tid) * 3;
+      const u64 hi = wait_word(kp, tag);
+      const u64 lo = wait_word(kp + 1, tag);
+      key = __builtin_bit_cast(double, (hi << 32) | lo);
+      row = static_cast<int>(wait_word(kp + 2, tag));
     }
     block_argmax(key, row, sk, sr);
     const int wwin = row / R;
-    for (int c = tid; c < NBX; c += 256)
-      prow[c] = ld_agent(cand + ((size_t)par * W + wwin) * NBX + c);
-    __syncthreads();
-    const T pv = prow[i];
-    if (w == 0 && tid == 0) piv[i] = row - i;
-    if (pv == static_cast<T>(0)) {
-      if (w == 0 && tid == 0) atomicCAS(info, 0, info_base + i + 1);
-      stop = 1;
-    }
-    // 5. interchange rows i and `row`
-    if (row != i) {
-      if (own_i)
-        for (int c = tid; c < NBX; c += 256) tile[(i - r0) * NBX + c] = prow[c];
-      if (row >= r0 && row < r0 + nloc) {
-        for (int c = tid; c < NBX; c += 256)
-          tile[(row - r0) * NBX + c] = own_i ? drow[c] : ld_agent(diag + (size_t)par * NBX + c);
+    const bool own_p = (row != i) && (row >= r0 && row < r0 + nloc);
+    {
+      const u64* cw = sy.cand + ((size_t)par * WMAX + wwin) * NBX * 2;
+      for (int c = tid; c < NBX; c += 256) prow[c] = wait_elem<T>(cw + 2 * c, tag);
+      if (own_p && !own_i) {
+        const u64* dg = sy.diag + (size_t)par * NBX * 2;
+        for (int c = tid; c < NBX; c += 256) drow[c] = wait_elem<T>(dg + 2 * c, tag);
       }
     }
     __syncthreads();
+    const T pv = prow[i];
+    if (w == 0 && tid == 0) piv[i] = row - i;
+    const bool stop = (pv == static_cast<T>(0));
+    if (stop && w == 0 && tid == 0) atomicCAS(info, 0, info_base + i + 1);
+    // 3. interchange rows i and `row`
+    if (row != i) {
+      if (own_i)
+        for (int c = tid; c < NBX; c += 256) tile[(i - r0) * NBX + c] = prow[c];
+      if (own_p)
+        for (int c = tid; c < NBX; c += 256) tile[(row - r0) * NBX + c] = drow[c];
+    }
+    __syncthreads();
     if (stop) break;
-    // 6. scale the column below the pivot and update the rest of the panel rows
+    // 4. scale the column below the pivot and update the rest of the panel rows
     const T rinv = static_cast<T>(1) / pv;
     for (int lr = wave; lr < nloc; lr += 4) {
       if (r0 + lr <= i) continue;
@@ -348,38 +350,34 @@ int max_coresident() {
   return cached;
 }
 
+template <typename T, int R>
+bool fits(int64_t m, int limit) {
+  const int64_t W = (m + R - 1) / R;
+  return W <= limit && W <= max_coresident<T, R>();
+}
+
 }  // namespace
 
-// exchange area for panel kernels (sizes for up to 1024 workgroups)
-int panel_sync(PanelSync* sy, int nb, unsigned** base_counter) {
-  constexpr int WMAX = 1024;
-  const size_t bytes = 2 * WMAX * sizeof(unsigned) + 2 * WMAX * sizeof(double) +
-                       2 * WMAX * sizeof(int) + 2 * (size_t)WMAX * NBX * sizeof(double) +
-                       2 * NBX * sizeof(double) + 4 * (size_t)NB_MAX * sizeof(int) + 256;
+// exchange area for panel kernels
+int panel_sync(PanelSync* sy, unsigned** base_counter) {
+  const size_t kb = 2 * (size_t)WMAX * 3 * sizeof(u64);
+  const size_t cb = 2 * (size_t)WMAX * NBX * 2 * sizeof(u64);
+  const size_t db = 2 * (size_t)NBX * 2 * sizeof(u64);
+  const size_t lb = 4 * (size_t)NB_MAX * sizeof(int) + 256;
+  const size_t bytes = kb + cb + db + lb;
   void* ws = nullptr;
-  static bool zeroed = false;
   static void* zeroed_ptr = nullptr;
   DDLA_RETURN_IF(workspace(WS_PANEL, bytes, &ws));
-  if (!zeroed || zeroed_ptr != ws) {
-    DDLA_HIP_CHECK(hipMemset(ws, 0, bytes));
-    zeroed = true;
+  if (zeroed_ptr != ws) {  // tags start from a clean slate once per allocation
+    DDLA_HIP_CHECK(hipMemsetAsync(ws, 0, bytes, stream()));
     zeroed_ptr = ws;
   }
   char* p = static_cast<char*>(ws);
-  sy->flags = reinterpret_cast<unsigned*>(p);
-  p += 2 * WMAX * sizeof(unsigned);
-  sy->keys = reinterpret_cast<double*>(p);
-  p += 2 * WMAX * sizeof(double);
-  sy->rows = reinterpret_cast<int*>(p);
-  p += 2 * WMAX * sizeof(int);
-  sy->cand = p;
-  p += 2 * (size_t)WMAX * NBX * sizeof(double);
-  sy->diag = p;
-  p += 2 * NBX * sizeof(double);
-  sy->list = reinterpret_cast<int*>(p);
-  p += 4 * (size_t)NB_MAX * sizeof(int);
-  sy->list_count = reinterpret_cast<int*>(p);
-  (void)nb;
+  sy->keys = reinterpret_cast<u64*>(p);
+  sy->cand = reinterpret_cast<u64*>(p + kb);
+  sy->diag = reinterpret_cast<u64*>(p + kb + cb);
+  sy->list = reinterpret_cast<int*>(p + kb + cb + db);
+  sy->list_count = sy->list + 4 * NB_MAX;
   static unsigned counter = 0;
   *base_counter = &counter;
   return 0;
@@ -388,36 +386,33 @@ int panel_sync(PanelSync* sy, int nb, unsigned** base_counter) {
 template <typename T>
 int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
                           int info_base, int* list_out, int* count_out, hipStream_t s) {
-  if (nb > NB_MAX || nb <= 0) {
-    set_error("panel width %d outside 1..%d", nb, NB_MAX);
+  if (nb > NB_MAX || nb <= 0 || m < nb) {
+    set_error("panel %lld x %d outside 1 <= nb <= min(m, %d)", (long long)m, nb, NB_MAX);
     return DDLA_ERR_INVALID_ARG;
   }
   PanelSync sy;
   unsigned* counter = nullptr;
-  DDLA_RETURN_IF(panel_sync(&sy, nb, &counter));
+  DDLA_RETURN_IF(panel_sync(&sy, &counter));
   if (list_out) {
     sy.list = list_out;
     sy.list_count = count_out;
   }
+  // step tags: every launch takes a fresh range, so stale words never match
   const unsigned base = *counter;
   *counter += static_cast<unsigned>(nb) + 2u;
-  const int W32 = ceil_div(m, 32), W64 = ceil_div(m, 64), W128 = ceil_div(m, 128);
+  if (*counter < base) *counter = 1;  // wrapped: restart (all words hold older tags)
   ProfScope prof(sizeof(T) == 8 ? "getf2_panel_f64" : "getf2_panel_f32",
                  (double)m * nb * nb - (double)nb * nb * nb / 3.0, 2.0 * sizeof(T) * m * nb, s);
-  int rc;
-  if (W32 <= 128 && W32 <= max_coresident<T, 32>())
-    rc = launch_panel_r<T, 32>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
-  else if (W64 <= 256 && W64 <= max_coresident<T, 64>())
-    rc = launch_panel_r<T, 64>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
-  else if (W128 <= 1024 && W128 <= max_coresident<T, 128>())
-    rc = launch_panel_r<T, 128>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
-  else {
-    set_error("panel of %lld rows exceeds the co-resident capacity", (long long)m);
-    return DDLA_ERR_UNSUPPORTED;
+  // fewest workgroups that hold the panel (fewer slots to read per step)
+  if (fits<T, 32>(m, 64)) return launch_panel_r<T, 32>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if (fits<T, 64>(m, 128)) return launch_panel_r<T, 64>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if (fits<T, 128>(m, WMAX)) return launch_panel_r<T, 128>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if constexpr (sizeof(T) == 4) {
+    if (fits<T, 256>(m, WMAX)) return launch_panel_r<T, 256>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
   }
-  return rc;
+  set_error("panel of %lld rows exceeds the co-resident capacity", (long long)m);
+  return DDLA_ERR_UNSUPPORTED;
 }
-
 
 template <typename T>
 int launch_laswp_list(T* A, int64_t ld, int64_t row0, int64_t ncols, int64_t skip_lo,
@@ -442,6 +437,12 @@ int launch_perm_compose(int n, int npanels, int nb, const int* lists, const int*
     set_error("permutation of %d rows does not fit in LDS", n);
     return DDLA_ERR_UNSUPPORTED;
   }
+  static bool attr = false;
+  if (!attr) {
+    DDLA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(perm_compose_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
   hipLaunchKernelGGL(perm_compose_kernel, dim3(1), dim3(256), sh, s, n, npanels, nb, lists, counts,
                      perm);
   DDLA_HIP_CHECK(hipGetLastError());
@@ -453,6 +454,12 @@ int launch_piv_to_perm(int64_t n, const int32_t* piv, int32_t* perm, hipStream_t
   if (sh > 160 * 1024) {
     set_error("permutation of %lld rows does not fit in LDS", (long long)n);
     return DDLA_ERR_UNSUPPORTED;
+  }
+  static bool attr = false;
+  if (!attr) {
+    DDLA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(piv_to_perm_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
   }
   hipLaunchKernelGGL(piv_to_perm_kernel, dim3(1), dim3(256), sh, s, (int)n, piv, perm);
   DDLA_HIP_CHECK(hipGetLastError());

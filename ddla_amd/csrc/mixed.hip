@@ -103,21 +103,25 @@ __global__ __launch_bounds__(256) void colblock_scale_kernel(int64_t k, int64_t 
 
 // C (FP32, m x n) = C + alpha * (A * B); A m x k FP32 (k contiguous); B k x n FP32,
 // n contiguous (B_KC = false) or k contiguous, B(k,j) at B[j*ldb + k] (B_KC = true).
-// 128 x 128 tile per workgroup, K step 32, operands rounded to FP16/BF16 in LDS.
-template <bool BF16, bool B_KC, int TRI>
-__global__ __launch_bounds__(256, 2) void hgemm_kernel(int M, int N, int K, float alpha,
-                                                       const float* __restrict__ A, int64_t lda,
-                                                       const float* __restrict__ B, int64_t ldb,
-                                                       float* __restrict__ C, int64_t ldc,
-                                                       const float* __restrict__ sa,
-                                                       const float* __restrict__ sb, int tiles_m,
-                                                       int tiles_n) {
+// 128 x 128 tile per workgroup, K step 32.  Each scaled FP32 operand x is split into
+// TERMS low-precision parts while it is staged into LDS: x = h + m + l with h = rn(x),
+// m = rn(x - h), l = rn(x - h - m).  The tile product sums the part products whose
+// combined weight reaches the FP32 level (TERMS=1: hh; 2: hh+hm+mh; 3: hh+hm+mh+hl+lh+mm),
+// all accumulated in FP32 by the matrix cores.
+template <bool BF16, int TERMS, bool B_KC, int TRI>
+__global__ __launch_bounds__(256) void hgemm_kernel(int M, int N, int K, float alpha,
+                                                    const float* __restrict__ A, int64_t lda,
+                                                    const float* __restrict__ B, int64_t ldb,
+                                                    float* __restrict__ C, int64_t ldc,
+                                                    const float* __restrict__ sa,
+                                                    const float* __restrict__ sb, int tiles_m,
+                                                    int tiles_n) {
   typedef LowP<BF16> LP;
   typedef typename LP::v8 v8;
   typedef typename LP::e E;
   constexpr int BM = 128, BN = 128, BK = 32, PK = BK + 8;  // padded rows: conflict-free b128 reads
-  __shared__ __attribute__((aligned(16))) E As[2][BM * PK];
-  __shared__ __attribute__((aligned(16))) E Bs[2][BN * PK];
+  __shared__ __attribute__((aligned(16))) E As[2][TERMS][BM * PK];
+  __shared__ __attribute__((aligned(16))) E Bs[2][TERMS][BN * PK];
 
   int tmi, tni;
   gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
@@ -168,18 +172,26 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int M, int N, int K, floa
       }
     }
   };
+  auto split = [&](const float (&x)[8], float sc, v8 (&parts)[TERMS]) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      float r = x[e] * sc;
+#pragma unroll
+      for (int t = 0; t < TERMS; ++t) {
+        const E h = static_cast<E>(r);
+        parts[t][e] = h;
+        r = r - static_cast<float>(h);
+      }
+    }
+  };
   auto store = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int v = tid + 256 * q;
-      v8 pa, pb;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        pa[e] = static_cast<E>(ra[q][e] * scA);
-        pb[e] = static_cast<E>(rb[q][e] * scB);
-      }
+      v8 pa[TERMS], pb[TERMS];
+      split(ra[q], scA, pa);
+      split(rb[q], scB, pb);
       const int arow = v >> 2, akg = (v & 3) * 8;
-      *reinterpret_cast<v8*>(&As[buf][arow * PK + akg]) = pa;
       int bcol, bkg;
       if (B_KC) {
         bcol = v >> 2;
@@ -188,7 +200,11 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int M, int N, int K, floa
         bcol = v & 127;
         bkg = (v >> 7) * 8;
       }
-      *reinterpret_cast<v8*>(&Bs[buf][bcol * PK + bkg]) = pb;
+#pragma unroll
+      for (int t = 0; t < TERMS; ++t) {
+        *reinterpret_cast<v8*>(&As[buf][t][arow * PK + akg]) = pa[t];
+        *reinterpret_cast<v8*>(&Bs[buf][t][bcol * PK + bkg]) = pb[t];
+      }
     }
   };
 
@@ -199,17 +215,32 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int M, int N, int K, floa
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) load((kt + 1) * BK);
-    v8 a[4], b[4];
+    v8 a[TERMS][4], b[TERMS][4];
+#pragma unroll
+    for (int t = 0; t < TERMS; ++t) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        a[t][i] = *reinterpret_cast<const v8*>(&As[cur][t][(wm * 64 + i * 16 + l16) * PK + lk * 8]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        b[t][j] = *reinterpret_cast<const v8*>(&Bs[cur][t][(wn * 64 + j * 16 + l16) * PK + lk * 8]);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      a[i] = *reinterpret_cast<const v8*>(&As[cur][(wm * 64 + i * 16 + l16) * PK + lk * 8]);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      b[j] = *reinterpret_cast<const v8*>(&Bs[cur][(wn * 64 + j * 16 + l16) * PK + lk * 8]);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = LP::mfma(a[i], b[j], acc[i][j]);
+      for (int j = 0; j < 4; ++j) {
+        // small products first
+        if (TERMS >= 3) {
+          acc[i][j] = LP::mfma(a[1][i], b[1][j], acc[i][j]);
+          acc[i][j] = LP::mfma(a[2][i], b[0][j], acc[i][j]);
+          acc[i][j] = LP::mfma(a[0][i], b[2][j], acc[i][j]);
+        }
+        if (TERMS >= 2) {
+          acc[i][j] = LP::mfma(a[1][i], b[0][j], acc[i][j]);
+          acc[i][j] = LP::mfma(a[0][i], b[1][j], acc[i][j]);
+        }
+        acc[i][j] = LP::mfma(a[0][i], b[0][j], acc[i][j]);
+      }
     if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
   }
@@ -350,9 +381,9 @@ struct Timer {
 }  // namespace
 
 // C += alpha * A * B with FP16/BF16 operands and FP32 accumulation (see hgemm_kernel).
-int launch_hgemm(bool bf16, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k, float alpha,
-                 const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
-                 float* scale_ws, hipStream_t s) {
+int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k,
+                 float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                 int64_t ldc, float* scale_ws, hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
   const int tm = ceil_div(m, 128), tn = ceil_div(n, 128);
   float* sa = scale_ws;
@@ -366,24 +397,25 @@ int launch_hgemm(bool bf16, bool b_kcontig, int tri, int64_t m, int64_t n, int64
   dim3 grid((unsigned)((int64_t)tm * tn)), block(256);
   const int M = (int)m, N = (int)n, K = (int)k;
   const double mn = (tri ? 0.5 : 1.0) * (double)m * n;
-  ProfScope prof(bf16 ? "hgemm_mfma_bf16" : "hgemm_mfma_f16", 2.0 * mn * k,
+  ProfScope prof(bf16 ? "hgemm_mfma_bf16" : "hgemm_mfma_f16", 2.0 * mn * k * std::max(1, terms * (terms + 1) / 2),
                  4.0 * ((double)m * k + (double)k * n + 2.0 * mn), s);
-#define HG(BF, KC, T)                                                                        \
-  hipLaunchKernelGGL((hgemm_kernel<BF, KC, T>), grid, block, 0, s, M, N, K, alpha, A, lda, B, \
-                     ldb, C, ldc, sa, sb, tm, tn)
+#define HG(BF, TM, KC, T)                                                                  \
+  hipLaunchKernelGGL((hgemm_kernel<BF, TM, KC, T>), grid, block, 0, s, M, N, K, alpha, A, lda, \
+                     B, ldb, C, ldc, sa, sb, tm, tn)
+#define HG_T(BF, TM)                                         \
+  do {                                                       \
+    if (b_kcontig) {                                         \
+      if (tri) HG(BF, TM, true, 1); else HG(BF, TM, true, 0); \
+    } else {                                                 \
+      if (tri) HG(BF, TM, false, 1); else HG(BF, TM, false, 0); \
+    }                                                        \
+  } while (0)
   if (bf16) {
-    if (b_kcontig) {
-      if (tri) HG(true, true, 1); else HG(true, true, 0);
-    } else {
-      if (tri) HG(true, false, 1); else HG(true, false, 0);
-    }
+    if (terms <= 1) HG_T(true, 1); else if (terms == 2) HG_T(true, 2); else HG_T(true, 3);
   } else {
-    if (b_kcontig) {
-      if (tri) HG(false, true, 1); else HG(false, true, 0);
-    } else {
-      if (tri) HG(false, false, 1); else HG(false, false, 0);
-    }
+    if (terms <= 1) HG_T(false, 1); else if (terms == 2) HG_T(false, 2); else HG_T(false, 3);
   }
+#undef HG_T
 #undef HG
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
@@ -395,6 +427,7 @@ namespace {
 // Workspace (WS_MIXED2): scales, perm, pivots, D (n x nrhs float), R (n x nrhs double),
 //                        row sums, norm words
 struct MixedWs {
+  int terms;  // low-precision parts per operand in the trailing updates
   float* W;
   float* scales;
   int* perm;
@@ -460,7 +493,7 @@ int sgetrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
         DDLA_RETURN_IF(launch_sgemm(rest, rest, b, -1.f, panel + b * lda, lda, true, panel + b, lda,
                                     true, 1.f, panel + b * lda + b, lda, 0, s));
       } else {
-        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, false, 0, rest, rest, b, -1.f,
+        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, w.terms, false, 0, rest, rest, b, -1.f,
                                     panel + b * lda, lda, panel + b, lda, panel + b * lda + b, lda,
                                     w.scales, s));
       }
@@ -493,7 +526,7 @@ int spotrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
         DDLA_RETURN_IF(launch_sgemm(rest, rest, b, -1.f, a21, lda, true, a21, lda, false, 1.f,
                                     a21 + b, lda, 1, s));
       } else {
-        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, true, 1, rest, rest, b, -1.f, a21, lda,
+        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, w.terms, true, 1, rest, rest, b, -1.f, a21, lda,
                                     a21, lda, a21 + b, lda, w.scales, s));
       }
     }
@@ -581,6 +614,9 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
   }
   MixedWs w;
   DDLA_RETURN_IF(mixed_ws(n, nrhs, &w));
+  // default split: FP16 x2 (22 bits) and BF16 x3 (24 bits) reach FP32 product accuracy
+  w.terms = (opts && opts->terms > 0) ? std::min(opts->terms, 3)
+                                      : (prec == DDLA_PREC_BF16 ? 3 : 2);
   double normA = 0.0;
   DDLA_RETURN_IF(norm_inf(n, a, lda, w, &normA));
   Timer t;

@@ -32,12 +32,14 @@ __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
   const int cw = (int)min<int64_t>(64, ncols - c0);
   // packed triangle: lower (r >= c) at r(r+1)/2 + c; upper (r <= c) stored as its
   // transpose, at c(c+1)/2 + r
-  for (int r = 0; r < bs; ++r) {
-    if (LOWER) {
-      for (int c = tid; c <= r; c += 256) Ls[r * (r + 1) / 2 + c] = A[r * rs + c * cs];
-    } else {
-      for (int c = tid; c <= r; c += 256) Ls[r * (r + 1) / 2 + c] = A[c * rs + r * cs];
-    }
+  // one flat loop over the packed index so every lane's loads are independent
+  const int npk = bs * (bs + 1) / 2;
+  for (int p = tid; p < npk; p += 256) {
+    int r = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
+    while (r * (r + 1) / 2 > p) --r;
+    while ((r + 1) * (r + 2) / 2 <= p) ++r;
+    const int c = p - r * (r + 1) / 2;
+    Ls[p] = LOWER ? A[r * rs + c * cs] : A[c * rs + r * cs];
   }
   for (int idx = tid; idx < bs * 64; idx += 256) {
     const int r = idx >> 6, cc = idx & 63;

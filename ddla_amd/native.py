@@ -30,7 +30,7 @@ cint = ctypes.c_int
 
 class MpirOpts(ctypes.Structure):
     _fields_ = [("factor_prec", i32), ("max_iters", i32), ("tol", ctypes.c_double),
-                ("gmres", i32), ("reserved", i32)]
+                ("gmres", i32), ("terms", i32)]
 
 
 class MpirInfo(ctypes.Structure):

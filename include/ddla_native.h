@@ -188,7 +188,9 @@ typedef struct ddla_mpir_opts {
   double tol;            /* target normwise backward error (default 1e-14 if <= 0) */
   int32_t gmres;         /* 0 = classic refinement only, 1 = switch to GMRES-IR if the
                             classic iteration stalls, 2 = GMRES-IR from the start */
-  int32_t reserved;
+  int32_t terms;         /* low-precision parts per FP32 operand in the trailing updates:
+                            1 = one FP16/BF16 rounding, 2 or 3 = split x = h + m (+ l) with
+                            the cross products summed; 0 = default (FP16: 2, BF16: 3) */
 } ddla_mpir_opts;
 
 typedef struct ddla_mpir_info {
