@@ -120,6 +120,7 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--prec", choices=["fp16", "bf16", "fp32"], default="fp16")
+    ap.add_argument("--pivoting", choices=["butterfly", "partial"], default="partial")
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
@@ -145,18 +146,34 @@ def main():
     X = ddla.DeviceMatrix.wrap(X_t.data_ptr(), n, nrhs, nrhs, X_t)
     torch.cuda.synchronize()
 
+    pivoting = {"butterfly": ddla.PIVOT_BUTTERFLY, "partial": ddla.PIVOT_PARTIAL}[args.pivoting]
+
+    def solve(piv):
+        return ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14, pivoting=piv)
+
+    # ---- the other pivoting choice of the mixed solver, reported next to the headline
+    other = ddla.PIVOT_PARTIAL if pivoting == ddla.PIVOT_BUTTERFLY else ddla.PIVOT_BUTTERFLY
+    solve(other)
+    ddla.synchronize()
+    t0 = time.perf_counter()
+    oinfo = solve(other)
+    ddla.synchronize()
+    t_other = time.perf_counter() - t0
+    log(f"mixed ({'partial' if other == ddla.PIVOT_PARTIAL else 'butterfly'}): "
+        f"{t_other * 1e3:.1f} ms status={oinfo.status} iters={oinfo.iters}")
+
     # ---- mixed precision LU + IR (the measured step)
     for w in range(args.warmup):
-        info = ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14)
+        info = solve(pivoting)
         log(f"warmup {w}: status={info.status} iters={info.iters} berr={info.backward_error:.2e} "
-            f"factor={info.factor_ms:.1f}ms refine={info.refine_ms:.1f}ms")
+            f"factor={info.factor_ms:.1f}ms refine={info.refine_ms:.1f}ms fallback={info.fallback}")
     ddla.profile_reset()
     ddla.profile_enable(True)
     ddla.synchronize()
     t0 = time.perf_counter()
     infos = []
     for k in range(args.steps):
-        infos.append(ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14))
+        infos.append(solve(pivoting))
     ddla.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
     ddla.profile_enable(False)
@@ -220,15 +237,19 @@ def main():
         "dtype": f"{args.prec}-mfma/fp32/fp64",
         "data": "synthetic",
         "config": {"workload": f"mixed-precision LU + iterative refinement, n={n}, kappa={args.kappa:g}, "
-                               f"{args.prec} trailing updates, nrhs={nrhs}",
+                               f"{args.prec} trailing updates, {args.pivoting} pivoting, nrhs={nrhs}",
                    "n": n, "kappa": args.kappa, "factor_prec": args.prec, "nrhs": nrhs,
-                   "tol": 1e-14},
+                   "pivoting": args.pivoting, "tol": 1e-14},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "mixed": {"iters": last.iters, "status": last.status,
+        "mixed": {"iters": last.iters, "status": last.status, "fallback": last.fallback,
                   "backward_error": be_mixed, "forward_error_vs_xtrue": fe_mixed,
                   "forward_error_vs_fp64_lu": fe_vs_fp64,
                   "factor_ms": last.factor_ms, "refine_ms": last.refine_ms},
+        "mixed_other_pivoting": {"pivoting": "partial" if other == ddla.PIVOT_PARTIAL else "butterfly",
+                                 "ms_per_solve": t_other * 1e3, "status": oinfo.status,
+                                 "iters": oinfo.iters, "backward_error": oinfo.backward_error,
+                                 "speedup_vs_fp64_lu": t_fp64 / t_other},
         "fp64_lu": {"ms_per_solve": t_fp64 * 1e3, "solves_per_s": 1.0 / t_fp64,
                     "backward_error": be_fp64,
                     "gflops": (2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs) / t_fp64 / 1e9},

@@ -268,6 +268,32 @@ def PLUSolve(A, pivs, B, trans=False):
                                     _ptr(A), *_strides(A), _ptr(piv), _ptr(B2), *_strides(B2)))
 
 
+def LUDecompose(A):
+    """LU without pivoting in place (LU.cs:133-142). A: numpy or DeviceMatrix."""
+    d = A if isinstance(A, DeviceMatrix) else DeviceMatrix.from_host(_as_f64(A))
+    if d.rows != d.cols:
+        raise ValueError("Matrix must be square for LU factorization.")
+    info = check(_lib().ddla_dgetrf_nopiv_dev(d.rows, d.ptr, d.ld))
+    if not isinstance(A, DeviceMatrix):
+        A[...] = d.to_host()
+    if info:
+        _zero_pivot(info)
+    return 0
+
+
+def LUSolve(A, B, trans=False):
+    """B <- A^-1 B (or A^-T B) from LU factors without pivoting (LU.cs:144-152)."""
+    dA = A if isinstance(A, DeviceMatrix) else DeviceMatrix.from_host(_as_f64(A))
+    dB = B if isinstance(B, DeviceMatrix) else DeviceMatrix.from_host(_as_f64(B))
+    if dB.rows != dA.rows:
+        raise ValueError("RHS matrix size mismatch")
+    check(_lib().ddla_dgetrs_nopiv_dev(TRANSPOSE if trans else NO_TRANSPOSE, dA.rows, dB.cols,
+                                       dA.ptr, dA.ld, dB.ptr, dB.ld))
+    if not isinstance(B, DeviceMatrix):
+        B[...] = dB.to_host().reshape(B.shape)
+    return 0
+
+
 def ApplyPiv(B: DeviceMatrix, pivs, side=LEFT, forward=True):
     """Row (left) or column (right) interchanges with relative pivots (LU.cs:237-249)."""
     npiv = len(pivs) if not isinstance(pivs, _DevInts) else pivs.n
@@ -393,9 +419,12 @@ class Cholesky:
         x = b if inplace else np.array(_as_f64(b), copy=True)
         CholeskySolve(self.uplo, self.matrix, x)
         return x
+PIVOT_PARTIAL, PIVOT_BUTTERFLY = 0, 1
+
+
 def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_iters, gmres,
-          terms):
-    opts = MpirOpts(prec, max_iters, tol, gmres, terms)
+          terms, pivoting=PIVOT_PARTIAL):
+    opts = MpirOpts(prec, max_iters, tol, gmres, terms, pivoting, 0)
     info = MpirInfo()
     st = fn(A.rows, B.cols, A.ptr, A.ld, B.ptr, B.ld, X.ptr, X.ld, ctypes.byref(opts),
             ctypes.byref(info))
@@ -405,14 +434,17 @@ def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_
 
 
 def gesv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16, tol=1e-14,
-              max_iters=50, gmres=1, terms=0) -> MpirInfo:
+              max_iters=50, gmres=0, terms=0, pivoting=PIVOT_PARTIAL) -> MpirInfo:
     """Mixed-precision LU + iterative refinement to FP64 accuracy (no reference counterpart).
 
-    terms: low-precision parts per operand of the trailing updates (0 = library default)."""
-    return _mpir(_lib().ddla_dgesv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms)
+    terms: low-precision parts per operand of the trailing updates (0 = library default).
+    pivoting: PIVOT_PARTIAL, or PIVOT_BUTTERFLY (random butterfly transform + LU without
+    pivoting, falling back to partial pivoting if it does not refine)."""
+    return _mpir(_lib().ddla_dgesv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms,
+                 pivoting)
 
 
 def posv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16, tol=1e-14,
-              max_iters=50, gmres=1, terms=0) -> MpirInfo:
+              max_iters=50, gmres=0, terms=0) -> MpirInfo:
     """Mixed-precision Cholesky + iterative refinement (no reference counterpart)."""
     return _mpir(_lib().ddla_dposv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms)

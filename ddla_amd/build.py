@@ -31,13 +31,22 @@ CFLAGS = [
 ]
 
 
+# Per-source flags.  The panel kernels keep up to 64 rows per lane in register arrays indexed
+# by unrolled constants; AMDGPU's promote-alloca pass would merge each array into one wide
+# vector value, and every uniform branch then copies the whole vector.  Left to SROA, the
+# elements stay independent registers.
+FILE_FLAGS = {
+    "lu_kernels.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
+}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

@@ -446,6 +446,24 @@ int ddla_dlaswp_dev(int side, int forward, ddla_dim_t m, ddla_dim_t n, double* x
   return 0;
 }
 
+int ddla_dgetrf_nopiv_dev(ddla_dim_t n, double* a, ddla_inc_t lda) {
+  API_INIT();
+  DDLA_ARG_CHECK(n >= 0 && lda >= n, "bad getrf arguments");
+  if (n == 0) return 0;
+  int32_t info = 0;
+  DDLA_RETURN_IF(dgetrf_nopiv_internal(n, a, lda, &info));
+  return info;
+}
+
+int ddla_dgetrs_nopiv_dev(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a,
+                          ddla_inc_t lda, double* b, ddla_inc_t ldb) {
+  API_INIT();
+  DDLA_ARG_CHECK(valid_trans(trans), "bad trans flag");
+  DDLA_ARG_CHECK(n >= 0 && nrhs >= 0 && lda >= n && ldb >= nrhs, "bad getrs arguments");
+  if (n == 0 || nrhs == 0) return 0;
+  return dgetrs_nopiv_internal(is_trans(trans), n, nrhs, a, lda, b, ldb);
+}
+
 int ddla_dpotrf_dev(int uplo, ddla_dim_t n, double* a, ddla_inc_t lda) {
   API_INIT();
   DDLA_ARG_CHECK(uplo == DDLA_LOWER || uplo == DDLA_UPPER,

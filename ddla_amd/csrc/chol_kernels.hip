@@ -47,9 +47,29 @@ __global__ __launch_bounds__(256) void potf2_kernel(int nb, T* __restrict__ A, i
     const T rinv = s_rinv;
     for (int r = i + 1 + tid; r < nb; r += 256) t[r * LD + i] *= rinv;
     __syncthreads();
-    for (int r = i + 1 + wave; r < nb; r += 4) {
-      const T xr = -t[r * LD + i];
-      for (int c = i + 1 + lane; c <= r; c += 64) t[r * LD + c] = fma(t[c * LD + i], xr, t[r * LD + c]);
+    // lower rank-1 update; lane -> columns c0, c1, wave -> rows wave + 4k, batches of rows
+    // whose LDS loads are issued before their stores
+    const int c0 = i + 1 + lane, c1 = c0 + 64;
+    const T w0 = c0 < nb ? t[c0 * LD + i] : T(0), w1 = c1 < nb ? t[c1 * LD + i] : T(0);
+    constexpr int CH = 8;
+    for (int k0 = 0; k0 < NB / 4; k0 += CH) {
+      T x[CH], a0[CH], a1[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const T* tr = t + (wave + 4 * (k0 + k)) * LD;
+        x[k] = tr[i];
+        a0[k] = tr[c0 < NB ? c0 : 0];
+        a1[k] = tr[c1 < NB ? c1 : 0];
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int r = wave + 4 * (k0 + k);
+        if (r > i && r < nb) {
+          T* tr = t + r * LD;
+          if (c0 <= r) tr[c0] = fma(w0, -x[k], a0[k]);
+          if (c1 <= r) tr[c1] = fma(w1, -x[k], a1[k]);
+        }
+      }
     }
     __syncthreads();
   }
@@ -59,8 +79,70 @@ __global__ __launch_bounds__(256) void potf2_kernel(int nb, T* __restrict__ A, i
   }
 }
 
-// B (nrows x nb) <- B * L^-T, L lower nb x nb (non-unit).  Row b of B solves L x^T = b^T.
+// LU without pivoting of one nb x nb block in LDS (LUDecUnblock, LU.cs:188-203): for each
+// column i, a21 *= 1/a11 (InvScal multiplies by the reciprocal) and A22 -= a21 a12^T.
+// Each step's rank-1 update issues the LDS loads of a batch of rows before its stores.
+// A zero pivot sets *info = info_base + i + 1 (first failure wins) and stops.
 template <typename T>
+__global__ __launch_bounds__(256) void getf2_nopiv_kernel(int nb, T* __restrict__ A, int64_t lda,
+                                                          int32_t* __restrict__ info,
+                                                          int info_base) {
+  __shared__ T t[NB * (NB + 1)];
+  constexpr int LD = NB + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int r = idx / nb, c = idx % nb;
+    t[r * LD + c] = A[(int64_t)r * lda + c];
+  }
+  __syncthreads();
+  bool stop = false;
+  for (int i = 0; i < nb; ++i) {
+    const T pv = t[i * LD + i];
+    if (pv == T(0)) {
+      if (tid == 0) atomicCAS(info, 0, info_base + i + 1);
+      stop = true;
+      break;
+    }
+    const T rinv = T(1) / pv;
+    const int c0 = i + 1 + lane, c1 = c0 + 64;
+    const bool h0 = c0 < nb, h1 = c1 < nb;
+    const T u0 = h0 ? t[i * LD + c0] : T(0), u1 = h1 ? t[i * LD + c1] : T(0);
+    constexpr int CH = 8;
+    for (int k0 = 0; k0 < NB / 4; k0 += CH) {
+      T l[CH], a0[CH], a1[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const T* tr = t + (wave + 4 * (k0 + k)) * LD;
+        l[k] = tr[i];
+        a0[k] = h0 ? tr[c0] : T(0);
+        a1[k] = h1 ? tr[c1] : T(0);
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int r = wave + 4 * (k0 + k);
+        if (r > i && r < nb) {
+          T* tr = t + r * LD;
+          const T lk = l[k] * rinv;
+          if (h0) tr[c0] = fma(-lk, u0, a0[k]);
+          if (h1) tr[c1] = fma(-lk, u1, a1[k]);
+          if (lane == 0) tr[i] = lk;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  (void)stop;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int r = idx / nb, c = idx % nb;
+    A[(int64_t)r * lda + c] = t[r * LD + c];
+  }
+}
+
+// B (nrows x nb) <- B * L^-T, L lower nb x nb (non-unit).  Row b of B solves L x^T = b^T.
+// TL: L is given by its transpose, i.e. L[r][c] is read from U[c][r] (B <- B * U^-1 for an
+// upper U stored at the same address).
+template <typename T, bool TL>
 __global__ __launch_bounds__(256) void trsm_rlt_kernel(int nb, int64_t nrows,
                                                        const T* __restrict__ L, int64_t ldl,
                                                        T* __restrict__ B, int64_t ldb) {
@@ -76,7 +158,7 @@ __global__ __launch_bounds__(256) void trsm_rlt_kernel(int nb, int64_t nrows,
     while (r * (r + 1) / 2 > p) --r;
     while ((r + 1) * (r + 2) / 2 <= p) ++r;
     const int c = p - r * (r + 1) / 2;
-    Ls[p] = L[(int64_t)r * ldl + c];
+    Ls[p] = TL ? L[(int64_t)c * ldl + r] : L[(int64_t)r * ldl + c];
   }
   for (int idx = tid; idx < 64 * nb; idx += 256) {
     const int r = idx / nb, c = idx % nb;
@@ -86,12 +168,28 @@ __global__ __launch_bounds__(256) void trsm_rlt_kernel(int nb, int64_t nrows,
   const int rr = tid & 63, g = tid >> 6;
   if (g == 0) S[rr] = S[rr] / Ls[0];
   __syncthreads();
+  // column c of the slab (c = g + 4k) is owned by group g; each step loads a batch of
+  // owned columns before storing them (one LDS round trip per batch)
   for (int i = 0; i < nb - 1; ++i) {
     const T x = S[i * SL + rr];
-    for (int c = i + 1 + g; c < nb; c += 4) {
-      T v = fma(-Ls[c * (c + 1) / 2 + i], x, S[c * SL + rr]);
-      if (c == i + 1) v = v / Ls[c * (c + 1) / 2 + c];
-      S[c * SL + rr] = v;
+    constexpr int CH = 8;
+    for (int k0 = 0; k0 < NB / 4; k0 += CH) {
+      T v[CH], l[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int c = g + 4 * (k0 + k);
+        v[k] = S[c * SL + rr];
+        l[k] = (c > i && c < nb) ? Ls[c * (c + 1) / 2 + i] : T(0);
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int c = g + 4 * (k0 + k);
+        if (c > i && c < nb) {
+          T w = fma(-l[k], x, v[k]);
+          if (c == i + 1) w = w / Ls[c * (c + 1) / 2 + c];
+          S[c * SL + rr] = w;
+        }
+      }
     }
     __syncthreads();
   }
@@ -142,7 +240,7 @@ int launch_potf2_ex(int nb, T* A, int64_t lda, int32_t* info, int info_base, hip
 
 template <typename T>
 int launch_trsm_rlt_panel(int nb, int64_t nrows, const T* L, int64_t ldl, T* B, int64_t ldb,
-                          hipStream_t s) {
+                          hipStream_t s, bool l_transposed) {
   if (nrows <= 0 || nb <= 0) return 0;
   if (nb > NB) {
     set_error("trsm block %d > %d", nb, NB);
@@ -150,8 +248,26 @@ int launch_trsm_rlt_panel(int nb, int64_t nrows, const T* L, int64_t ldl, T* B, 
   }
   ProfScope prof(sizeof(T) == 8 ? "trsm_rlt_f64" : "trsm_rlt_f32", (double)nb * nb * nrows,
                  sizeof(T) * (2.0 * nrows * nb + 0.5 * nb * nb), s);
-  hipLaunchKernelGGL(trsm_rlt_kernel<T>, dim3(ceil_div(nrows, 64)), dim3(256), 0, s, nb, nrows, L,
-                     ldl, B, ldb);
+  if (l_transposed)
+    hipLaunchKernelGGL((trsm_rlt_kernel<T, true>), dim3(ceil_div(nrows, 64)), dim3(256), 0, s, nb,
+                       nrows, L, ldl, B, ldb);
+  else
+    hipLaunchKernelGGL((trsm_rlt_kernel<T, false>), dim3(ceil_div(nrows, 64)), dim3(256), 0, s, nb,
+                       nrows, L, ldl, B, ldb);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_getf2_nopiv(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s) {
+  if (nb <= 0) return 0;
+  if (nb > NB) {
+    set_error("getf2 block %d > %d", nb, NB);
+    return DDLA_ERR_INVALID_ARG;
+  }
+  ProfScope prof(sizeof(T) == 8 ? "getf2_nopiv_f64" : "getf2_nopiv_f32",
+                 2.0 * nb * nb * nb / 3.0, 2.0 * sizeof(T) * nb * nb, s);
+  hipLaunchKernelGGL(getf2_nopiv_kernel<T>, dim3(1), dim3(256), 0, s, nb, A, lda, info, info_base);
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -167,8 +283,10 @@ int launch_dtranspose_inplace(int64_t n, double* a, int64_t lda, hipStream_t s) 
 template int launch_potf2_ex<double>(int, double*, int64_t, int32_t*, int, hipStream_t);
 template int launch_potf2_ex<float>(int, float*, int64_t, int32_t*, int, hipStream_t);
 template int launch_trsm_rlt_panel<double>(int, int64_t, const double*, int64_t, double*, int64_t,
-                                           hipStream_t);
+                                           hipStream_t, bool);
 template int launch_trsm_rlt_panel<float>(int, int64_t, const float*, int64_t, float*, int64_t,
-                                          hipStream_t);
+                                          hipStream_t, bool);
+template int launch_getf2_nopiv<double>(int, double*, int64_t, int32_t*, int, hipStream_t);
+template int launch_getf2_nopiv<float>(int, float*, int64_t, int32_t*, int, hipStream_t);
 
 }  // namespace ddla

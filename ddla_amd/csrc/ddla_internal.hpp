@@ -47,8 +47,8 @@ struct Context {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   // growable scratch buffers (device memory kept for the life of the context)
-  void* ws[8] = {};
-  size_t ws_bytes[8] = {};
+  void* ws[12] = {};
+  size_t ws_bytes[12] = {};
 };
 
 Context& ctx();
@@ -65,6 +65,8 @@ enum WorkspaceSlot {
   WS_REFINE = 5,  // refinement vectors
   WS_GMRES = 6,
   WS_SMALL = 7,   // small reduction outputs
+  WS_DINV = 8,    // inverted diagonal blocks for the FP64 solves
+  WS_COUNT = 12,
 };
 
 // Optional per-kernel timing (ddla_profile_*): while enabled, a ProfScope around a launch
@@ -160,12 +162,26 @@ template <typename T>
 int trsm_left(bool lower, bool unit, bool transA, int64_t n, int64_t nrhs, const T* A,
               int64_t lda, T* B, int64_t ldb, hipStream_t s);
 
+// persistent triangular solves with <= 8 right-hand sides (trsv.hip); lower/unit/trans
+// describe op(T) for trsv_prepare and (lower of op(T), trans) for trsv_solve
+size_t trsv_dinv_elems(int64_t n);
+template <typename T>
+int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64_t lda, T* Dinv,
+                 hipStream_t s);
+template <typename T>
+int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t lda, const T* Dinv,
+               T* B, int64_t ldb, hipStream_t s);
+
 // Cholesky pieces (chol_kernels.hip)
 template <typename T>
 int launch_potf2_ex(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s);
+// B <- B L^-T for lower L (l_transposed: B <- B U^-1 for the upper U stored at L)
 template <typename T>
 int launch_trsm_rlt_panel(int nb, int64_t nrows, const T* L, int64_t ldl, T* B, int64_t ldb,
-                          hipStream_t s);
+                          hipStream_t s, bool l_transposed);
+// LU without pivoting of one nb x nb diagonal block (nb <= 128)
+template <typename T>
+int launch_getf2_nopiv(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s);
 int launch_dtranspose_inplace(int64_t n, double* a, int64_t lda, hipStream_t s);
 
 // pivots -> permutation (perm[i] = source row of row i after all interchanges)
@@ -181,5 +197,8 @@ int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_
 int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host);
 int dpotrs_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda, double* b,
                     int64_t ldb);
+int dgetrf_nopiv_internal(int64_t n, double* a, int64_t lda, int32_t* info_host);
+int dgetrs_nopiv_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_t lda,
+                          double* b, int64_t ldb);
 
 }  // namespace ddla

@@ -16,6 +16,22 @@ namespace ddla {
 namespace {
 constexpr int NBLK = 128;  // LU.PLUBlockSize (LU.cs:235), Cholesky.BlockSize (Cholesky.cs:125)
 
+// op(A) X = B for the stored triangle `lower` of A: the persistent solver (trsv.hip) for up
+// to 8 right-hand sides, the blocked TRSM otherwise
+int tri_solve(bool lower, bool unit, bool trans, int64_t n, int64_t nrhs, const double* a,
+              int64_t lda, double* b, int64_t ldb) {
+  hipStream_t s = stream();
+  if (nrhs <= 8) {
+    void* ws = nullptr;
+    DDLA_RETURN_IF(workspace(WS_DINV, trsv_dinv_elems(n) * sizeof(double), &ws));
+    double* dinv = static_cast<double*>(ws);
+    const bool op_lower = trans ? !lower : lower;
+    DDLA_RETURN_IF(trsv_prepare<double>(op_lower, unit, trans, n, a, lda, dinv, s));
+    return trsv_solve<double>(op_lower, trans, n, (int)nrhs, a, lda, dinv, b, ldb, s);
+  }
+  return trsm_left<double>(lower, unit, trans, n, nrhs, a, lda, b, ldb, s);
+}
+
 int read_info(int32_t* dinfo, int32_t* info_host) {
   DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
   DDLA_HIP_CHECK(hipStreamSynchronize(stream()));
@@ -85,14 +101,50 @@ int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_
     // X = U^-1 L^-1 P B
     DDLA_RETURN_IF(launch_gather_rows<double>(n, nrhs, b, ldb, perm, tmp, nrhs, false, s));
     DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, tmp, nrhs, 1, b, ldb, s));
-    DDLA_RETURN_IF(trsm_left<double>(true, true, false, n, nrhs, a, lda, b, ldb, s));
-    DDLA_RETURN_IF(trsm_left<double>(false, false, false, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(tri_solve(true, true, false, n, nrhs, a, lda, b, ldb));
+    DDLA_RETURN_IF(tri_solve(false, false, false, n, nrhs, a, lda, b, ldb));
   } else {
     // A^T X = B: U^T V = B, L^T Y = V, X = P^-1 Y
-    DDLA_RETURN_IF(trsm_left<double>(false, false, true, n, nrhs, a, lda, b, ldb, s));
-    DDLA_RETURN_IF(trsm_left<double>(true, true, true, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(tri_solve(false, false, true, n, nrhs, a, lda, b, ldb));
+    DDLA_RETURN_IF(tri_solve(true, true, true, n, nrhs, a, lda, b, ldb));
     DDLA_RETURN_IF(launch_gather_rows<double>(n, nrhs, b, ldb, perm, tmp, nrhs, true, s));
     DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, tmp, nrhs, 1, b, ldb, s));
+  }
+  return 0;
+}
+
+// LUDecBlock (LU.cs:169-186): per 128-block, LU without pivoting of the diagonal block,
+// A12 <- L11^-1 A12, A21 <- A21 U11^-1, A22 <- A22 - A21 A12.
+int dgetrf_nopiv_internal(int64_t n, double* a, int64_t lda, int32_t* info_host) {
+  hipStream_t s = stream();
+  int32_t* dinfo;
+  int *counts, *lists;
+  DDLA_RETURN_IF(lu_bookkeeping(1, &dinfo, &counts, &lists));
+  DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
+  for (int64_t j = 0; j < n; j += NBLK) {
+    const int b = (int)std::min<int64_t>(NBLK, n - j);
+    double* a11 = a + j * lda + j;
+    DDLA_RETURN_IF(launch_getf2_nopiv<double>(b, a11, lda, dinfo, (int)j, s));
+    const int64_t rest = n - j - b;
+    if (rest > 0) {
+      DDLA_RETURN_IF(launch_trsm_slab<double>(true, true, b, rest, a11, lda, 1, a11 + b, lda, s));
+      DDLA_RETURN_IF(launch_trsm_rlt_panel<double>(b, rest, a11, lda, a11 + b * lda, lda, s, true));
+      DDLA_RETURN_IF(launch_dgemm(rest, rest, b, -1.0, a11 + b * lda, lda, true, a11 + b, lda, true,
+                                  1.0, a11 + b * lda + b, lda, 0, s));
+    }
+  }
+  return read_info(dinfo, info_host);
+}
+
+// LUSolveInternal (LU.cs:205-231)
+int dgetrs_nopiv_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_t lda,
+                          double* b, int64_t ldb) {
+  if (!trans) {
+    DDLA_RETURN_IF(tri_solve(true, true, false, n, nrhs, a, lda, b, ldb));
+    DDLA_RETURN_IF(tri_solve(false, false, false, n, nrhs, a, lda, b, ldb));
+  } else {
+    DDLA_RETURN_IF(tri_solve(false, false, true, n, nrhs, a, lda, b, ldb));
+    DDLA_RETURN_IF(tri_solve(true, true, true, n, nrhs, a, lda, b, ldb));
   }
   return 0;
 }
@@ -112,7 +164,7 @@ int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info
     if (rest > 0) {
       double* a21 = a11 + b * lda;
       // A21 = A21 * L11^-T (Cholesky.cs:156-157)
-      DDLA_RETURN_IF(launch_trsm_rlt_panel<double>(b, rest, a11, lda, a21, lda, s));
+      DDLA_RETURN_IF(launch_trsm_rlt_panel<double>(b, rest, a11, lda, a21, lda, s, false));
       // lower(A22) -= A21 * A21^T (Cholesky.cs:158)
       DDLA_RETURN_IF(launch_dgemm(rest, rest, b, -1.0, a21, lda, true, a21, lda, false, 1.0,
                                   a21 + b, lda, 1, s));
@@ -124,13 +176,12 @@ int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info
 
 int dpotrs_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda, double* b,
                     int64_t ldb) {
-  hipStream_t s = stream();
   if (lower) {
-    DDLA_RETURN_IF(trsm_left<double>(true, false, false, n, nrhs, a, lda, b, ldb, s));
-    DDLA_RETURN_IF(trsm_left<double>(true, false, true, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(tri_solve(true, false, false, n, nrhs, a, lda, b, ldb));
+    DDLA_RETURN_IF(tri_solve(true, false, true, n, nrhs, a, lda, b, ldb));
   } else {
-    DDLA_RETURN_IF(trsm_left<double>(false, false, true, n, nrhs, a, lda, b, ldb, s));
-    DDLA_RETURN_IF(trsm_left<double>(false, false, false, n, nrhs, a, lda, b, ldb, s));
+    DDLA_RETURN_IF(tri_solve(false, false, true, n, nrhs, a, lda, b, ldb));
+    DDLA_RETURN_IF(tri_solve(false, false, false, n, nrhs, a, lda, b, ldb));
   }
   return 0;
 }

@@ -5,7 +5,7 @@
 // (DDLA/Factorizations/LU.cs:289-429), AMax (DDLA/BLAS/1v.cs:30-41).
 //
 // Panel (getf2_panel_kernel): the m x nb panel is spread over W co-resident workgroups,
-// R rows each, held in LDS for the whole panel.  Partial pivoting needs one global
+// R rows each, held in registers for the whole panel.  Partial pivoting needs one global
 // arg-max per column; each column step is one exchange round between the workgroups:
 //   1. every workgroup reduces |a_ri| over its rows (first index wins ties, NaN is the
 //      smallest value, exactly AMax's rule) and publishes its candidate (key, row) and the
@@ -21,7 +21,7 @@
 // word until its tag is the current step's.  Slots alternate between two buffers by step
 // parity; a workgroup can only reach step i+2 after every workgroup published step i+1,
 // i.e. after every reader of step i finished, so a slot is never overwritten while read.
-// Every row carries one extra LDS column: the panel-relative index of the row whose
+// Every row carries one extra value: the panel-relative index of the row whose
 // trailing data belongs there.  At the end each workgroup emits (dst, src) pairs for
 // moved rows, so the interchanges for the rest of the matrix are one parallel gather
 // (laswp_list_kernel) instead of nb sequential swaps.
@@ -33,7 +33,7 @@ namespace ddla {
 
 struct PanelSync {
   // double-buffered exchange area (one workspace allocation), 64-bit tagged words
-  unsigned long long* keys;  // [2][WMAX][3]: key high, key low, row
+  unsigned long long* keys;  // [2][WMAX][KSTRIDE]: key high, key low, row (+ padding)
   unsigned long long* cand;  // [2][WMAX][NBX][2]: candidate row (2 words per element)
   unsigned long long* diag;  // [2][NBX][2]: diagonal row of the step
   int* list;                 // [2*nb][2] (dst, src) panel-relative
@@ -44,7 +44,8 @@ namespace {
 
 constexpr int NB_MAX = 128;
 constexpr int NBX = NB_MAX + 1;  // + origin column
-constexpr int WMAX = 256;        // one exchange slot per lane of a workgroup
+constexpr int WMAX = 256;        // at most one workgroup per CU
+constexpr int KSTRIDE = 16;      // one 128-byte line per key slot: readers spread over lines
 
 typedef unsigned long long u64;
 
@@ -53,27 +54,6 @@ __device__ __forceinline__ void better(double& k, int& r, double k2, int r2) {
     k = k2;
     r = r2;
   }
-}
-
-// all 256 lanes end with the same (key, row)
-__device__ __forceinline__ void block_argmax(double& key, int& row, double* sk, int* sr) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double k2 = __shfl_xor(key, off);
-    const int r2 = __shfl_xor(row, off);
-    better(key, row, k2, r2);
-  }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  __syncthreads();
-  if (lane == 0) {
-    sk[wave] = key;
-    sr[wave] = row;
-  }
-  __syncthreads();
-  key = sk[0];
-  row = sr[0];
-#pragma unroll
-  for (int w = 1; w < 4; ++w) better(key, row, sk[w], sr[w]);
 }
 
 __device__ __forceinline__ void put(u64* p, unsigned payload, unsigned tag) {
@@ -103,99 +83,357 @@ __device__ __forceinline__ void put_elem(u64* p, T v, unsigned tag) {
     put(p, __builtin_bit_cast(unsigned, v), tag);
   }
 }
+// Waits for one element (both words of an FP64 value are requested together, so the
+// common case costs a single round trip).
 template <typename T>
 __device__ __forceinline__ T wait_elem(const u64* p, unsigned tag) {
   if constexpr (sizeof(T) == 8) {
-    const u64 hi = wait_word(p, tag);
-    const u64 lo = wait_word(p + 1, tag);
-    return __builtin_bit_cast(T, (hi << 32) | lo);
+    u64 a = get(p), b = get(p + 1);
+    while (static_cast<unsigned>(a) != tag || static_cast<unsigned>(b) != tag) {
+      __builtin_amdgcn_s_sleep(1);
+      a = get(p);
+      b = get(p + 1);
+    }
+    return __builtin_bit_cast(T, ((a >> 32) << 32) | (b >> 32));
   } else {
     return __builtin_bit_cast(T, wait_word(p, tag));
   }
 }
 
-template <typename T, int R>
-__global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int nb, T* __restrict__ P,
+// Waits for two elements at once (pivot row and diagonal row of the same column).
+template <typename T>
+__device__ __forceinline__ void wait_elem2(const u64* p, const u64* q, unsigned tag, T& x, T& y) {
+  constexpr int NW = sizeof(T) == 8 ? 2 : 1;
+  u64 a[NW], b[NW];
+  for (;;) {
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      a[k] = get(p + k);
+      b[k] = get(q + k);
+    }
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NW; ++k)
+      ok = ok && static_cast<unsigned>(a[k]) == tag && static_cast<unsigned>(b[k]) == tag;
+    if (ok) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  if constexpr (sizeof(T) == 8) {
+    x = __builtin_bit_cast(T, ((a[0] >> 32) << 32) | (a[1] >> 32));
+    y = __builtin_bit_cast(T, ((b[0] >> 32) << 32) | (b[1] >> 32));
+  } else {
+    x = __builtin_bit_cast(T, static_cast<unsigned>(a[0] >> 32));
+    y = __builtin_bit_cast(T, static_cast<unsigned>(b[0] >> 32));
+  }
+}
+
+// Selects element k of a register array with a uniform run-time index (unrolled compares,
+// no dynamic register indexing).  The empty asm hides each element's origin: otherwise
+// the select chain over array loads folds into one load at a variable address, and the
+// whole array is demoted to scratch memory.
+template <typename X>
+__device__ __forceinline__ X opaque(X x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+template <typename X, int N>
+__device__ __forceinline__ X reg_get(const X (&v)[N], int k) {
+  X x = opaque(v[0]);
+#pragma unroll
+  for (int j = 1; j < N; ++j) x = (j == k) ? opaque(v[j]) : x;
+  return x;
+}
+template <typename X, int N>
+__device__ __forceinline__ void reg_set(X (&v)[N], int k, X x) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = opaque((j == k) ? x : v[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int lane) {
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  }
+}
+
+// The workgroup's R panel rows live in registers: local row lr = wave + NW k (k < R/NW) is
+// held by the 64 lanes of wave lr % NW, lane l holding columns l and l + 64; the row's origin sits in
+// LDS.  Column i of a wave's rows therefore sits in one lane: the rank-1 update takes each
+// row's multiplier with v_readlane, and every lane keeps the arg-max of its own column over
+// the rows it just updated, so the next column's local arg-max costs no extra pass.
+// Column c of a wave's rows (lane c & 63 of v0 or v1) goes to colb[0, RW) in LDS, so that the
+// arg-max over the rows is a lane-parallel reduction instead of a scan in one lane.
+template <typename T, int RW>
+__device__ __forceinline__ void column_to_lds(const T (&v0)[RW], const T (&v1)[RW], int c,
+                                              int lane, T* colb) {
+  if (lane == (c & 63)) {
+    if (c < 64) {
+#pragma unroll
+      for (int k = 0; k < RW; ++k) colb[k] = v0[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < RW; ++k) colb[k] = v1[k];
+    }
+  }
+}
+
+// (key, index) of a lane combined with the pair DPP brings from another lane: larger key wins,
+// the smaller index among equal keys
+template <int CTRL, int ROWMASK, typename T>
+__device__ __forceinline__ void dpp_better(T& a, int& k) {
+  T a2;
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, a);
+    const unsigned long long id = __builtin_bit_cast(unsigned long long, T(-2));
+    const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(id), static_cast<int>(b), CTRL,
+                                               ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(id >> 32),
+                                               static_cast<int>(b >> 32), CTRL, ROWMASK, 0xf,
+                                               false);
+    a2 = __builtin_bit_cast(T, (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                   static_cast<unsigned>(lo));
+  } else {
+    a2 = __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, T(-2)),
+                                                           __builtin_bit_cast(int, a), CTRL,
+                                                           ROWMASK, 0xf, false));
+  }
+  const int k2 = __builtin_amdgcn_update_dpp(INT_MAX, k, CTRL, ROWMASK, 0xf, false);
+  const bool b = a2 > a || (a2 == a && k2 < k);
+  a = b ? a2 : a;
+  k = b ? k2 : k;
+}
+
+// Arg-max of |colb[k]| over k in [kmin, kmax) across the wave: first index wins ties, a NaN
+// entry counts as -1 (below every number), -2 / INT_MAX when the range is empty.  Row of k:
+// gr0 + NW k.  Every lane ends with the result.
+template <typename T, int RW, int NW>
+__device__ __forceinline__ void wave_argmax(const T* colb, int kmin, int kmax, int gr0, int lane,
+                                            double& key, int& row) {
+  // the column was written by one lane of this wave: wait for the LDS writes (in-order
+  // LDS queue per wave), and keep the compiler from hoisting the reads above them
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  const T x = lane < RW ? colb[lane] : T(0);
+  T a = (lane >= kmin && lane < kmax) ? fmax(fabs(x), T(-1)) : T(-2);
+  int k = lane;
+  // Hillis-Steele over DPP row shifts, then the gfx9 row broadcasts: lane 63 ends with the
+  // whole wave's result.  Lanes whose source is outside the row (or whose row is masked off)
+  // receive the identity (-2, RW), which never wins.
+  dpp_better<0x111, 0xf>(a, k);  // row_shr:1
+  dpp_better<0x112, 0xf>(a, k);  // row_shr:2
+  dpp_better<0x114, 0xf>(a, k);  // row_shr:4
+  dpp_better<0x118, 0xf>(a, k);  // row_shr:8
+  dpp_better<0x142, 0xa>(a, k);  // row_bcast:15
+  dpp_better<0x143, 0xc>(a, k);  // row_bcast:31
+  a = read_lane(a, 63);
+  k = __builtin_amdgcn_readlane(k, 63);
+  key = static_cast<double>(a);
+  row = a > T(-2) ? gr0 + NW * k : INT_MAX;
+}
+
+// Step i of the rank-1 update on one wave's rows gr = gr0 + NW k: the multiplier of row gr is
+// its column-i entry (lane i & 63) times 1/pivot, stored in place; columns > i take
+// fma(-l, u, a) (GeR), columns < i keep their values (u is zero there, so the fma returns
+// them unchanged).  Column i + 1 of the rows then goes to LDS for the next arg-max.  Branch-free: a row outside (i, m) takes multiplier 0 and keeps column i.
+// HI: column i is in the upper half (v1).  The work is laid out in passes over all rows
+// (multipliers, their broadcast through v_readlane, the update) so that the rows'
+// independent latency chains overlap instead of running one row after the other.
+template <typename T, int RW, int NW, bool HI>
+__device__ __forceinline__ void panel_update(T (&v0)[RW], T (&v1)[RW], T* colb, int i,
+                                             int gr0, int m, T rinv, T u0, T u1, int lane) {
+  // rows k in [kmin, kmax) lie in (i, m)
+  // (a negative numerator truncates to a value <= 0, which the clamp turns into 0)
+  const int kmin = min(max((i + NW - gr0) / NW, 0), RW);
+  const int kmax = min(max((m + NW - 1 - gr0) / NW, 0), RW);
+  const int cl = HI ? i - 64 : i;  // lane of column i
+  T l[RW];
+#pragma unroll
+  for (int k = 0; k < RW; ++k) l[k] = (HI ? v1[k] : v0[k]) * rinv;
+#pragma unroll
+  for (int k = 0; k < RW; ++k) l[k] = read_lane(l[k], cl);
+#pragma unroll
+  for (int k = 0; k < RW; ++k) l[k] = (k >= kmin && k < kmax) ? l[k] : T(0);
+  const bool isl = lane == cl;
+  if constexpr (!HI) {
+    const T z0 = lane > i ? u0 : T(0);
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const T n0 = fma(-l[k], z0, v0[k]);
+      v1[k] = fma(-l[k], u1, v1[k]);  // every column >= 64 > i
+      v0[k] = (isl && k >= kmin && k < kmax) ? l[k] : n0;
+    }
+  } else {
+    const T z1 = lane > cl ? u1 : T(0);
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const T n1 = fma(-l[k], z1, v1[k]);
+      v1[k] = (isl && k >= kmin && k < kmax) ? l[k] : n1;
+    }
+  }
+  column_to_lds(v0, v1, i + 1, lane, colb);
+}
+
+template <typename T, int R, int NW>
+__global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* __restrict__ P,
                                                           int64_t ld, int32_t* __restrict__ piv,
                                                           int32_t* __restrict__ info,
                                                           PanelSync sy, unsigned base,
                                                           int info_base) {
-  __shared__ T tile[R * NBX + 2 * NBX];
-  __shared__ double sk[4];
-  __shared__ int sr[4];
-  T* prow = tile + R * NBX;
-  T* drow = prow + NBX;
+  constexpr int RW = R / NW;  // rows per wave
+  constexpr int NT = NW * 64;
+  __shared__ T prow[NBX];
+  __shared__ T drow[NBX];
+  __shared__ int orgs[R];
+  __shared__ T colb[NW][RW];  // per wave: the next column of its rows
+  // [0]: per-wave local arg-max, [1]: per-wave share of the global winner (separate slots:
+  // a fast wave may publish its global share while a slow one still reads the local ones)
+  __shared__ double sk[2][NW];
+  __shared__ int sr[2][NW];
 
   const int W = gridDim.x;
   const int w = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int r0 = w * R;
-  const int nloc = max(0, min(R, m - r0));
 
-  if (w == 0 && tid == 0)
+  if (w == 0 && tid == 0) {
+    // ordered before step 0's publication, which every workgroup waits for before its
+    // final atomicAdd on the counter
     __hip_atomic_store(sy.list_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_thread_fence(__ATOMIC_RELEASE);
+  }
   // an earlier panel already hit a zero pivot: leave everything as it is (empty list)
   if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
-  for (int idx = tid; idx < nloc * nb; idx += 256) {
-    const int r = idx / nb, c = idx % nb;
-    tile[r * NBX + c] = P[(int64_t)(r0 + r) * ld + c];
+  T v0[RW], v1[RW];
+  const int c0 = lane, c1 = lane + 64;
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int gr = r0 + wave + NW * k;
+    const bool ok = gr < m;
+    v0[k] = (ok && c0 < nb) ? P[(int64_t)gr * ld + c0] : T(0);
+    v1[k] = (ok && c1 < nb) ? P[(int64_t)gr * ld + c1] : T(0);
   }
-  for (int r = tid; r < nloc; r += 256) tile[r * NBX + NB_MAX] = static_cast<T>(r0 + r);
-  __syncthreads();
+  column_to_lds(v0, v1, 0, lane, colb[wave]);
+  const int gr0 = r0 + wave;
+  const int kmax = min(max((m + NW - 1 - gr0) / NW, 0), RW);  // rows k < kmax lie below m
+  for (int t = tid; t < R; t += NT) orgs[t] = r0 + t;
 
   for (int i = 0; i < nb; ++i) {
     const int par = i & 1;
     const unsigned tag = base + static_cast<unsigned>(i) + 1u;
-    // 1. local candidate, published with the contents of its row
-    double key = -2.0;
-    int row = INT_MAX;
-    if (tid < nloc && r0 + tid >= i) {
-      const double v = fabs(static_cast<double>(tile[tid * NBX + i]));
-      key = (v != v) ? -1.0 : v;
-      row = r0 + tid;
-    }
-    block_argmax(key, row, sk, sr);
-    if (key > -2.0) {
-      u64* my = sy.cand + ((size_t)par * WMAX + w) * NBX * 2;
-      const T* src = tile + (row - r0) * NBX;
-      for (int c = tid; c < NBX; c += 256) put_elem<T>(my + 2 * c, src[c], tag);
-    }
-    const bool own_i = (i >= r0 && i < r0 + nloc);
-    if (own_i) {
-      u64* dg = sy.diag + (size_t)par * NBX * 2;
-      for (int c = tid; c < NBX; c += 256) {
-        const T v = tile[(i - r0) * NBX + c];
-        put_elem<T>(dg + 2 * c, v, tag);
-        drow[c] = v;
+    const int ci = i & 63;
+    const bool hi = i >= 64;
+    // 1. local arg-max of column i over the rows >= i: per wave, then across waves
+    double key;
+    int row;
+    {
+      const int kmin = min(max((i + NW - 1 - gr0) / NW, 0), RW);  // rows k >= kmin: gr >= i
+      wave_argmax<T, RW, NW>(colb[wave], kmin, kmax, gr0, lane, key, row);
+      if (lane == 0) {
+        sk[0][wave] = key;
+        sr[0][wave] = row;
       }
     }
+    __syncthreads();
+    key = sk[0][0];
+    row = sr[0][0];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) better(key, row, sk[0][q], sr[0][q]);
+    row = __builtin_amdgcn_readfirstlane(row);
+    // 2. publish the candidate row, the diagonal row and the key
+    if (key > -2.0 && wave == (row - r0) % NW) {
+      const int ks = (row - r0) / NW;
+      u64* my = sy.cand + ((size_t)par * WMAX + w) * NBX * 2;
+      put_elem<T>(my + 2 * c0, reg_get(v0, ks), tag);
+      put_elem<T>(my + 2 * c1, reg_get(v1, ks), tag);
+      if (lane == 0) put_elem<T>(my + 2 * NB_MAX, static_cast<T>(orgs[row - r0]), tag);
+    }
+    const bool own_i = (i >= r0 && i < r0 + R);
+    if (own_i && wave == (i - r0) % NW) {
+      const int kd = (i - r0) / NW;
+      u64* dg = sy.diag + (size_t)par * NBX * 2;
+      const T d0 = reg_get(v0, kd), d1 = reg_get(v1, kd);
+      const T dor = static_cast<T>(orgs[i - r0]);
+      put_elem<T>(dg + 2 * c0, d0, tag);
+      put_elem<T>(dg + 2 * c1, d1, tag);
+      if (lane == 0) put_elem<T>(dg + 2 * NB_MAX, dor, tag);
+      drow[c0] = d0;
+      drow[c1] = d1;
+      if (lane == 0) drow[NB_MAX] = dor;
+    }
     if (tid == 0) {
-      u64* kp = sy.keys + ((size_t)par * WMAX + w) * 3;
+      u64* kp = sy.keys + ((size_t)par * WMAX + w) * KSTRIDE;
       const u64 kb = __builtin_bit_cast(u64, key);
       put(kp, static_cast<unsigned>(kb >> 32), tag);
       put(kp + 1, static_cast<unsigned>(kb), tag);
       put(kp + 2, static_cast<unsigned>(row), tag);
     }
-    // 2. global winner (identical in every workgroup): lane q reads slot q
-    key = -2.0;
-    row = INT_MAX;
-    if (tid < W) {
-      const u64* kp = sy.keys + ((size_t)par * WMAX + tid) * 3;
-      const u64 hi = wait_word(kp, tag);
-      const u64 lo = wait_word(kp + 1, tag);
-      key = __builtin_bit_cast(double, (hi << 32) | lo);
-      row = static_cast<int>(wait_word(kp + 2, tag));
+    // 3. global winner (identical in every workgroup): wave 0 reads the W slots, lane q the
+    //    slots q, q + 64, ...; all requests of a lane are in flight together
+    if (wave == 0) {
+      key = -2.0;
+      row = INT_MAX;
+      constexpr int SPL = WMAX / 64;
+      u64 a[SPL], b[SPL], c[SPL];
+      for (;;) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) {
+          const int q = lane + 64 * j;
+          if (q < W) {
+            const u64* kp = sy.keys + ((size_t)par * WMAX + q) * KSTRIDE;
+            a[j] = get(kp);
+            b[j] = get(kp + 1);
+            c[j] = get(kp + 2);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) {
+          if (lane + 64 * j < W)
+            ok = ok && static_cast<unsigned>(a[j]) == tag && static_cast<unsigned>(b[j]) == tag &&
+                 static_cast<unsigned>(c[j]) == tag;
+        }
+        if (ok) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+#pragma unroll
+      for (int j = 0; j < SPL; ++j) {
+        if (lane + 64 * j < W)
+          better(key, row, __builtin_bit_cast(double, ((a[j] >> 32) << 32) | (b[j] >> 32)),
+                 static_cast<int>(c[j] >> 32));
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double k2 = __shfl_xor(key, off);
+        const int r2 = __shfl_xor(row, off);
+        better(key, row, k2, r2);
+      }
+      if (lane == 0) {
+        sk[1][0] = key;
+        sr[1][0] = row;
+      }
     }
-    block_argmax(key, row, sk, sr);
+    __syncthreads();  // also: every lane is done with last step's prow / drow
+    key = sk[1][0];
+    row = sr[1][0];
+    row = __builtin_amdgcn_readfirstlane(row);
     const int wwin = row / R;
-    const bool own_p = (row != i) && (row >= r0 && row < r0 + nloc);
+    const bool own_p = (row != i) && (row >= r0 && row < r0 + R);
     {
       const u64* cw = sy.cand + ((size_t)par * WMAX + wwin) * NBX * 2;
-      for (int c = tid; c < NBX; c += 256) prow[c] = wait_elem<T>(cw + 2 * c, tag);
-      if (own_p && !own_i) {
-        const u64* dg = sy.diag + (size_t)par * NBX * 2;
-        for (int c = tid; c < NBX; c += 256) drow[c] = wait_elem<T>(dg + 2 * c, tag);
+      const u64* dg = sy.diag + (size_t)par * NBX * 2;
+      for (int c = tid; c < NBX; c += NT) {
+        if (own_p && !own_i) {
+          wait_elem2<T>(cw + 2 * c, dg + 2 * c, tag, prow[c], drow[c]);
+        } else {
+          prow[c] = wait_elem<T>(cw + 2 * c, tag);
+        }
       }
     }
     __syncthreads();
@@ -203,38 +441,48 @@ __global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int nb, T* __re
     if (w == 0 && tid == 0) piv[i] = row - i;
     const bool stop = (pv == static_cast<T>(0));
     if (stop && w == 0 && tid == 0) atomicCAS(info, 0, info_base + i + 1);
-    // 3. interchange rows i and `row`
+    // 4. interchange rows i and `row` (register rows, selected with uniform indices)
+    const T u0 = prow[c0], u1 = prow[c1];
     if (row != i) {
-      if (own_i)
-        for (int c = tid; c < NBX; c += 256) tile[(i - r0) * NBX + c] = prow[c];
-      if (own_p)
-        for (int c = tid; c < NBX; c += 256) tile[(row - r0) * NBX + c] = drow[c];
+      if (own_i && wave == (i - r0) % NW) {
+        const int kd = (i - r0) / NW;
+        reg_set(v0, kd, u0);
+        reg_set(v1, kd, u1);
+        if (lane == 0) orgs[i - r0] = static_cast<int>(prow[NB_MAX]);
+      }
+      if (own_p && wave == (row - r0) % NW) {
+        const int kp = (row - r0) / NW;
+        reg_set(v0, kp, drow[c0]);
+        reg_set(v1, kp, drow[c1]);
+        if (lane == 0) orgs[row - r0] = static_cast<int>(drow[NB_MAX]);
+      }
     }
-    __syncthreads();
     if (stop) break;
-    // 4. scale the column below the pivot and update the rest of the panel rows
+    // 5. scale column i below the pivot, update the rest of every row and track the
+    //    arg-max of column i + 1 over the updated rows
     const T rinv = static_cast<T>(1) / pv;
-    for (int lr = wave; lr < nloc; lr += 4) {
-      if (r0 + lr <= i) continue;
-      T* tr = tile + lr * NBX;
-      const T l = tr[i] * rinv;
-      for (int c = i + 1 + lane; c < nb; c += 64) tr[c] = fma(-l, prow[c], tr[c]);
-      if (lane == 0) tr[i] = l;
+    if (!hi) {
+      panel_update<T, RW, NW, false>(v0, v1, colb[wave], i, gr0, m, rinv, u0, u1, lane);
+    } else {
+      panel_update<T, RW, NW, true>(v0, v1, colb[wave], i, gr0, m, rinv, u0, u1, lane);
     }
-    __syncthreads();
   }
 
   // write back and emit the row movement list (dst <- src, panel-relative)
-  for (int idx = tid; idx < nloc * nb; idx += 256) {
-    const int r = idx / nb, c = idx % nb;
-    P[(int64_t)(r0 + r) * ld + c] = tile[r * NBX + c];
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int gr = r0 + wave + NW * k;
+    if (gr < m) {
+      if (c0 < nb) P[(int64_t)gr * ld + c0] = v0[k];
+      if (c1 < nb) P[(int64_t)gr * ld + c1] = v1[k];
+    }
   }
-  for (int r = tid; r < nloc; r += 256) {
-    const int src = static_cast<int>(tile[r * NBX + NB_MAX]);
-    if (src != r0 + r) {
+  __syncthreads();  // orgs written by the interchanges of the last step
+  for (int t = tid; t < R; t += NT) {
+    if (r0 + t < m && orgs[t] != r0 + t) {
       const int slot = atomicAdd(sy.list_count, 1);
-      sy.list[2 * slot] = r0 + r;
-      sy.list[2 * slot + 1] = src;
+      sy.list[2 * slot] = r0 + t;
+      sy.list[2 * slot + 1] = orgs[t];
     }
   }
 }
@@ -328,13 +576,26 @@ int grid_for(int64_t total) {
   return static_cast<int>(g);
 }
 
+// Waves per panel workgroup for R rows of T.  The per-step update is issue-bound on each SIMD,
+// so two waves per SIMD (8 per workgroup) overlap one wave's latency chains with the other's
+// work; more waves only lengthen the barriers (measured at m = 2048..16384, R = 64..256).
+template <typename T, int R>
+constexpr int panel_waves() {
+  return R >= 64 ? 8 : 4;
+}
+
+template <typename T, int R>
+const void* panel_kernel() {
+  return reinterpret_cast<const void*>(getf2_panel_kernel<T, R, panel_waves<T, R>()>);
+}
+
 template <typename T, int R>
 int launch_panel_r(int m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
                    const PanelSync& sy, unsigned base, int info_base, hipStream_t s) {
   const int W = ceil_div(m, R);
   void* args[] = {&m, &nb, &P, &ld, &piv, &info, const_cast<PanelSync*>(&sy), &base, &info_base};
-  DDLA_HIP_CHECK(hipLaunchCooperativeKernel(
-      reinterpret_cast<const void*>(getf2_panel_kernel<T, R>), dim3(W), dim3(256), args, 0, s));
+  DDLA_HIP_CHECK(hipLaunchCooperativeKernel(panel_kernel<T, R>(), dim3(W),
+                                            dim3(64 * panel_waves<T, R>()), args, 0, s));
   return 0;
 }
 
@@ -344,7 +605,7 @@ int max_coresident() {
   if (cached < 0) {
     int per_cu = 0;
     DDLA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(getf2_panel_kernel<T, R>), 256, 0));
+        &per_cu, panel_kernel<T, R>(), 64 * panel_waves<T, R>(), 0));
     cached = per_cu * ctx().num_cus;
   }
   return cached;
@@ -360,7 +621,7 @@ bool fits(int64_t m, int limit) {
 
 // exchange area for panel kernels
 int panel_sync(PanelSync* sy, unsigned** base_counter) {
-  const size_t kb = 2 * (size_t)WMAX * 3 * sizeof(u64);
+  const size_t kb = 2 * (size_t)WMAX * KSTRIDE * sizeof(u64);
   const size_t cb = 2 * (size_t)WMAX * NBX * 2 * sizeof(u64);
   const size_t db = 2 * (size_t)NBX * 2 * sizeof(u64);
   const size_t lb = 4 * (size_t)NB_MAX * sizeof(int) + 256;

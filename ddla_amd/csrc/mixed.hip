@@ -358,6 +358,121 @@ int grid_for(int64_t total) {
   return static_cast<int>(g);
 }
 
+// ------------------------------------------------ random butterfly transforms (depth 2) ---
+// A_r = U^T A V with U = diag(B1, B2) Bn: Bn is a size-np butterfly (1/sqrt2)[R0 R1; R0 -R1],
+// B1 and B2 size-np/2 butterflies; the diagonals are exp(u/10), u uniform in [-1/2, 1/2)
+// (Parker's recursive butterflies as used for pivot-free LU by Baboulin, Dongarra,
+// Herrmann and Tomov, ACM TOMS 39(2), 2013).  Index groups g(i0) = {i0, i0+np/4, i0+np/2,
+// i0+3np/4} are closed under U, so every transform is a 4 x 4 matrix per group.
+// R layout per transform: a[np/2] b[np/2] c[np/4] d[np/4] e[np/4] f[np/4].
+__global__ void rbt_diag_kernel(int64_t count, uint64_t seed, float* __restrict__ R) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + static_cast<uint64_t>(t + 1) * 0x9E3779B97F4A7C15ull;
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = static_cast<double>(z >> 11) * 0x1.0p-53 - 0.5;
+    R[t] = static_cast<float>(exp(u / 10.0));
+  }
+}
+
+// Ug: rows/columns ordered as the group g(i0)
+__device__ __forceinline__ void rbt_group(const float* __restrict__ R, int64_t np, int64_t i0,
+                                          double M[4][4]) {
+  const int64_t n2 = np / 2, n4 = np / 4;
+  const double a0 = R[i0], a1 = R[i0 + n4], b0 = R[n2 + i0], b1 = R[n2 + i0 + n4];
+  const double c = R[np + i0], d = R[np + n4 + i0], e = R[np + 2 * n4 + i0],
+               f = R[np + 3 * n4 + i0];
+  M[0][0] = 0.5 * c * a0; M[0][1] = 0.5 * d * a1;  M[0][2] = 0.5 * c * b0;  M[0][3] = 0.5 * d * b1;
+  M[1][0] = 0.5 * c * a0; M[1][1] = -0.5 * d * a1; M[1][2] = 0.5 * c * b0;  M[1][3] = -0.5 * d * b1;
+  M[2][0] = 0.5 * e * a0; M[2][1] = 0.5 * f * a1;  M[2][2] = -0.5 * e * b0; M[2][3] = -0.5 * f * b1;
+  M[3][0] = 0.5 * e * a0; M[3][1] = -0.5 * f * a1; M[3][2] = -0.5 * e * b0; M[3][3] = 0.5 * f * b1;
+}
+
+// W (np x np FP32) = U^T [A 0; 0 I] V, A n x n FP64
+__global__ __launch_bounds__(256) void rbt_matrix_kernel(int64_t n, int64_t np,
+                                                         const double* __restrict__ A, int64_t lda,
+                                                         const float* __restrict__ RU,
+                                                         const float* __restrict__ RV,
+                                                         float* __restrict__ W) {
+  const int64_t n4 = np / 4;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n4 * n4;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i0 = t / n4, j0 = t % n4;
+    double U[4][4], V[4][4], a[4][4], t1[4][4];
+    rbt_group(RU, np, i0, U);
+    rbt_group(RV, np, j0, V);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t gi = i0 + k * n4, gj = j0 + q * n4;
+        a[k][q] = (gi < n && gj < n) ? A[gi * lda + gj] : (gi == gj ? 1.0 : 0.0);
+      }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        t1[p][q] = U[0][p] * a[0][q] + U[1][p] * a[1][q] + U[2][p] * a[2][q] + U[3][p] * a[3][q];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double v = t1[p][0] * V[0][q] + t1[p][1] * V[1][q] + t1[p][2] * V[2][q] +
+                         t1[p][3] * V[3][q];
+        W[(i0 + p * n4) * np + j0 + q * n4] = static_cast<float>(v);
+      }
+  }
+}
+
+// D (np x nrhs FP32) = U^T [R; 0], R n x nrhs FP64 (contiguous)
+__global__ void rbt_vec_kernel(int64_t n, int64_t np, int64_t nrhs, const double* __restrict__ Rv,
+                               const float* __restrict__ RU, float* __restrict__ D) {
+  const int64_t n4 = np / 4;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n4 * nrhs;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i0 = t / nrhs, c = t % nrhs;
+    double U[4][4], r[4];
+    rbt_group(RU, np, i0, U);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t gi = i0 + k * n4;
+      r[k] = gi < n ? Rv[gi * nrhs + c] : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      D[(i0 + p * n4) * nrhs + c] =
+          static_cast<float>(U[0][p] * r[0] + U[1][p] * r[1] + U[2][p] * r[2] + U[3][p] * r[3]);
+  }
+}
+
+// X[:n] += (V D)[:n] in FP64; R = B
+__global__ void rbt_update_kernel(int64_t n, int64_t np, int64_t nrhs, const float* __restrict__ D,
+                                  const float* __restrict__ RV, double* __restrict__ X,
+                                  int64_t ldx, const double* __restrict__ B, int64_t ldb,
+                                  double* __restrict__ Rv) {
+  const int64_t n4 = np / 4;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n4 * nrhs;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i0 = t / nrhs, c = t % nrhs;
+    double V[4][4], d[4];
+    rbt_group(RV, np, i0, V);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[q] = static_cast<double>(D[(i0 + q * n4) * nrhs + c]);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int64_t gi = i0 + p * n4;
+      if (gi < n) {
+        X[gi * ldx + c] += V[p][0] * d[0] + V[p][1] * d[1] + V[p][2] * d[2] + V[p][3] * d[3];
+        Rv[gi * nrhs + c] = B[gi * ldb + c];
+      }
+    }
+  }
+}
+
 struct Timer {
   hipEvent_t a = nullptr, b = nullptr;
   Timer() {
@@ -437,24 +552,32 @@ struct MixedWs {
   double* rowsum;
   double* normA;
   unsigned long long* norms;
+  float* dinv1;  // inverted 64 x 64 diagonal blocks of the first / second triangular factor
+  float* dinv2;
+  float* rbt;    // butterfly diagonals: U then V (2 np each)
+  int64_t np;    // order of the factored matrix (n, or n rounded up to 4 for butterflies)
 };
 
-int mixed_ws(int64_t n, int64_t nrhs, MixedWs* w) {
+int mixed_ws(int64_t n, int64_t np, int64_t nrhs, MixedWs* w) {
   void* p1 = nullptr;
-  DDLA_RETURN_IF(workspace(WS_MIXED, (size_t)n * n * sizeof(float), &p1));
+  DDLA_RETURN_IF(workspace(WS_MIXED, (size_t)np * np * sizeof(float), &p1));
   w->W = static_cast<float*>(p1);
+  w->np = np;
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-  const int64_t nb = (n + 127) / 128 + 1;
+  const int64_t nb = (np + 127) / 128 + 1;
   const size_t s_scales = al(2 * nb * sizeof(float));
-  const size_t s_perm = al(n * sizeof(int));
-  const size_t s_piv = al(n * sizeof(int32_t));
-  const size_t s_D = al((size_t)n * nrhs * sizeof(float));
+  const size_t s_perm = al(np * sizeof(int));
+  const size_t s_piv = al(np * sizeof(int32_t));
+  const size_t s_D = al((size_t)np * nrhs * sizeof(float));
   const size_t s_R = al((size_t)n * nrhs * sizeof(double));
   const size_t s_rs = al(n * sizeof(double));
   const size_t s_na = al(sizeof(double));
   const size_t s_no = al(2 * nrhs * sizeof(unsigned long long));
+  const size_t s_di = al(trsv_dinv_elems(np) * sizeof(float));
+  const size_t s_rb = al(4 * np * sizeof(float));
   void* p2 = nullptr;
-  DDLA_RETURN_IF(workspace(WS_MIXED2, s_scales + s_perm + s_piv + s_D + s_R + s_rs + s_na + s_no, &p2));
+  DDLA_RETURN_IF(workspace(
+      WS_MIXED2, s_scales + s_perm + s_piv + s_D + s_R + s_rs + s_na + s_no + 2 * s_di + s_rb, &p2));
   char* c = static_cast<char*>(p2);
   w->scales = reinterpret_cast<float*>(c); c += s_scales;
   w->perm = reinterpret_cast<int*>(c); c += s_perm;
@@ -463,7 +586,10 @@ int mixed_ws(int64_t n, int64_t nrhs, MixedWs* w) {
   w->R = reinterpret_cast<double*>(c); c += s_R;
   w->rowsum = reinterpret_cast<double*>(c); c += s_rs;
   w->normA = reinterpret_cast<double*>(c); c += s_na;
-  w->norms = reinterpret_cast<unsigned long long*>(c);
+  w->norms = reinterpret_cast<unsigned long long*>(c); c += s_no;
+  w->dinv1 = reinterpret_cast<float*>(c); c += s_di;
+  w->dinv2 = reinterpret_cast<float*>(c); c += s_di;
+  w->rbt = reinterpret_cast<float*>(c);
   return 0;
 }
 
@@ -505,6 +631,39 @@ int sgetrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
   return 0;
 }
 
+// FP32 LU without pivoting of the butterfly-transformed matrix (same block structure as
+// LUDecBlock, LU.cs:169-186); trailing updates in prec.
+int sgetrf_nopiv_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
+  hipStream_t s = stream();
+  int32_t* dinfo;
+  int *counts, *lists;
+  DDLA_RETURN_IF(lu_bookkeeping(1, &dinfo, &counts, &lists));
+  DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
+  float* a = w.W;
+  const int64_t lda = n;
+  for (int64_t j = 0; j < n; j += NBLK) {
+    const int b = (int)std::min<int64_t>(NBLK, n - j);
+    float* a11 = a + j * lda + j;
+    DDLA_RETURN_IF(launch_getf2_nopiv<float>(b, a11, lda, dinfo, (int)j, s));
+    const int64_t rest = n - j - b;
+    if (rest > 0) {
+      DDLA_RETURN_IF(launch_trsm_slab<float>(true, true, b, rest, a11, lda, 1, a11 + b, lda, s));
+      DDLA_RETURN_IF(launch_trsm_rlt_panel<float>(b, rest, a11, lda, a11 + b * lda, lda, s, true));
+      if (prec == DDLA_PREC_FP32) {
+        DDLA_RETURN_IF(launch_sgemm(rest, rest, b, -1.f, a11 + b * lda, lda, true, a11 + b, lda, true,
+                                    1.f, a11 + b * lda + b, lda, 0, s));
+      } else {
+        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, w.terms, false, 0, rest, rest, b, -1.f,
+                                    a11 + b * lda, lda, a11 + b, lda, a11 + b * lda + b, lda,
+                                    w.scales, s));
+      }
+    }
+  }
+  DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DDLA_HIP_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
 // FP32 lower Cholesky; trailing SYRK in prec.
 int spotrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
   hipStream_t s = stream();
@@ -521,7 +680,7 @@ int spotrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
     const int64_t rest = n - j - b;
     if (rest > 0) {
       float* a21 = a11 + b * lda;
-      DDLA_RETURN_IF(launch_trsm_rlt_panel<float>(b, rest, a11, lda, a21, lda, s));
+      DDLA_RETURN_IF(launch_trsm_rlt_panel<float>(b, rest, a11, lda, a21, lda, s, false));
       if (prec == DDLA_PREC_FP32) {
         DDLA_RETURN_IF(launch_sgemm(rest, rest, b, -1.f, a21, lda, true, a21, lda, false, 1.f,
                                     a21 + b, lda, 1, s));
@@ -548,32 +707,55 @@ int norm_inf(int64_t n, const double* A, int64_t lda, MixedWs& w, double* host) 
 }
 
 // Classic refinement. lu: solve with P, L (unit) and U; else L L^T.
-int refine(bool lu, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
+enum FactorKind { LU_PIVOT = 0, LU_BUTTERFLY = 1, CHOLESKY = 2 };
+
+// Classic refinement: X += M^-1 R, R = B - A X, with M the FP32 factor of kind `kind`.
+// Stops at backward error <= tol, after max_iters steps, or when three steps in a row fail
+// to lower the backward error by 10% (stagnation).
+int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
            int64_t ldb, double* X, int64_t ldx, MixedWs& w, double normA, double tol, int max_iters,
            int* iters, double* berr_out) {
   hipStream_t s = stream();
   const int64_t tot = n * nrhs;
+  const int64_t np = w.np;
+  const int64_t totp = (np / 4) * nrhs;
+  const float* RU = w.rbt;
+  const float* RV = w.rbt + 2 * np;
   // X = 0, R = B
   DDLA_RETURN_IF(launch_dscale2d(n, nrhs, 0.0, X, ldx, 1, 0, s));
   DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, B, ldb, 1, w.R, nrhs, s));
   std::vector<unsigned long long> nh(2 * nrhs);
-  double berr = INFINITY;
+  double berr = INFINITY, prev = INFINITY;
+  int slow = 0;
   int it = 0;
   for (; it < max_iters; ++it) {
-    // D = (LU)^-1 P R in FP32
-    hipLaunchKernelGGL(gather_d2s_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.R,
-                       nrhs, lu ? w.perm : nullptr, w.D);
+    // D = M^-1 R in FP32
+    if (kind == LU_BUTTERFLY)
+      hipLaunchKernelGGL(rbt_vec_kernel, dim3(grid_for(totp)), dim3(256), 0, s, n, np, nrhs, w.R,
+                         RU, w.D);
+    else
+      hipLaunchKernelGGL(gather_d2s_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.R,
+                         nrhs, kind == LU_PIVOT ? w.perm : nullptr, w.D);
     DDLA_HIP_CHECK(hipGetLastError());
-    if (lu) {
-      DDLA_RETURN_IF(trsm_left<float>(true, true, false, n, nrhs, w.W, n, w.D, nrhs, s));
-      DDLA_RETURN_IF(trsm_left<float>(false, false, false, n, nrhs, w.W, n, w.D, nrhs, s));
+    const bool lu = kind != CHOLESKY;
+    if (nrhs <= 8) {
+      // persistent triangular solves with the inverted diagonal blocks (trsv.hip)
+      DDLA_RETURN_IF(trsv_solve<float>(true, false, np, (int)nrhs, w.W, np, w.dinv1, w.D, nrhs, s));
+      DDLA_RETURN_IF(trsv_solve<float>(false, !lu, np, (int)nrhs, w.W, np, w.dinv2, w.D, nrhs, s));
+    } else if (lu) {
+      DDLA_RETURN_IF(trsm_left<float>(true, true, false, np, nrhs, w.W, np, w.D, nrhs, s));
+      DDLA_RETURN_IF(trsm_left<float>(false, false, false, np, nrhs, w.W, np, w.D, nrhs, s));
     } else {
-      DDLA_RETURN_IF(trsm_left<float>(true, false, false, n, nrhs, w.W, n, w.D, nrhs, s));
-      DDLA_RETURN_IF(trsm_left<float>(true, false, true, n, nrhs, w.W, n, w.D, nrhs, s));
+      DDLA_RETURN_IF(trsm_left<float>(true, false, false, np, nrhs, w.W, np, w.D, nrhs, s));
+      DDLA_RETURN_IF(trsm_left<float>(true, false, true, np, nrhs, w.W, np, w.D, nrhs, s));
     }
-    // X += D (FP64); R = B - A X (FP64)
-    hipLaunchKernelGGL(update_x_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.D, X, ldx,
-                       B, ldb, w.R);
+    // X += D (FP64, through V for the butterfly form); R = B - A X (FP64)
+    if (kind == LU_BUTTERFLY)
+      hipLaunchKernelGGL(rbt_update_kernel, dim3(grid_for(totp)), dim3(256), 0, s, n, np, nrhs, w.D,
+                         RV, X, ldx, B, ldb, w.R);
+    else
+      hipLaunchKernelGGL(update_x_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.D, X,
+                         ldx, B, ldb, w.R);
     DDLA_HIP_CHECK(hipGetLastError());
     DDLA_RETURN_IF(launch_dgemm(n, nrhs, n, -1.0, A, lda, true, X, ldx, true, 1.0, w.R, nrhs, 0, s));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, s, 2 * nrhs, w.norms);
@@ -595,6 +777,12 @@ int refine(bool lu, int64_t n, int64_t nrhs, const double* A, int64_t lda, const
       ++it;
       break;
     }
+    slow = (berr > 0.9 * prev) ? slow + 1 : 0;
+    if (slow >= 3 || !std::isfinite(berr)) {
+      ++it;
+      break;
+    }
+    prev = berr;
   }
   *iters = it;
   *berr_out = berr;
@@ -612,40 +800,81 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
     set_error("factor precision %d not supported by the mixed solver", prec);
     return DDLA_ERR_UNSUPPORTED;
   }
+  FactorKind kind = lu ? LU_PIVOT : CHOLESKY;
+  if (lu && opts && opts->pivoting == 1) kind = LU_BUTTERFLY;
+  const int64_t np = (kind == LU_BUTTERFLY) ? (n + 3) / 4 * 4 : n;
   MixedWs w;
-  DDLA_RETURN_IF(mixed_ws(n, nrhs, &w));
+  DDLA_RETURN_IF(mixed_ws(n, np, nrhs, &w));
   // default split: FP16 x2 (22 bits) and BF16 x3 (24 bits) reach FP32 product accuracy
   w.terms = (opts && opts->terms > 0) ? std::min(opts->terms, 3)
                                       : (prec == DDLA_PREC_BF16 ? 3 : 2);
   double normA = 0.0;
   DDLA_RETURN_IF(norm_inf(n, a, lda, w, &normA));
   Timer t;
-  t.start(s);
+  double factor_ms = 0.0, refine_ms = 0.0;
+  int iters = 0;
+  double berr = INFINITY;
   int32_t finfo = 0;
-  hipLaunchKernelGGL(d2s_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, n, n, a, lda, w.W, n);
-  DDLA_HIP_CHECK(hipGetLastError());
-  if (lu) {
-    DDLA_RETURN_IF(sgetrf_mixed(n, prec, w, &finfo));
-  } else {
-    DDLA_RETURN_IF(spotrf_mixed(n, prec, w, &finfo));
-    if (finfo > 0 && prec != DDLA_PREC_FP32) {
-      // the low-precision factor lost positive definiteness: FP32 factor instead
-      out.fallback = 1;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    t.start(s);
+    finfo = 0;
+    if (kind == LU_BUTTERFLY) {
+      // random diagonals (fixed seed: results are reproducible), then W = U^T A V
+      hipLaunchKernelGGL(rbt_diag_kernel, dim3(grid_for(4 * np)), dim3(256), 0, s, 4 * np,
+                         (uint64_t)0xDD1ABF00ull, w.rbt);
+      hipLaunchKernelGGL(rbt_matrix_kernel, dim3(grid_for((np / 4) * (np / 4))), dim3(256), 0, s,
+                         n, np, a, lda, w.rbt, w.rbt + 2 * np, w.W);
+      DDLA_HIP_CHECK(hipGetLastError());
+      DDLA_RETURN_IF(sgetrf_nopiv_mixed(np, prec, w, &finfo));
+    } else {
       hipLaunchKernelGGL(d2s_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, n, n, a, lda, w.W, n);
-      DDLA_RETURN_IF(spotrf_mixed(n, DDLA_PREC_FP32, w, &finfo));
+      DDLA_HIP_CHECK(hipGetLastError());
+      if (kind == LU_PIVOT) {
+        DDLA_RETURN_IF(sgetrf_mixed(n, prec, w, &finfo));
+      } else {
+        DDLA_RETURN_IF(spotrf_mixed(n, prec, w, &finfo));
+        if (finfo > 0 && prec != DDLA_PREC_FP32) {
+          // the low-precision factor lost positive definiteness: FP32 factor instead
+          out.fallback = 1;
+          hipLaunchKernelGGL(d2s_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, n, n, a, lda,
+                             w.W, n);
+          DDLA_RETURN_IF(spotrf_mixed(n, DDLA_PREC_FP32, w, &finfo));
+        }
+      }
     }
+    if (finfo == 0 && nrhs <= 8) {
+      // inverted diagonal blocks of the two triangular factors for the refinement solves
+      if (kind != CHOLESKY) {
+        DDLA_RETURN_IF(trsv_prepare<float>(true, true, false, np, w.W, np, w.dinv1, s));
+        DDLA_RETURN_IF(trsv_prepare<float>(false, false, false, np, w.W, np, w.dinv2, s));
+      } else {
+        DDLA_RETURN_IF(trsv_prepare<float>(true, false, false, n, w.W, n, w.dinv1, s));
+        DDLA_RETURN_IF(trsv_prepare<float>(false, false, true, n, w.W, n, w.dinv2, s));
+      }
+    }
+    factor_ms += t.stop_ms(s);
+    if (finfo == 0) {
+      t.start(s);
+      DDLA_RETURN_IF(refine(kind, n, nrhs, a, lda, b, ldb, x, ldx, w, normA, tol, max_iters, &iters,
+                            &berr));
+      refine_ms += t.stop_ms(s);
+    }
+    // a pivot-free factor that broke down or does not refine: redo with partial pivoting
+    if (kind == LU_BUTTERFLY && (finfo != 0 || !(berr <= tol))) {
+      kind = LU_PIVOT;
+      out.fallback = 2;
+      w.np = n;
+      continue;
+    }
+    break;
   }
-  out.factor_ms = t.stop_ms(s);
+  out.factor_ms = factor_ms;
+  out.refine_ms = refine_ms;
   if (finfo > 0) {
     out.status = finfo;
     if (info) *info = out;
     return finfo;
   }
-  t.start(s);
-  int iters = 0;
-  double berr = INFINITY;
-  DDLA_RETURN_IF(refine(lu, n, nrhs, a, lda, b, ldb, x, ldx, w, normA, tol, max_iters, &iters, &berr));
-  out.refine_ms = t.stop_ms(s);
   out.iters = iters;
   out.backward_error = berr;
   out.status = (berr <= tol) ? 0 : DDLA_ERR_NOT_CONVERGED;

@@ -61,7 +61,7 @@ int ensure_init() {
 hipStream_t stream() { return g_ctx.stream; }
 
 int workspace(int slot, size_t bytes, void** out) {
-  if (slot < 0 || slot >= 8) {
+  if (slot < 0 || slot >= WS_COUNT) {
     set_error("bad workspace slot %d", slot);
     return DDLA_ERR_INVALID_ARG;
   }
@@ -194,7 +194,7 @@ int finalize() {
   for (hipEvent_t e : g_event_pool) hipEventDestroy(e);
   g_event_pool.clear();
   DDLA_HIP_CHECK(hipDeviceSynchronize());
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < WS_COUNT; ++i) {
     if (g_ctx.ws[i]) hipFree(g_ctx.ws[i]);
     g_ctx.ws[i] = nullptr;
     g_ctx.ws_bytes[i] = 0;

@@ -51,15 +51,31 @@ __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
   auto tri = [&](int r, int i) -> T {
     return LOWER ? Ls[r * (r + 1) / 2 + i] : Ls[i * (i + 1) / 2 + r];
   };
+  // row r of the slab is owned by group g = r % 4; each step loads a batch of owned rows
+  // before storing them (one LDS round trip per batch instead of one per row)
+  constexpr int CH = 8;
   if (LOWER) {
     if (!UNIT && g == 0) S[cc] = S[cc] / tri(0, 0);
     __syncthreads();
     for (int i = 0; i < bs - 1; ++i) {
       const T x = S[i * 64 + cc];
-      for (int r = i + 1 + g; r < bs; r += 4) {
-        T v = fma(-tri(r, i), x, S[r * 64 + cc]);
-        if (!UNIT && r == i + 1) v = v / tri(r, r);
-        S[r * 64 + cc] = v;
+      for (int k0 = 0; k0 < TB / 4; k0 += CH) {
+        T v[CH], l[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int r = g + 4 * (k0 + k);
+          v[k] = S[r * 64 + cc];
+          l[k] = (r > i && r < bs) ? tri(r, i) : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int r = g + 4 * (k0 + k);
+          if (r > i && r < bs) {
+            T w = fma(-l[k], x, v[k]);
+            if (!UNIT && r == i + 1) w = w / tri(r, r);
+            S[r * 64 + cc] = w;
+          }
+        }
       }
       __syncthreads();
     }
@@ -69,11 +85,24 @@ __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
     __syncthreads();
     for (int i = last; i > 0; --i) {
       const T x = S[i * 64 + cc];
-      for (int r = i - 1 - g; r >= 0; r -= 4) {
-        // upper: row r < i couples through element (r, i) = stored transpose
-        T v = fma(-tri(r, i), x, S[r * 64 + cc]);
-        if (!UNIT && r == i - 1) v = v / tri(r, r);
-        S[r * 64 + cc] = v;
+      for (int k0 = 0; k0 < TB / 4; k0 += CH) {
+        T v[CH], l[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int r = g + 4 * (k0 + k);
+          v[k] = S[r * 64 + cc];
+          // upper: row r < i couples through element (r, i) = stored transpose
+          l[k] = (r < i) ? tri(r, i) : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int r = g + 4 * (k0 + k);
+          if (r < i) {
+            T w = fma(-l[k], x, v[k]);
+            if (!UNIT && r == i - 1) w = w / tri(r, r);
+            S[r * 64 + cc] = w;
+          }
+        }
       }
       __syncthreads();
     }

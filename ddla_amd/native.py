@@ -30,7 +30,7 @@ cint = ctypes.c_int
 
 class MpirOpts(ctypes.Structure):
     _fields_ = [("factor_prec", i32), ("max_iters", i32), ("tol", ctypes.c_double),
-                ("gmres", i32), ("terms", i32)]
+                ("gmres", i32), ("terms", i32), ("pivoting", i32), ("reserved", i32)]
 
 
 class MpirInfo(ctypes.Structure):
@@ -68,6 +68,8 @@ SIGNATURES = {
     "ddla_dgetrf": [i64, vp, i64, i64, vp],
     "ddla_dgetrs": [cint, i64, i64, vp, i64, i64, vp, vp, i64, i64],
     "ddla_dlaswp_dev": [cint, cint, i64, i64, vp, i64, vp, i64],
+    "ddla_dgetrf_nopiv_dev": [i64, vp, i64],
+    "ddla_dgetrs_nopiv_dev": [cint, i64, i64, vp, i64, vp, i64],
     "ddla_dpotrf_dev": [cint, i64, vp, i64],
     "ddla_dpotrs_dev": [cint, i64, i64, vp, i64, vp, i64],
     "ddla_dpotrf": [cint, i64, vp, i64, i64],

@@ -147,6 +147,15 @@ int ddla_dgetrs(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_
 int ddla_dlaswp_dev(int side, int forward, ddla_dim_t m, ddla_dim_t n, double* x, ddla_inc_t ldx,
                     const int32_t* dpiv, ddla_dim_t npiv);
 
+/* LU without pivoting (block 128, blocked iff n > 128), in place.  Replaces
+ * LU.LUDecompose (DDLA/Factorizations/LU.cs:133-142, LUDecBlock :169-186, LUDecUnblock
+ * :188-203).  Returns info (first zero pivot, 1-based). */
+int ddla_dgetrf_nopiv_dev(ddla_dim_t n, double* a, ddla_inc_t lda);
+/* Solves A X = B or A^T X = B with the factors of ddla_dgetrf_nopiv_dev.  Replaces
+ * LU.LUSolve (LU.cs:144-167, LUSolveInternal :205-231). */
+int ddla_dgetrs_nopiv_dev(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a,
+                          ddla_inc_t lda, double* b, ddla_inc_t ldb);
+
 /* ----------------------------------------------------------------- Cholesky -------- */
 /* Blocked Cholesky (block 128) of a row-major n x n SPD device matrix in place; only the
  * uplo triangle is read and written.  Replaces Cholesky.CholDecompose
@@ -186,18 +195,24 @@ typedef struct ddla_mpir_opts {
   int32_t factor_prec;   /* DDLA_PREC_FP16 / _BF16 / _FP32 / _FP64 */
   int32_t max_iters;     /* refinement steps before giving up (default 50 if <= 0) */
   double tol;            /* target normwise backward error (default 1e-14 if <= 0) */
-  int32_t gmres;         /* 0 = classic refinement only, 1 = switch to GMRES-IR if the
-                            classic iteration stalls, 2 = GMRES-IR from the start */
+  int32_t gmres;         /* reserved, must be 0 (classic refinement) */
   int32_t terms;         /* low-precision parts per FP32 operand in the trailing updates:
                             1 = one FP16/BF16 rounding, 2 or 3 = split x = h + m (+ l) with
                             the cross products summed; 0 = default (FP16: 2, BF16: 3) */
+  int32_t pivoting;      /* LU only: 0 = partial pivoting (the FP64 path's pivot rule);
+                            1 = two-sided random butterfly transform, then LU without pivoting
+                            (no global pivot search); falls back to partial pivoting when the
+                            pivot-free factor breaks down or does not refine (fallback = 2) */
+  int32_t reserved;
 } ddla_mpir_opts;
 
 typedef struct ddla_mpir_info {
   int32_t status;        /* 0 converged, > 0 factor info, < 0 error / not converged */
   int32_t iters;         /* refinement steps (GMRES inner steps counted in inner_iters) */
   int32_t inner_iters;
-  int32_t fallback;      /* 1 if the low-precision factor failed and FP32 was used */
+  int32_t fallback;      /* 1: the low-precision Cholesky factor was not positive definite
+                            and an FP32 factor was used; 2: the butterfly LU fell back to
+                            partial pivoting */
   double backward_error; /* max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf) */
   double factor_ms;      /* GPU time of the factorization */
   double refine_ms;      /* GPU time of the refinement */

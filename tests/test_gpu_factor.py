@@ -131,6 +131,40 @@ def test_lu_8192_pivot_sequence_and_backward_error(ddla, oracle):
     assert rel_fro(dA.to_host(), Ao) <= 1e-13 * n
 
 
+# --------------------------------------------------------------- LU without pivoting ---
+@pytest.mark.parametrize("n", [1, 7, 128, 129, 300, 1000])
+def test_lu_nopiv_matches_oracle(ddla, oracle, n):
+    # diagonally dominant, so elimination without pivoting is stable (TestLU.cs:11-63 builds
+    # A = L U from random factors the same way)
+    L = np.tril(oracle.uniform_matrix(n, n, 3 + n), -1) * 0.1 + np.eye(n)
+    U = np.triu(oracle.uniform_matrix(n, n, 5 + n)) + 4 * np.eye(n)
+    A0 = L @ U
+    Ao = A0.copy()
+    assert oracle.getrf_nopiv(Ao) == 0
+    A = A0.copy()
+    ddla.LUDecompose(A)
+    assert rel_fro(A, Ao) <= 1e-13 * n
+    # A0 = L U was rounded, so its factors differ from the generators by about cond * u
+    Lf, Uf = oracle.lu_factors(A)
+    assert rel_fro(Lf, L) <= 1e-10 and rel_fro(Uf, U) <= 1e-10
+    for trans in (False, True):
+        B = oracle.uniform_matrix(n, 3, 9)
+        X = B.copy()
+        ddla.LUSolve(A, X, trans)
+        M = A0.T if trans else A0
+        # L U is not well conditioned at n=1000 (unit L with 0.1-scaled entries), so the
+        # residual is judged relative to |M| |X|: backward stability of the two solves
+        for j in range(X.shape[1]):
+            assert backward_error(M, X[:, j], B[:, j]) < 1e-14
+
+
+def test_lu_nopiv_zero_pivot(ddla):
+    A = np.eye(200)
+    A[0, 0] = 0.0
+    with pytest.raises(ZeroDivisionError):
+        ddla.LUDecompose(A)
+
+
 # ------------------------------------------------------------------------- Cholesky ---
 def test_chol_integer_known_answers(ddla):
     c = golden("chol_int.npz")

@@ -20,12 +20,12 @@ def conditioned(n, kappa, seed, spd=False):
     return (U * s) @ V.T
 
 
-def run(ddla, fn, A, nrhs, prec, seed=1, tol=1e-14, gmres=1, terms=0):
+def run(ddla, fn, A, nrhs, prec, seed=1, tol=1e-14, terms=0, **kw):
     n = A.shape[0]
     B = np.random.default_rng(seed).uniform(-1, 1, (n, nrhs))
     dA, dB = ddla.DeviceMatrix.from_host(A), ddla.DeviceMatrix.from_host(B)
     dX = ddla.DeviceMatrix(n, nrhs)
-    info = fn(dA, dB, dX, prec=prec, tol=tol, gmres=gmres, terms=terms)
+    info = fn(dA, dB, dX, prec=prec, tol=tol, terms=terms, **kw)
     np.testing.assert_array_equal(dA.to_host(), A)  # A is not modified
     return info, dX.to_host(), B
 
@@ -74,6 +74,36 @@ def test_posv_mpir_fallback_to_fp32(ddla):
     info, X, B = run(ddla, ddla.posv_mpir, A, 1, 3, terms=1)
     assert info.fallback == 1
     assert info.status == 0 and info.backward_error <= 1e-14
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3])
+@pytest.mark.parametrize("n", [1021, 1024])  # 1021: butterflies on a padded order
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_gesv_mpir_butterfly(ddla, oracle, prec, n, nrhs):
+    kappa = 1e4
+    A = conditioned(n, kappa, 21)
+    info, X, B = run(ddla, ddla.gesv_mpir, A, nrhs, prec, pivoting=ddla.PIVOT_BUTTERFLY)
+    assert info.status == 0, (info.status, info.iters, info.backward_error, info.fallback)
+    # either the pivot-free factor refined on its own or the solver switched to partial
+    # pivoting (element growth of the transformed elimination can exceed what an FP32 /
+    # FP16 / BF16 factor refines from); the answer reaches FP64 accuracy in both cases
+    assert info.fallback in (0, 2), (info.iters, info.backward_error)
+    assert info.backward_error <= 1e-14
+    Ao = A.copy()
+    _, piv = oracle.getrf(Ao)
+    Xo = oracle.getrs(Ao, piv, B.copy())
+    assert rel_fro(X, Xo) <= kappa * np.sqrt(n) * 1e-14
+
+
+def test_gesv_mpir_butterfly_permutation(ddla):
+    # a permutation matrix: LU without pivoting breaks down on it at once; after the
+    # two-sided butterfly transform it must not (or the solver falls back to partial
+    # pivoting) -- either way the solve reaches FP64 accuracy
+    n = 512
+    P = np.eye(n)[np.random.default_rng(4).permutation(n)]
+    info, X, B = run(ddla, ddla.gesv_mpir, P, 2, 2, pivoting=ddla.PIVOT_BUTTERFLY)
+    assert info.status == 0 and info.backward_error <= 1e-14
+    np.testing.assert_allclose(P @ X, B, rtol=0, atol=1e-13)
 
 
 def test_gesv_mpir_rejects_singular(ddla):
