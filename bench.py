@@ -27,16 +27,23 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_system(n, kappa, nrhs, seed, torch):
-    """A = U diag(s) V^T (QR of Gaussians), x_true uniform[-1,1], b = A x_true (FP64)."""
+def make_system(n, kappa, nrhs, seed, torch, qr_device="cuda"):
+    """A = U diag(s) V^T (QR of Gaussians), x_true uniform[-1,1], b = A x_true (FP64).
+    qr_device="cpu" runs the two QRs on the host (slower; for profiler runs that cannot
+    instrument the GPU QR library)."""
     dev = torch.device("cuda")
-    g = torch.Generator(device=dev)
+    qd = torch.device(qr_device)
+    g = torch.Generator(device=qd)
     g.manual_seed(seed)
     t0 = time.time()
-    U, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    U, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=qd, generator=g))
+    U = U.to(dev)
     log(f"QR 1 done {time.time() - t0:.1f}s")
-    V, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    V, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=qd, generator=g))
+    V = V.to(dev)
     log(f"QR 2 done {time.time() - t0:.1f}s")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed + 1)
     s = kappa ** (-torch.arange(n, dtype=torch.float64, device=dev) / (n - 1))
     A = (U * s) @ V.T.contiguous()
     del U, V
@@ -70,7 +77,18 @@ def time_gemm(ddla, n, reps):
     return {"n": n, "ms": dt * 1e3, "tflops": tf / 1e12, "frac_fp64_peak": tf / PEAK["f64"]}
 
 
-def roofline_of(name, e):
+def measured_traffic(name, n):
+    """HBM bytes per launch of a kernel group from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled against a device copy), when it covers this workload."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_n{n}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        e = json.load(f).get(name)
+    return e["traffic_bytes"] if e else None
+
+
+def roofline_of(name, e, n):
     """Roofline object for one profiled kernel (totals over the timed region)."""
     if e["ms"] <= 0 or e["count"] == 0:
         return None
@@ -83,12 +101,12 @@ def roofline_of(name, e):
     if e["flops"] > 0 and ai >= ridge:
         ach = e["flops"] / sec
         return {"kernel": name, "bound": "mfma", "achieved": ach / 1e12, "peak": peak_f / 1e12,
-                "unit": "TFLOP/s", "frac": ach / peak_f, "traffic": None,
+                "unit": "TFLOP/s", "frac": ach / peak_f, "traffic": measured_traffic(name, n),
                 "avg_launch_ms": per_launch_ms, "launches": e["count"],
                 "flops_per_launch": e["flops"] / e["count"]}
     ach = e["bytes"] / sec
     return {"kernel": name, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK["hbm"] / 1e9,
-            "unit": "GB/s", "frac": ach / PEAK["hbm"], "traffic": None,
+            "unit": "GB/s", "frac": ach / PEAK["hbm"], "traffic": measured_traffic(name, n),
             "avg_launch_ms": per_launch_ms, "launches": e["count"],
             "bytes_per_launch": e["bytes"] / e["count"]}
 
@@ -122,9 +140,15 @@ def main():
     ap.add_argument("--prec", choices=["fp16", "bf16", "fp32"], default="fp16")
     ap.add_argument("--pivoting", choices=["butterfly", "partial"], default="partial")
     ap.add_argument("--nrhs", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compare-butterfly", action="store_true",
+                    help="also time the butterfly (pivot-free) variant of the mixed solver")
+    ap.add_argument("--sweep-kappa", default="1e2,1e4",
+                    help="extra condition numbers timed (one solve each) besides --kappa")
     ap.add_argument("--seed", type=int, default=0xDD1A0003)
+    ap.add_argument("--qr-device", choices=["cuda", "cpu"], default="cuda",
+                    help="where the test matrix's QR factors are computed")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -139,7 +163,7 @@ def main():
     n, nrhs = args.n, args.nrhs
     prec = {"fp16": ddla.PREC_FP16, "bf16": ddla.PREC_BF16, "fp32": ddla.PREC_FP32}[args.prec]
     log(f"n={n} kappa={args.kappa:g} prec={args.prec} nrhs={nrhs}")
-    A_t, b_t, xt = make_system(n, args.kappa, nrhs, args.seed, torch)
+    A_t, b_t, xt = make_system(n, args.kappa, nrhs, args.seed, torch, args.qr_device)
     A = ddla.DeviceMatrix.wrap(A_t.data_ptr(), n, n, n, A_t)
     B = ddla.DeviceMatrix.wrap(b_t.data_ptr(), n, nrhs, nrhs, b_t)
     X_t = torch.zeros(n, nrhs, dtype=torch.float64, device="cuda")
@@ -151,16 +175,21 @@ def main():
     def solve(piv):
         return ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14, pivoting=piv)
 
-    # ---- the other pivoting choice of the mixed solver, reported next to the headline
-    other = ddla.PIVOT_PARTIAL if pivoting == ddla.PIVOT_BUTTERFLY else ddla.PIVOT_BUTTERFLY
-    solve(other)
-    ddla.synchronize()
-    t0 = time.perf_counter()
-    oinfo = solve(other)
-    ddla.synchronize()
-    t_other = time.perf_counter() - t0
-    log(f"mixed ({'partial' if other == ddla.PIVOT_PARTIAL else 'butterfly'}): "
-        f"{t_other * 1e3:.1f} ms status={oinfo.status} iters={oinfo.iters}")
+    # ---- optionally, the other pivoting choice of the mixed solver
+    other_out = None
+    if args.compare_butterfly:
+        other = ddla.PIVOT_PARTIAL if pivoting == ddla.PIVOT_BUTTERFLY else ddla.PIVOT_BUTTERFLY
+        solve(other)
+        ddla.synchronize()
+        t0 = time.perf_counter()
+        oinfo = solve(other)
+        ddla.synchronize()
+        t_other = time.perf_counter() - t0
+        other_out = {"pivoting": "partial" if other == ddla.PIVOT_PARTIAL else "butterfly",
+                     "ms_per_solve": t_other * 1e3, "status": oinfo.status, "iters": oinfo.iters,
+                     "fallback": oinfo.fallback, "backward_error": oinfo.backward_error}
+        log(f"mixed ({other_out['pivoting']}): {t_other * 1e3:.1f} ms status={oinfo.status} "
+            f"iters={oinfo.iters} fallback={oinfo.fallback}")
 
     # ---- mixed precision LU + IR (the measured step)
     for w in range(args.warmup):
@@ -191,16 +220,22 @@ def main():
     W = ddla.DeviceMatrix.wrap(W_t.data_ptr(), n, n, n, W_t)
     Xf = ddla.DeviceMatrix.wrap(Xf_t.data_ptr(), n, nrhs, nrhs, Xf_t)
     t_fp64 = []
-    for k in range(args.warmup + max(1, min(args.steps, 2))):
+    nfp = max(1, min(args.steps, 2))
+    for k in range(args.warmup + nfp):
         W_t.copy_(A_t)
         Xf_t.copy_(b_t)
         torch.cuda.synchronize()
+        if k == args.warmup:
+            ddla.profile_reset()
+            ddla.profile_enable(True)
         t0 = time.perf_counter()
         piv = ddla.PLUDecompose(W)
         ddla.PLUSolve(W, piv, Xf)
         ddla.synchronize()
         if k >= args.warmup:
             t_fp64.append(time.perf_counter() - t0)
+    ddla.profile_enable(False)
+    prof64 = ddla.profile_read()
     t_fp64 = sum(t_fp64) / len(t_fp64)
     be_fp64 = backward_error(torch, A_t, Xf_t, b_t)
     fe_vs_fp64 = float(((X_t - Xf_t).abs().amax() / Xf_t.abs().amax()))
@@ -213,9 +248,35 @@ def main():
 
     # ---- roofline of the dominant kernel of the timed region
     dom = max(prof.items(), key=lambda kv: kv[1]["ms"])[0] if prof else None
-    roof = roofline_of(dom, prof[dom]) if dom else None
+    roof = roofline_of(dom, prof[dom], n) if dom else None
     breakdown = {k: {"launches": v["count"], "ms_per_step": v["ms"] / args.steps}
                  for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+    breakdown64 = {k: {"launches": v["count"] // nfp, "ms_per_solve": v["ms"] / nfp}
+                   for k, v in sorted(prof64.items(), key=lambda kv: -kv[1]["ms"])}
+
+    # ---- the other condition numbers of the configuration, one timed solve each
+    sweep = []
+    for ks in [x for x in args.sweep_kappa.split(",") if x.strip()]:
+        kap = float(ks)
+        if kap == args.kappa:
+            continue
+        del A, B, X, A_t, b_t, X_t
+        torch.cuda.empty_cache()
+        A_t, b_t, xt = make_system(n, kap, nrhs, args.seed, torch, args.qr_device)
+        A = ddla.DeviceMatrix.wrap(A_t.data_ptr(), n, n, n, A_t)
+        B = ddla.DeviceMatrix.wrap(b_t.data_ptr(), n, nrhs, nrhs, b_t)
+        X_t = torch.zeros(n, nrhs, dtype=torch.float64, device="cuda")
+        X = ddla.DeviceMatrix.wrap(X_t.data_ptr(), n, nrhs, nrhs, X_t)
+        solve(pivoting)
+        ddla.synchronize()
+        t0 = time.perf_counter()
+        si = solve(pivoting)
+        ddla.synchronize()
+        ts = time.perf_counter() - t0
+        sweep.append({"kappa": kap, "ms_per_solve": ts * 1e3, "solves_per_s": 1.0 / ts,
+                      "iters": si.iters, "status": si.status, "backward_error": si.backward_error,
+                      "speedup_vs_fp64_lu": t_fp64 / ts})
+        log(f"kappa={kap:g}: {ts * 1e3:.1f} ms iters={si.iters} berr={si.backward_error:.2e}")
 
     cpu = None
     if not args.no_cpu:
@@ -246,16 +307,15 @@ def main():
                   "backward_error": be_mixed, "forward_error_vs_xtrue": fe_mixed,
                   "forward_error_vs_fp64_lu": fe_vs_fp64,
                   "factor_ms": last.factor_ms, "refine_ms": last.refine_ms},
-        "mixed_other_pivoting": {"pivoting": "partial" if other == ddla.PIVOT_PARTIAL else "butterfly",
-                                 "ms_per_solve": t_other * 1e3, "status": oinfo.status,
-                                 "iters": oinfo.iters, "backward_error": oinfo.backward_error,
-                                 "speedup_vs_fp64_lu": t_fp64 / t_other},
+        "mixed_other_pivoting": other_out,
+        "kappa_sweep": sweep,
         "fp64_lu": {"ms_per_solve": t_fp64 * 1e3, "solves_per_s": 1.0 / t_fp64,
                     "backward_error": be_fp64,
                     "gflops": (2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs) / t_fp64 / 1e9},
         "speedup_vs_fp64_lu": t_fp64 / t_mixed,
         "gemm_fp64": gemm,
         "kernel_breakdown": breakdown,
+        "kernel_breakdown_fp64_lu": breakdown64,
     }
     print(json.dumps(out), flush=True)
 
