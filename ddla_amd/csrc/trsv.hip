@@ -21,7 +21,7 @@
 namespace ddla {
 namespace {
 
-constexpr int BS = 64;    // block rows
+constexpr int BS_INV = 64;  // order of the inverted diagonal blocks
 constexpr int RMAX = 8;   // right-hand sides per launch
 typedef unsigned long long u64;
 
@@ -74,6 +74,7 @@ __device__ __forceinline__ T opT(const T* A, int64_t lda, int64_t i, int64_t k) 
 template <typename T, bool LOWER, bool UNIT, bool TRANS>
 __global__ __launch_bounds__(256) void trsv_dinv_kernel(int64_t n, const T* __restrict__ A,
                                                         int64_t lda, T* __restrict__ Dinv) {
+  constexpr int BS = BS_INV;
   __shared__ T M[BS][BS + 1];
   __shared__ T X[BS][BS + 1];
   const int64_t b0 = (int64_t)blockIdx.x * BS;
@@ -111,91 +112,216 @@ __global__ __launch_bounds__(256) void trsv_dinv_kernel(int64_t n, const T* __re
   }
 }
 
-template <typename T, bool LOWER, bool TRANS>
+// block rows per step of the solve: 128 for FP32 (64 KB blocks), 64 for FP64
+template <typename T>
+constexpr int solve_bs() {
+  return sizeof(T) == 4 ? 128 : 64;
+}
+
+template <typename T, int BS, int NR, bool LOWER, bool TRANS>
 __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T* __restrict__ A,
                                                    int64_t lda, const T* __restrict__ Dinv,
                                                    T* __restrict__ B, int64_t ldb,
                                                    u64* __restrict__ XW, unsigned tag) {
+  constexpr int Q = 256 / BS;       // threads per row of a block product (k split)
+  constexpr int KQ = BS / Q;        // k range per thread
+  constexpr int E = BS * BS / 256;  // staged block elements per thread
+  constexpr int NSUB = BS / BS_INV; // inverted sub-blocks on the diagonal
+  constexpr int XPT = (BS * NR + 255) / 256;  // X / B words per thread
+  constexpr int NW = Words<T>::N;
   __shared__ T Ts[BS][BS + 1];
-  __shared__ T xs[BS][RMAX];
-  __shared__ T red[4][BS][RMAX];
+  __shared__ T xs[BS][NR];
+  __shared__ T red[4][BS][NR];
   const int nblk = (int)((n + BS - 1) / BS);
   const int G = gridDim.x;
   const int tid = threadIdx.x;
-  const int row = tid & 63, q = tid >> 6;  // q: k-quarter of the block product
-  constexpr int NW = Words<T>::N;
+  const int row = tid % BS, q = tid / BS;
 
   for (int jb = blockIdx.x; jb < nblk; jb += G) {
     const int kb = LOWER ? jb : nblk - 1 - jb;  // dependency order
     const int64_t r0 = (int64_t)kb * BS;
     const int rows = (int)min<int64_t>(BS, n - r0);
-    T part[RMAX];
-#pragma unroll
-    for (int c = 0; c < RMAX; ++c) part[c] = T(0);
-    // off-diagonal blocks, in the order their X become available
-    for (int jc = 0; jc < jb; ++jc) {
+    // Blocks in dependency order: jc < jb are the off-diagonal blocks, jc == jb the diagonal
+    // one.  Block jc + 1 (and the X it multiplies) is loaded while block jc is used, so the
+    // stream of a long block row runs at the memory system's pace, not one latency per block.
+    T stage[E];
+    auto load_block = [&](int jc) {
       const int kc = LOWER ? jc : nblk - 1 - jc;
       const int64_t c0 = (int64_t)kc * BS;
       const int cols = (int)min<int64_t>(BS, n - c0);
-      __syncthreads();  // previous step finished with Ts / xs
-      for (int idx = tid; idx < BS * BS; idx += 256) {
-        int r, c;
-        if (TRANS) {  // coalesced along T's rows = op(T)'s columns
-          c = idx / BS;
-          r = idx % BS;
-        } else {
-          r = idx / BS;
-          c = idx % BS;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int idx = tid + 256 * e;
+        const int i = idx / BS, j = idx % BS;  // j runs along memory
+        // !TRANS: op(T)(r0 + i, c0 + j) = A[(r0 + i) lda + c0 + j]
+        //  TRANS: op(T)(r0 + j, c0 + i) = A[(c0 + i) lda + r0 + j]
+        const bool ok = TRANS ? (j < rows && i < cols) : (i < rows && j < cols);
+        stage[e] = ok ? (TRANS ? A[(c0 + i) * lda + r0 + j] : A[(r0 + i) * lda + c0 + j]) : T(0);
+      }
+    };
+    auto store_block = [&]() {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int idx = tid + 256 * e;
+        const int i = idx / BS, j = idx % BS;
+        if (TRANS) Ts[j][i] = stage[e];
+        else Ts[i][j] = stage[e];
+      }
+    };
+    u64 xw[XPT][NW];
+    auto load_x = [&](int jc) {
+      const int kc = LOWER ? jc : nblk - 1 - jc;
+      const int64_t c0 = (int64_t)kc * BS;
+#pragma unroll
+      for (int t = 0; t < XPT; ++t) {
+        const int idx = tid + 256 * t;
+        const int k = idx / NR, c = idx % NR;
+        if (idx < BS * NR && c < nrhs && c0 + k < n) {
+#pragma unroll
+          for (int w = 0; w < NW; ++w)
+            xw[t][w] = __hip_atomic_load(XW + ((c0 + k) * nrhs + c) * NW + w, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
         }
-        Ts[r][c] = (r < rows && c < cols) ? opT<T, TRANS>(A, lda, r0 + r, c0 + c) : T(0);
       }
-      for (int idx = tid; idx < BS * nrhs; idx += 256) {
-        const int k = idx / nrhs, c = idx % nrhs;
-        xs[k][c] = (k < cols) ? await<T>(XW + ((c0 + k) * nrhs + c) * NW, tag) : T(0);
+    };
+    auto use_x = [&](int jc) {  // xw (re-read until current) -> xs
+      const int kc = LOWER ? jc : nblk - 1 - jc;
+      const int64_t c0 = (int64_t)kc * BS;
+#pragma unroll
+      for (int t = 0; t < XPT; ++t) {
+        const int idx = tid + 256 * t;
+        if (idx >= BS * NR) break;
+        const int k = idx / NR, c = idx % NR;
+        T v = T(0);
+        if (c < nrhs && c0 + k < n) {
+          const u64* p = XW + ((c0 + k) * nrhs + c) * NW;
+          for (;;) {
+            bool ok = true;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) ok = ok && static_cast<unsigned>(xw[t][w]) == tag;
+            if (ok) break;
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int w = 0; w < NW; ++w)
+              xw[t][w] = __hip_atomic_load(p + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          if constexpr (NW == 2)
+            v = __builtin_bit_cast(T, ((xw[t][0] >> 32) << 32) | (xw[t][1] >> 32));
+          else
+            v = __builtin_bit_cast(T, static_cast<unsigned>(xw[t][0] >> 32));
+        }
+        xs[k][c] = v;
       }
+    };
+
+    // the diagonal step's operands do not depend on X: fetch them first (the inverted
+    // sub-blocks of this thread's 64 x 16 slice, and B)
+    const int sr = tid & 63, sq = tid >> 6;  // 64 rows x 4 k-quarters for the 64 x 64 products
+    T dreg[NSUB][16];
+#pragma unroll
+    for (int sb = 0; sb < NSUB; ++sb) {
+      const bool okb = sb * BS_INV < rows;
+      const T* D = Dinv + ((int64_t)kb * NSUB + sb) * BS_INV * BS_INV + sr * BS_INV + sq * 16;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) dreg[sb][k] = okb ? D[k] : T(0);
+    }
+    T breg[XPT];
+#pragma unroll
+    for (int t = 0; t < XPT; ++t) {
+      const int idx = tid + 256 * t;
+      const int r = idx / NR, c = idx % NR;
+      breg[t] = (idx < BS * NR && c < nrhs && r < rows) ? B[(r0 + r) * ldb + c] : T(0);
+    }
+    T part[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) part[c] = T(0);
+    if (jb > 0) load_x(0);
+    load_block(0);
+    for (int jc = 0; jc < jb; ++jc) {
+      __syncthreads();  // the previous block's product is done with Ts / xs
+      store_block();
+      use_x(jc);
+      if (jc + 1 < jb) load_x(jc + 1);
+      load_block(jc + 1);  // the next off-diagonal block, or the diagonal block
       __syncthreads();
-#pragma unroll 4
-      for (int k = q * 16; k < q * 16 + 16; ++k) {
+#pragma unroll 16
+      for (int k = q * KQ; k < q * KQ + KQ; ++k) {
         const T a = Ts[row][k];
 #pragma unroll
-        for (int c = 0; c < RMAX; ++c)
-          if (c < nrhs) part[c] = fma(a, xs[k][c], part[c]);
+        for (int c = 0; c < NR; ++c) part[c] = fma(a, xs[k][c], part[c]);
       }
     }
-    // rhs = B[kb] - sum; then X[kb] = Dinv[kb] rhs
-#pragma unroll
-    for (int c = 0; c < RMAX; ++c)
-      if (c < nrhs) red[q][row][c] = part[c];
+    // diagonal block: rhs = B - sum, then sub-block substitution with the inverted 64 x 64
+    // diagonal sub-blocks and the block's own off-diagonal sub-blocks
     __syncthreads();
-    for (int idx = tid; idx < BS * nrhs; idx += 256) {
-      const int r = idx / nrhs, c = idx % nrhs;
-      const T s = red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
-      xs[r][c] = (r < rows) ? B[(r0 + r) * ldb + c] - s : T(0);
+    store_block();
+#pragma unroll
+    for (int c = 0; c < NR; ++c) red[q][row][c] = part[c];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < XPT; ++t) {
+      const int idx = tid + 256 * t;
+      if (idx < BS * NR) {
+        const int r = idx / NR, c = idx % NR;
+        T sum = T(0);
+#pragma unroll
+        for (int qq = 0; qq < Q; ++qq) sum += red[qq][r][c];
+        xs[r][c] = (c < nrhs && r < rows) ? breg[t] - sum : T(0);
+      }
     }
-    const T* D = Dinv + (int64_t)kb * BS * BS;
-    for (int idx = tid; idx < BS * BS; idx += 256) Ts[idx / BS][idx % BS] = D[idx];
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < RMAX; ++c) part[c] = T(0);
-#pragma unroll 4
-    for (int k = q * 16; k < q * 16 + 16; ++k) {
-      const T a = Ts[row][k];
+    for (int ss = 0; ss < NSUB; ++ss) {
+      const int sb = LOWER ? ss : NSUB - 1 - ss;
+      if (sb * BS_INV >= rows) continue;  // sub-block beyond n (uniform)
+      // x_sb = Dinv_sb rhs_sb
+      T acc[NR];
 #pragma unroll
-      for (int c = 0; c < RMAX; ++c)
-        if (c < nrhs) part[c] = fma(a, xs[k][c], part[c]);
+      for (int c = 0; c < NR; ++c) acc[c] = T(0);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const T d = dreg[sb][k];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] = fma(d, xs[sb * BS_INV + sq * 16 + k][c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < NR; ++c) red[sq][sr][c] = acc[c];
+      __syncthreads();
+      for (int idx = tid; idx < BS_INV * NR; idx += 256) {
+        const int r = idx / NR, c = idx % NR;
+        xs[sb * BS_INV + r][c] = red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
+      }
+      __syncthreads();
+      // the sub-blocks still to solve: rhs_t -= T[t][sb] x_sb
+      for (int tt = ss + 1; tt < NSUB; ++tt) {
+        const int tb = LOWER ? tt : NSUB - 1 - tt;
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] = T(0);
+#pragma unroll
+        for (int k = sq * 16; k < sq * 16 + 16; ++k) {
+          const T a = Ts[tb * BS_INV + sr][sb * BS_INV + k];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] = fma(a, xs[sb * BS_INV + k][c], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NR; ++c) red[sq][sr][c] = acc[c];
+        __syncthreads();
+        for (int idx = tid; idx < BS_INV * NR; idx += 256) {
+          const int r = idx / NR, c = idx % NR;
+          xs[tb * BS_INV + r][c] -= red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
+        }
+        __syncthreads();
+      }
     }
-#pragma unroll
-    for (int c = 0; c < RMAX; ++c)
-      if (c < nrhs) red[q][row][c] = part[c];
-    __syncthreads();
-    for (int idx = tid; idx < BS * nrhs; idx += 256) {
-      const int r = idx / nrhs, c = idx % nrhs;
-      if (r < rows) {
-        const T x = red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
+    for (int idx = tid; idx < BS * NR; idx += 256) {
+      const int r = idx / NR, c = idx % NR;
+      if (r < rows && c < nrhs) {
+        const T x = xs[r][c];
         publish<T>(XW + ((r0 + r) * nrhs + c) * NW, x, tag);
         B[(r0 + r) * ldb + c] = x;
       }
     }
+    __syncthreads();  // xs / Ts are reused by the next block row of this workgroup
   }
 }
 
@@ -212,7 +338,8 @@ int max_trsv_groups() {
   if (cached < 0) {
     int per_cu = 0;
     DDLA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(trsv_kernel<T, LOWER, TRANS>), 256, 0));
+        &per_cu, reinterpret_cast<const void*>(trsv_kernel<T, solve_bs<T>(), 8, LOWER, TRANS>),
+        256, 0));
     cached = per_cu * ctx().num_cus;
   }
   return cached;
@@ -224,7 +351,7 @@ int max_trsv_groups() {
 template <typename T>
 int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64_t lda, T* Dinv,
                  hipStream_t s) {
-  const int nblk = ceil_div(n, BS);
+  const int nblk = ceil_div(n, BS_INV);
 #define TD(L, U, TR)                                                                            \
   hipLaunchKernelGGL((trsv_dinv_kernel<T, L, U, TR>), dim3(nblk), dim3(256), 0, s, n, A, lda, \
                      Dinv)
@@ -241,7 +368,7 @@ int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64
   return 0;
 }
 
-size_t trsv_dinv_elems(int64_t n) { return (size_t)ceil_div(n, BS) * BS * BS; }
+size_t trsv_dinv_elems(int64_t n) { return (size_t)ceil_div(n, BS_INV) * BS_INV * BS_INV; }
 
 // op(T) X = B in place, nrhs <= 8, with Dinv from trsv_prepare (same lower/trans).
 template <typename T>
@@ -263,7 +390,7 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
   }
   tag += 1;
   if (tag == 0) tag = 1;
-  const int nblk = ceil_div(n, BS);
+  const int nblk = ceil_div(n, solve_bs<T>());
   ProfScope prof(sizeof(T) == 8 ? "trsv_f64" : "trsv_f32", (double)n * n * nrhs,
                  (0.5 * n * n + 2.0 * n * nrhs) * sizeof(T), s);
   u64* XW = static_cast<u64*>(ws);
@@ -272,17 +399,26 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
   int64_t lda_ = lda, ldb_ = ldb;
   unsigned tag_ = tag;
   void* args[] = {&n_, &nrhs_, &A, &lda_, &Dinv, &B, &ldb_, &XW, &tag_};
-#define TS(L, TR)                                                                        \
+#define TS(NR, L, TR)                                                                    \
   do {                                                                                   \
     const int G = std::min(nblk, max_trsv_groups<T, L, TR>());                           \
     DDLA_HIP_CHECK(hipLaunchCooperativeKernel(                                           \
-        reinterpret_cast<const void*>(trsv_kernel<T, L, TR>), dim3(G), dim3(256), args, 0, s)); \
+        reinterpret_cast<const void*>(trsv_kernel<T, solve_bs<T>(), NR, L, TR>), dim3(G), \
+        dim3(256), args, 0, s));                                                         \
+  } while (0)
+#define TSN(L, TR)                          \
+  do {                                      \
+    if (nrhs == 1) TS(1, L, TR);            \
+    else if (nrhs == 2) TS(2, L, TR);       \
+    else if (nrhs <= 4) TS(4, L, TR);       \
+    else TS(8, L, TR);                      \
   } while (0)
   if (lower) {
-    if (trans) TS(true, true); else TS(true, false);
+    if (trans) TSN(true, true); else TSN(true, false);
   } else {
-    if (trans) TS(false, true); else TS(false, false);
+    if (trans) TSN(false, true); else TSN(false, false);
   }
+#undef TSN
 #undef TS
   return 0;
 }
