@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -111,6 +113,124 @@ def roofline_of(name, e, n):
             "bytes_per_launch": e["bytes"] / e["count"]}
 
 
+def batched_workload(torch, ddla, count, prec, seed):
+    """The batched configuration: count problems, n uniform on 8..256, half LU on uniform[-1,1]
+    entries, half SPD (G G^T + n I, G Gaussian), packed row-major in one device buffer."""
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(seed)
+    n = rng.integers(8, 257, count).astype(np.int32)
+    kind = np.zeros(count, np.int32)
+    kind[rng.permutation(count)[: count // 2]] = ddla.BATCH_CHOL
+    off = np.zeros(count, np.int64)
+    off[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+    total = int((n.astype(np.int64) ** 2).sum())
+    dt = torch.float64 if prec == ddla.PREC_FP64 else torch.float32
+    a = torch.empty(total, dtype=dt, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    for size in np.unique(n):
+        for k in (ddla.BATCH_LU, ddla.BATCH_CHOL):
+            idx = np.nonzero((n == size) & (kind == k))[0]
+            if idx.size == 0:
+                continue
+            c = int(idx.size)
+            if k == ddla.BATCH_LU:
+                M = torch.rand(c, size, size, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+            else:
+                G = torch.randn(c, size, size, dtype=torch.float64, device=dev, generator=g)
+                M = torch.bmm(G, G.transpose(1, 2).contiguous())
+                M += size * torch.eye(size, dtype=torch.float64, device=dev)
+            pos = (torch.from_numpy(off[idx]).to(dev)[:, None]
+                   + torch.arange(size * size, device=dev)[None, :])
+            a[pos.reshape(-1)] = M.reshape(-1).to(dt)
+    torch.cuda.synchronize()
+    return n, kind, off, a
+
+
+def bench_batched(torch, ddla, count, steps, seed, check_sample, cpu_sample, threads):
+    """Problems/s of the batched factorization (one graph replay per step), FP64 and FP32,
+    a sample of problems checked against the oracle, and the oracle's own rate."""
+    from oracle import oracle as O
+    out = {}
+    for prec, name in ((ddla.PREC_FP64, "fp64"), (ddla.PREC_FP32, "fp32")):
+        n, kind, off, a = batched_workload(torch, ddla, count, prec, seed)
+        a0 = a.clone()
+        poff = np.zeros(count, np.int64)
+        poff[1:] = np.cumsum(n.astype(np.int64))[:-1]
+        piv = torch.zeros(int(n.sum()), dtype=torch.int32, device="cuda")
+        info = torch.zeros(count, dtype=torch.int32, device="cuda")
+        plan = ddla.BatchPlan(n, kind, prec, a.data_ptr(), piv.data_ptr(), info.data_ptr(),
+                              tune=True)
+        desc = plan.describe()
+        times = []
+        for _ in range(steps):
+            a.copy_(a0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plan.execute()
+            ddla.synchronize()
+            times.append(time.perf_counter() - t0)
+        ms = 1e3 * float(np.median(times))
+        st = info.cpu().numpy()
+        # a sample of problems against the oracle (FP64 reference on the same input)
+        rng = np.random.default_rng(seed + 1)
+        sample = rng.choice(count, min(check_sample, count), replace=False)
+        P = piv.cpu().numpy()
+        worst = 0.0
+        piv_match = 0
+        nlu = 0
+        for i in sample:
+            ni = int(n[i])
+            Ai = a0[off[i]: off[i] + ni * ni].double().cpu().numpy().reshape(ni, ni).copy()
+            Fi = a[off[i]: off[i] + ni * ni].double().cpu().numpy().reshape(ni, ni)
+            if kind[i] == ddla.BATCH_LU:
+                _, po = O.getrf(Ai)
+                pi = P[poff[i]: poff[i] + ni]
+                piv_match += int(np.array_equal(pi, po))
+                nlu += 1
+                err = np.linalg.norm(Fi - Ai) / np.linalg.norm(Ai)
+            else:
+                O.potrf(Ai)
+                err = np.linalg.norm(np.tril(Fi) - np.tril(Ai)) / np.linalg.norm(np.tril(Ai))
+            worst = max(worst, float(err))
+        out[name] = {"problems": count, "ms_per_batch": ms, "problems_per_s": count / (ms / 1e3),
+                     "gflops": desc["flops"] / (ms / 1e3) / 1e9,
+                     "gbytes_per_s": desc["bytes"] / (ms / 1e3) / 1e9,
+                     "nonzero_info": int((st != 0).sum()), "plan": desc,
+                     "sample_checked": int(len(sample)),
+                     "sample_max_rel_diff_vs_oracle_fp64": worst,
+                     "sample_lu_pivots_identical": f"{piv_match}/{nlu}"}
+        log(f"batched {name}: {ms:.2f} ms for {count} problems = {count / (ms / 1e3):.3g}/s "
+            f"(nb={desc['nb']}, wg/CU={desc['wg_per_cu']}), sample max rel diff {worst:.1e}, "
+            f"pivots {piv_match}/{nlu}")
+        del plan, a, a0, piv, info
+        torch.cuda.empty_cache()
+    if cpu_sample > 0:
+        # the oracle, one problem per thread, on a sample of the same distribution
+        from concurrent.futures import ThreadPoolExecutor
+        O.set_threads(1)
+        rng = np.random.default_rng(seed + 2)
+        ns = rng.integers(8, 257, cpu_sample)
+        mats = []
+        for j, ni in enumerate(ns):
+            M = O.uniform_matrix(int(ni), int(ni), seed + 10 + j)
+            mats.append((M @ M.T + ni * np.eye(ni)) if j % 2 else M)
+
+        def run(j):
+            return O.potrf(mats[j]) if j % 2 else O.getrf(mats[j])
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(run, range(cpu_sample)))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_sample / dt, "unit": "problems/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle LU / Cholesky (FP64) of {cpu_sample} problems, "
+                                         f"n uniform on 8..256, one problem per thread: {dt:.2f}s"}
+        log("batched cpu", out["cpu_baseline"])
+    return out
+
+
 def cpu_baseline(n_target, n_sample, nrhs):
     """The oracle (CPU restatement of DDLA's managed FP64 LU + solve) on a bounded sample."""
     from oracle import oracle as O
@@ -147,6 +267,8 @@ def main():
     ap.add_argument("--sweep-kappa", default="1e2,1e4",
                     help="extra condition numbers timed (one solve each) besides --kappa")
     ap.add_argument("--seed", type=int, default=0xDD1A0003)
+    ap.add_argument("--batched", type=int, default=20000,
+                    help="problems in the batched-factorization measurement (0 skips it)")
     ap.add_argument("--qr-device", choices=["cuda", "cpu"], default="cuda",
                     help="where the test matrix's QR factors are computed")
     args = ap.parse_args()
@@ -246,6 +368,14 @@ def main():
     gemm = [time_gemm(ddla, 1024, 50), time_gemm(ddla, 8192, 5)]
     log("gemm", gemm)
 
+    # ---- batched uneven factorizations (20000 problems, FP64 and FP32)
+    batched = None
+    if args.batched > 0:
+        nproc = os.cpu_count() or 2
+        batched = bench_batched(torch, ddla, args.batched, max(3, args.steps), 0xDD1A0006,
+                                check_sample=200, cpu_sample=0 if args.no_cpu else 2000,
+                                threads=max(1, min(nproc // 2, 16)))
+
     # ---- roofline of the dominant kernel of the timed region
     dom = max(prof.items(), key=lambda kv: kv[1]["ms"])[0] if prof else None
     roof = roofline_of(dom, prof[dom], n) if dom else None
@@ -316,6 +446,7 @@ def main():
         "gemm_fp64": gemm,
         "kernel_breakdown": breakdown,
         "kernel_breakdown_fp64_lu": breakdown64,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

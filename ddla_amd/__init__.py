@@ -24,6 +24,7 @@ __all__ = [
     "CholDecompose", "CholeskySolve", "LinalgException", "gesv_mpir", "posv_mpir", "init",
     "synchronize", "NO_TRANSPOSE", "TRANSPOSE", "LOWER", "UPPER", "UNIT_DIAG", "NONUNIT_DIAG",
     "LEFT", "RIGHT", "PREC_FP64", "PREC_FP32", "PREC_FP16", "PREC_BF16", "DDLAError",
+    "BatchPlan", "batch_factor", "BATCH_LU", "BATCH_CHOL",
 ]
 
 
@@ -448,3 +449,109 @@ def posv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16,
               max_iters=50, gmres=0, terms=0) -> MpirInfo:
     """Mixed-precision Cholesky + iterative refinement (no reference counterpart)."""
     return _mpir(_lib().ddla_dposv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms)
+
+
+# ------------------------------------------------------------ batched factorizations ---
+BATCH_LU, BATCH_CHOL = 0, 1
+
+
+class _DevBuf:
+    """Raw device buffer from ddla_malloc (freed with the object)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(_lib().ddla_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(_lib().ddla_memcpy_h2d(self.ptr, _ptr(arr), arr.nbytes))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        check(_lib().ddla_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                _lib().ddla_free(self.ptr)
+        except Exception:
+            pass
+
+
+class BatchPlan:
+    """Batched factorization of many small problems of uneven order (no reference
+    counterpart; each problem is what LU.PLUDecompose / Cholesky.CholDecompose do for one
+    matrix).  Problems are packed row-major one after another in device memory; the plan
+    schedules them over one persistent launch captured in a hipGraph (ddla_batch_create).
+
+    a_ptr / piv_ptr / info_ptr are device addresses (ints or c_void_p) of the packed matrices,
+    pivots (sum n) and per-problem status (count)."""
+
+    def __init__(self, n, kind, prec, a_ptr, piv_ptr, info_ptr, tune=False):
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.kind = np.ascontiguousarray(kind, dtype=np.int32)
+        assert self.n.shape == self.kind.shape
+        self.prec = prec
+        h = ctypes.c_void_p()
+        check(_lib().ddla_batch_create(ctypes.byref(h), prec, self.n.size, _ptr(self.n),
+                                       _ptr(self.kind), ctypes.c_void_p(int(_addr(a_ptr))),
+                                       ctypes.c_void_p(int(_addr(piv_ptr))),
+                                       ctypes.c_void_p(int(_addr(info_ptr))), 1 if tune else 0))
+        self.handle = h
+
+    def execute(self):
+        check(_lib().ddla_batch_execute(self.handle))
+
+    def describe(self) -> dict:
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        check(_lib().ddla_batch_describe(self.handle, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                _lib().ddla_batch_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _addr(p):
+    if isinstance(p, ctypes.c_void_p):
+        return p.value or 0
+    return int(p)
+
+
+def batch_factor(mats, kinds, prec=PREC_FP64, tune=False):
+    """Factors host matrices (list of square numpy arrays, order <= 256) in one batched launch.
+    kinds[i] is BATCH_LU or BATCH_CHOL.  Returns (factors, pivots, info): factors in the
+    layout of PLUDecompose / CholDecompose(LOWER) (the upper triangle of a Cholesky problem
+    keeps its input), relative pivots per LU problem (None for Cholesky), status per problem."""
+    dt = np.float64 if prec == PREC_FP64 else np.float32
+    n = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    kinds = np.asarray(kinds, dtype=np.int32)
+    flat = np.concatenate([np.ascontiguousarray(m, dtype=dt).ravel() for m in mats]) if len(mats) \
+        else np.zeros(0, dt)
+    a = _DevBuf(flat.nbytes)
+    a.upload(flat)
+    piv = _DevBuf(int(n.sum()) * 4)
+    info = _DevBuf(len(mats) * 4)
+    plan = BatchPlan(n, kinds, prec, a.ptr, piv.ptr, info.ptr, tune=tune)
+    plan.execute()
+    synchronize()
+    out = a.download(dt, flat.size)
+    pv = piv.download(np.int32, int(n.sum()))
+    st = info.download(np.int32, len(mats))
+    del plan
+    factors, pivots = [], []
+    o = po = 0
+    for i, ni in enumerate(n):
+        factors.append(out[o:o + ni * ni].reshape(ni, ni).copy())
+        pivots.append(pv[po:po + ni].copy() if kinds[i] == BATCH_LU else None)
+        o += ni * ni
+        po += ni
+    return factors, pivots, st

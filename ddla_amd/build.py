@@ -31,12 +31,15 @@ CFLAGS = [
 ]
 
 
-# Per-source flags.  The panel kernels keep up to 64 rows per lane in register arrays indexed
+# Per-source flags.  The LU panel kernels keep up to 64 rows per lane in register arrays indexed
 # by unrolled constants; AMDGPU's promote-alloca pass would merge each array into one wide
 # vector value, and every uniform branch then copies the whole vector.  Left to SROA, the
 # elements stay independent registers.
 FILE_FLAGS = {
     "lu_kernels.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
+    # the batched kernels unroll every column step of a panel (register-indexed rows), which
+    # exceeds the default size limit for fully unrolled loops
+    "batched.hip": ["-mllvm", "-pragma-unroll-threshold=200000"],
 }
 
 

@@ -571,4 +571,26 @@ int ddla_damaxv_dev(ddla_dim_t n, const double* x, ddla_inc_t incx, ddla_dim_t* 
   return 0;
 }
 
+int ddla_batch_create(ddla_batch_t* plan, int prec, int64_t count, const int32_t* n,
+                      const int32_t* kind, void* a, int32_t* piv, int32_t* info, int tune) {
+  API_INIT();
+  BatchPlan* p = nullptr;
+  DDLA_RETURN_IF(batch_create(&p, prec, count, n, kind, a, piv, info, tune));
+  *plan = reinterpret_cast<ddla_batch_t>(p);
+  return 0;
+}
+
+int ddla_batch_execute(ddla_batch_t plan) {
+  API_INIT();
+  return batch_execute(reinterpret_cast<BatchPlan*>(plan));
+}
+
+int ddla_batch_describe(ddla_batch_t plan, char* buf, size_t len) {
+  return batch_describe(reinterpret_cast<const BatchPlan*>(plan), buf, len);
+}
+
+int ddla_batch_destroy(ddla_batch_t plan) {
+  return batch_destroy(reinterpret_cast<BatchPlan*>(plan));
+}
+
 }  // extern "C"

@@ -201,4 +201,12 @@ int dgetrf_nopiv_internal(int64_t n, double* a, int64_t lda, int32_t* info_host)
 int dgetrs_nopiv_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_t lda,
                           double* b, int64_t ldb);
 
+// batched uneven factorizations (batched.hip)
+struct BatchPlan;
+int batch_create(BatchPlan** out, int prec, int64_t count, const int32_t* n_host,
+                 const int32_t* kind_host, void* a, int32_t* piv, int32_t* info, int tune);
+int batch_execute(BatchPlan* p);
+int batch_destroy(BatchPlan* p);
+int batch_describe(const BatchPlan* p, char* buf, size_t len);
+
 }  // namespace ddla

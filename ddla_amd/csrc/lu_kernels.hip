@@ -26,6 +26,7 @@
 // moved rows, so the interchanges for the rest of the matrix are one parallel gather
 // (laswp_list_kernel) instead of nb sequential swaps.
 #include "ddla_internal.hpp"
+#include "wave_ops.hpp"
 
 #include <climits>
 
@@ -149,23 +150,11 @@ __device__ __forceinline__ void reg_set(X (&v)[N], int k, X x) {
   for (int j = 0; j < N; ++j) v[j] = opaque((j == k) ? x : v[j]);
 }
 
-template <typename T>
-__device__ __forceinline__ T read_lane(T v, int lane) {
-  if constexpr (sizeof(T) == 8) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
-    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
-    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
-  } else {
-    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
-  }
-}
-
 // The workgroup's R panel rows live in registers: local row lr = wave + NW k (k < R/NW) is
-// held by the 64 lanes of wave lr % NW, lane l holding columns l and l + 64; the row's origin sits in
-// LDS.  Column i of a wave's rows therefore sits in one lane: the rank-1 update takes each
-// row's multiplier with v_readlane, and every lane keeps the arg-max of its own column over
-// the rows it just updated, so the next column's local arg-max costs no extra pass.
+// held by the 64 lanes of wave lr % NW, lane l holding columns l and l + 64; the row's origin
+// sits in LDS.  Column i of a wave's rows therefore sits in one lane: the rank-1 update takes
+// each row's multiplier with v_readlane, and the next column goes to LDS for the arg-max.
+//
 // Column c of a wave's rows (lane c & 63 of v0 or v1) goes to colb[0, RW) in LDS, so that the
 // arg-max over the rows is a lane-parallel reduction instead of a scan in one lane.
 template <typename T, int RW>
@@ -182,32 +171,6 @@ __device__ __forceinline__ void column_to_lds(const T (&v0)[RW], const T (&v1)[R
   }
 }
 
-// (key, index) of a lane combined with the pair DPP brings from another lane: larger key wins,
-// the smaller index among equal keys
-template <int CTRL, int ROWMASK, typename T>
-__device__ __forceinline__ void dpp_better(T& a, int& k) {
-  T a2;
-  if constexpr (sizeof(T) == 8) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, a);
-    const unsigned long long id = __builtin_bit_cast(unsigned long long, T(-2));
-    const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(id), static_cast<int>(b), CTRL,
-                                               ROWMASK, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(id >> 32),
-                                               static_cast<int>(b >> 32), CTRL, ROWMASK, 0xf,
-                                               false);
-    a2 = __builtin_bit_cast(T, (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
-                                   static_cast<unsigned>(lo));
-  } else {
-    a2 = __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, T(-2)),
-                                                           __builtin_bit_cast(int, a), CTRL,
-                                                           ROWMASK, 0xf, false));
-  }
-  const int k2 = __builtin_amdgcn_update_dpp(INT_MAX, k, CTRL, ROWMASK, 0xf, false);
-  const bool b = a2 > a || (a2 == a && k2 < k);
-  a = b ? a2 : a;
-  k = b ? k2 : k;
-}
-
 // Arg-max of |colb[k]| over k in [kmin, kmax) across the wave: first index wins ties, a NaN
 // entry counts as -1 (below every number), -2 / INT_MAX when the range is empty.  Row of k:
 // gr0 + NW k.  Every lane ends with the result.
@@ -221,17 +184,7 @@ __device__ __forceinline__ void wave_argmax(const T* colb, int kmin, int kmax, i
   const T x = lane < RW ? colb[lane] : T(0);
   T a = (lane >= kmin && lane < kmax) ? fmax(fabs(x), T(-1)) : T(-2);
   int k = lane;
-  // Hillis-Steele over DPP row shifts, then the gfx9 row broadcasts: lane 63 ends with the
-  // whole wave's result.  Lanes whose source is outside the row (or whose row is masked off)
-  // receive the identity (-2, RW), which never wins.
-  dpp_better<0x111, 0xf>(a, k);  // row_shr:1
-  dpp_better<0x112, 0xf>(a, k);  // row_shr:2
-  dpp_better<0x114, 0xf>(a, k);  // row_shr:4
-  dpp_better<0x118, 0xf>(a, k);  // row_shr:8
-  dpp_better<0x142, 0xa>(a, k);  // row_bcast:15
-  dpp_better<0x143, 0xc>(a, k);  // row_bcast:31
-  a = read_lane(a, 63);
-  k = __builtin_amdgcn_readlane(k, 63);
+  wave_argmax_dpp(a, k);
   key = static_cast<double>(a);
   row = a > T(-2) ? gr0 + NW * k : INT_MAX;
 }

@@ -229,6 +229,31 @@ int ddla_dposv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc
                         const double* b, ddla_inc_t ldb, double* x, ddla_inc_t ldx,
                         const ddla_mpir_opts* opts, ddla_mpir_info* info);
 
+/* ------------------------------------------------ batched uneven factorizations ----- */
+/* No counterpart in the reference (DDLA factors one matrix per call: LU.PLUDecompose,
+ * DDLA/Factorizations/LU.cs:251-264, Cholesky.CholDecompose, Cholesky.cs:52-69).  Many
+ * independent square problems of order 1..256, FP64 or FP32, packed row-major one after
+ * another in one device buffer: problem i (order n[i], leading dimension n[i]) starts at
+ * element sum_{j<i} n[j]^2, its pivots (LU only, relative encoding as ddla_dgetrf_dev) at
+ * sum_{j<i} n[j] in piv, its status in info[i] (0, or the 1-based column of the first zero
+ * pivot / non-positive Cholesky pivot, where that problem stopped).
+ *
+ * A plan binds the buffers, sorts and balances the problems over one persistent launch,
+ * chooses the panel width and workgroups per CU (timing every candidate on a copy of the
+ * batch when tune != 0; the input is not modified), and captures the launch in a hipGraph.
+ * ddla_batch_execute replays the graph on the library stream (asynchronous; ddla_sync waits).
+ * n and kind are host arrays; a, piv and info are device buffers. */
+#define DDLA_BATCH_LU 0   /* LU with partial pivoting */
+#define DDLA_BATCH_CHOL 1 /* lower Cholesky (the upper triangle is not touched) */
+typedef struct ddla_batch_s* ddla_batch_t;
+int ddla_batch_create(ddla_batch_t* plan, int prec, int64_t count, const int32_t* n,
+                      const int32_t* kind, void* a, int32_t* piv, int32_t* info, int tune);
+int ddla_batch_execute(ddla_batch_t plan);
+/* JSON text: chosen panel width, workgroups per CU, workgroups, flop and byte counts, and the
+ * autotuner's candidate timings (ms) */
+int ddla_batch_describe(ddla_batch_t plan, char* buf, size_t len);
+int ddla_batch_destroy(ddla_batch_t plan);
+
 #ifdef __cplusplus
 }
 #endif
