@@ -66,6 +66,7 @@ enum WorkspaceSlot {
   WS_GMRES = 6,
   WS_SMALL = 7,   // small reduction outputs
   WS_DINV = 8,    // inverted diagonal blocks for the FP64 solves
+  WS_SPLIT = 9,   // low-precision parts of the trailing-update operands
   WS_COUNT = 12,
 };
 

@@ -57,61 +57,67 @@ __device__ __forceinline__ float block_scale(float amax, bool bf16) {
   return exp2f((float)(14 - e));
 }
 
-// scales[b] for row blocks (rows [b*HB, b*HB+HB) of an m x k row-major matrix)
-__global__ __launch_bounds__(256) void rowblock_scale_kernel(int64_t m, int64_t k,
-                                                             const float* __restrict__ X,
-                                                             int64_t ld, float* __restrict__ out,
-                                                             bool bf16) {
+// Operand preparation for the trailing update: rows of an operand (rows of A, columns of B)
+// in blocks of 128 share one power-of-two scale; each scaled FP32 value x is split into
+// TERMS low-precision parts x = h + m + l with h = rn(x), m = rn(x - h), l = rn(x - h - m),
+// written K-contiguous with rows padded to Kp (a multiple of 32) with zeros:
+// out[t][r * Kp + k].  One workgroup per 128-row block; the block is
+// staged in LDS so that both the read (KC: x(r,k) = src[r ld + k]; else src[k ld + r]) and the
+// write are coalesced.  Done once per update, not once per output tile.
+template <bool BF16, int TERMS, bool KC>
+__global__ __launch_bounds__(256) void split_kernel(int rows, int K, int Kp,
+                                                    const float* __restrict__ src, int64_t ld,
+                                                    typename LowP<BF16>::e* __restrict__ out,
+                                                    float* __restrict__ scales) {
+  typedef typename LowP<BF16>::e E;
+  __shared__ float xs[HB][NBLK + 1];
   __shared__ float red[4];
-  const int64_t r0 = (int64_t)blockIdx.x * HB;
-  const int64_t rows = min<int64_t>(HB, m - r0);
+  const int r0 = blockIdx.x * HB;
+  const int nr = min(HB, rows - r0);
+  const int tid = threadIdx.x;
   float mx = 0.0f;
-  for (int64_t t = threadIdx.x; t < rows * k; t += 256) {
-    const int64_t r = t / k, c = t % k;
-    mx = fmaxf(mx, fabsf(X[(r0 + r) * ld + c]));
+  for (int t = tid; t < HB * K; t += 256) {
+    int r, k;
+    if (KC) {
+      r = t / K;
+      k = t % K;
+    } else {
+      k = t / HB;
+      r = t % HB;
+    }
+    const float x = (r < nr) ? (KC ? src[(int64_t)(r0 + r) * ld + k] : src[(int64_t)k * ld + r0 + r])
+                             : 0.0f;
+    xs[r][k] = x;
+    mx = fmaxf(mx, fabsf(x));
   }
   for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  if ((tid & 63) == 0) red[tid >> 6] = mx;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    out[blockIdx.x] = block_scale(mx, bf16);
+  const float sc = block_scale(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])), BF16);
+  if (tid == 0) scales[blockIdx.x] = sc;
+  const size_t plane = (size_t)rows * Kp;
+  for (int t = tid; t < nr * Kp; t += 256) {
+    const int r = t / Kp, k = t % Kp;
+    float x = k < K ? xs[r][k] * sc : 0.0f;
+#pragma unroll
+    for (int q = 0; q < TERMS; ++q) {
+      const E h = static_cast<E>(x);
+      out[q * plane + (size_t)(r0 + r) * Kp + k] = h;
+      x = x - static_cast<float>(h);
+    }
   }
 }
 
-// scales[b] for column blocks (columns [b*HB, b*HB+HB) of a k x n row-major matrix)
-__global__ __launch_bounds__(256) void colblock_scale_kernel(int64_t k, int64_t n,
-                                                             const float* __restrict__ X,
-                                                             int64_t ld, float* __restrict__ out,
-                                                             bool bf16) {
-  __shared__ float red[4];
-  const int64_t c0 = (int64_t)blockIdx.x * HB;
-  const int64_t cols = min<int64_t>(HB, n - c0);
-  float mx = 0.0f;
-  for (int64_t t = threadIdx.x; t < cols * k; t += 256) {
-    const int64_t r = t / cols, c = t % cols;
-    mx = fmaxf(mx, fabsf(X[r * ld + c0 + c]));
-  }
-  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    out[blockIdx.x] = block_scale(mx, bf16);
-  }
-}
-
-// C (FP32, m x n) = C + alpha * (A * B); A m x k FP32 (k contiguous); B k x n FP32,
-// n contiguous (B_KC = false) or k contiguous, B(k,j) at B[j*ldb + k] (B_KC = true).
-// 128 x 128 tile per workgroup, K step 32.  Each scaled FP32 operand x is split into
-// TERMS low-precision parts while it is staged into LDS: x = h + m + l with h = rn(x),
-// m = rn(x - h), l = rn(x - h - m).  The tile product sums the part products whose
-// combined weight reaches the FP32 level (TERMS=1: hh; 2: hh+hm+mh; 3: hh+hm+mh+hl+lh+mm),
-// all accumulated in FP32 by the matrix cores.
-template <bool BF16, int TERMS, bool B_KC, int TRI>
+// C (FP32, m x n) += alpha * A B from the split operands Ap[t][m][K], Bp[t][n][K] (both
+// K-contiguous, K a multiple of 32 with zero padding) with FP32 accumulation on the matrix cores (v_mfma_f32_16x16x32_{f16,bf16}).
+// The tile product sums the part products whose combined weight reaches the FP32 level
+// (TERMS=1: hh; 2: hh+hm+mh; 3: hh+hm+mh+hl+lh+mm), small ones first.  128 x 128 tile per
+// workgroup, K step 32 through LDS with register prefetch; the C tile is read at the start
+// (its latency overlaps the products) and updated once at the end.  TRI == 1: lower triangle.
+template <bool BF16, int TERMS, int TRI>
 __global__ __launch_bounds__(256) void hgemm_kernel(int M, int N, int K, float alpha,
-                                                    const float* __restrict__ A, int64_t lda,
-                                                    const float* __restrict__ B, int64_t ldb,
+                                                    const typename LowP<BF16>::e* __restrict__ Ap,
+                                                    const typename LowP<BF16>::e* __restrict__ Bp,
                                                     float* __restrict__ C, int64_t ldc,
                                                     const float* __restrict__ sa,
                                                     const float* __restrict__ sb, int tiles_m,
@@ -120,8 +126,8 @@ __global__ __launch_bounds__(256) void hgemm_kernel(int M, int N, int K, float a
   typedef typename LP::v8 v8;
   typedef typename LP::e E;
   constexpr int BM = 128, BN = 128, BK = 32, PK = BK + 8;  // padded rows: conflict-free b128 reads
-  __shared__ __attribute__((aligned(16))) E As[2][TERMS][BM * PK];
-  __shared__ __attribute__((aligned(16))) E Bs[2][TERMS][BN * PK];
+  __shared__ __attribute__((aligned(16))) E As[TERMS][BM * PK];
+  __shared__ __attribute__((aligned(16))) E Bs[TERMS][BN * PK];
 
   int tmi, tni;
   gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
@@ -131,6 +137,7 @@ __global__ __launch_bounds__(256) void hgemm_kernel(int M, int N, int K, float a
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int l16 = lane & 15, lk = lane >> 4;
+  const size_t pa = (size_t)M * K, pb = (size_t)N * K;
 
   f4 acc[4][4];
 #pragma unroll
@@ -138,128 +145,106 @@ __global__ __launch_bounds__(256) void hgemm_kernel(int M, int N, int K, float a
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  // staging: 2 items per lane for A (row, k-group of 8) and for B (column, k-group of 8)
-  float ra[2][8], rb[2][8];
+  // staging: per term, 2 items of 8 k-values per lane for A and for B
+  v8 ra[TERMS][2], rb[TERMS][2];
   auto load = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int v = tid + 256 * q;
-      {
-        const int row = v >> 2, kg = (v & 3) * 8;
-        const int gi = m0 + row;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int gk = k0 + kg + e;
-          ra[q][e] = (gi < M && gk < K) ? A[(int64_t)gi * lda + gk] : 0.f;
-        }
-      }
-      if (B_KC) {
-        const int col = v >> 2, kg = (v & 3) * 8;
-        const int gj = n0 + col;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int gk = k0 + kg + e;
-          rb[q][e] = (gj < N && gk < K) ? B[(int64_t)gj * ldb + gk] : 0.f;
-        }
-      } else {
-        const int col = v & 127, kg = (v >> 7) * 8;
-        const int gj = n0 + col;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int gk = k0 + kg + e;
-          rb[q][e] = (gj < N && gk < K) ? B[(int64_t)gk * ldb + gj] : 0.f;
-        }
-      }
-    }
-  };
-  auto split = [&](const float (&x)[8], float sc, v8 (&parts)[TERMS]) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      float r = x[e] * sc;
+      const int row = v >> 2, kg = (v & 3) * 8;
+      const int gi = m0 + row, gj = n0 + row, gk = k0 + kg;
 #pragma unroll
       for (int t = 0; t < TERMS; ++t) {
-        const E h = static_cast<E>(r);
-        parts[t][e] = h;
-        r = r - static_cast<float>(h);
+        v8 za = {}, zb = {};
+        ra[t][q] = gi < M ? *reinterpret_cast<const v8*>(Ap + t * pa + (size_t)gi * K + gk) : za;
+        rb[t][q] = gj < N ? *reinterpret_cast<const v8*>(Bp + t * pb + (size_t)gj * K + gk) : zb;
       }
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&]() {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int v = tid + 256 * q;
-      v8 pa[TERMS], pb[TERMS];
-      split(ra[q], scA, pa);
-      split(rb[q], scB, pb);
-      const int arow = v >> 2, akg = (v & 3) * 8;
-      int bcol, bkg;
-      if (B_KC) {
-        bcol = v >> 2;
-        bkg = (v & 3) * 8;
-      } else {
-        bcol = v & 127;
-        bkg = (v >> 7) * 8;
-      }
+      const int row = v >> 2, kg = (v & 3) * 8;
 #pragma unroll
       for (int t = 0; t < TERMS; ++t) {
-        *reinterpret_cast<v8*>(&As[buf][t][arow * PK + akg]) = pa[t];
-        *reinterpret_cast<v8*>(&Bs[buf][t][bcol * PK + bkg]) = pb[t];
+        *reinterpret_cast<v8*>(&As[t][row * PK + kg]) = ra[t][q];
+        *reinterpret_cast<v8*>(&Bs[t][row * PK + kg]) = rb[t][q];
       }
     }
   };
 
-  const int nk = (K + BK - 1) / BK;
+  const int nk = K / BK;
   load(0);
-  store(0);
-  __syncthreads();
+  // The products run as B^T A^T (operands swapped), so each lane's four results are four
+  // consecutive columns of one row of C: 16-byte C accesses.  C is read right after the first
+  // operands (its latency overlaps the products; requested after them, it does not delay them).
+  const bool vec = (ldc & 3) == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
+  f4 cv[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gi = m0 + wm * 64 + i * 16 + l16;
+      const int gj = n0 + wn * 64 + j * 16 + lk * 4;
+      const float* p = C + (int64_t)gi * ldc + gj;
+      if (vec && gi < M && gj + 3 < N) {
+        cv[i][j] = *reinterpret_cast<const f4*>(p);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[i][j][r] = (gi < M && gj + r < N) ? p[r] : 0.f;
+      }
+    }
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
+    __syncthreads();  // the previous step's reads of As / Bs are done
+    store();
+    __syncthreads();
     if (kt + 1 < nk) load((kt + 1) * BK);
     v8 a[TERMS][4], b[TERMS][4];
 #pragma unroll
     for (int t = 0; t < TERMS; ++t) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        a[t][i] = *reinterpret_cast<const v8*>(&As[cur][t][(wm * 64 + i * 16 + l16) * PK + lk * 8]);
+        a[t][i] = *reinterpret_cast<const v8*>(&As[t][(wm * 64 + i * 16 + l16) * PK + lk * 8]);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        b[t][j] = *reinterpret_cast<const v8*>(&Bs[cur][t][(wn * 64 + j * 16 + l16) * PK + lk * 8]);
+        b[t][j] = *reinterpret_cast<const v8*>(&Bs[t][(wn * 64 + j * 16 + l16) * PK + lk * 8]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        // small products first
         if (TERMS >= 3) {
-          acc[i][j] = LP::mfma(a[1][i], b[1][j], acc[i][j]);
-          acc[i][j] = LP::mfma(a[2][i], b[0][j], acc[i][j]);
-          acc[i][j] = LP::mfma(a[0][i], b[2][j], acc[i][j]);
+          acc[i][j] = LP::mfma(b[1][j], a[1][i], acc[i][j]);
+          acc[i][j] = LP::mfma(b[0][j], a[2][i], acc[i][j]);
+          acc[i][j] = LP::mfma(b[2][j], a[0][i], acc[i][j]);
         }
         if (TERMS >= 2) {
-          acc[i][j] = LP::mfma(a[1][i], b[0][j], acc[i][j]);
-          acc[i][j] = LP::mfma(a[0][i], b[1][j], acc[i][j]);
+          acc[i][j] = LP::mfma(b[0][j], a[1][i], acc[i][j]);
+          acc[i][j] = LP::mfma(b[1][j], a[0][i], acc[i][j]);
         }
-        acc[i][j] = LP::mfma(a[0][i], b[0][j], acc[i][j]);
+        acc[i][j] = LP::mfma(b[0][j], a[0][i], acc[i][j]);
       }
-    if (kt + 1 < nk) store(cur ^ 1);
-    __syncthreads();
   }
   const float f = alpha / (scA * scB);  // exact: the scales are powers of two
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+      const int gi = m0 + wm * 64 + i * 16 + l16;
+      const int gj = n0 + wn * 64 + j * 16 + lk * 4;
+      float* p = C + (int64_t)gi * ldc + gj;
+      f4 o;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wm * 64 + i * 16 + lk * 4 + r;
-        const int gj = n0 + wn * 64 + j * 16 + l16;
-        bool ok = gi < M && gj < N;
-        if (TRI == 1) ok = ok && gj <= gi;
-        if (ok) {
-          float* p = C + (int64_t)gi * ldc + gj;
-          *p = fmaf(f, acc[i][j][r], *p);
-        }
+      for (int r = 0; r < 4; ++r) o[r] = fmaf(f, acc[i][j][r], cv[i][j][r]);
+      if (vec && gi < M && gj + 3 < N && (TRI == 0 || gj + 3 <= gi)) {
+        *reinterpret_cast<f4*>(p) = o;
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (gi < M && gj + r < N && (TRI == 0 || gj + r <= gi)) p[r] = o[r];
       }
+    }
 }
 
 __global__ void d2s_kernel(int64_t m, int64_t n, const double* __restrict__ src, int64_t lds,
@@ -495,43 +480,55 @@ struct Timer {
 
 }  // namespace
 
-// C += alpha * A * B with FP16/BF16 operands and FP32 accumulation (see hgemm_kernel).
+// C += alpha * A * B with FP16/BF16 operands and FP32 accumulation (see hgemm_kernel): A m x k
+// (k contiguous); B k x n with n contiguous, or (b_kcontig) B(k, j) at B[j ldb + k]; k <= 128.
 int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k,
                  float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
                  int64_t ldc, float* scale_ws, hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
+  if (k > NBLK) {
+    set_error("hgemm: k = %lld exceeds %d", (long long)k, NBLK);
+    return DDLA_ERR_INVALID_ARG;
+  }
+  terms = std::min(3, std::max(1, terms));
   const int tm = ceil_div(m, 128), tn = ceil_div(n, 128);
   float* sa = scale_ws;
   float* sb = scale_ws + tm;
-  hipLaunchKernelGGL(rowblock_scale_kernel, dim3(tm), dim3(256), 0, s, m, k, A, lda, sa, bf16);
-  if (b_kcontig)
-    hipLaunchKernelGGL(rowblock_scale_kernel, dim3(tn), dim3(256), 0, s, n, k, B, ldb, sb, bf16);
-  else
-    hipLaunchKernelGGL(colblock_scale_kernel, dim3(tn), dim3(256), 0, s, k, n, B, ldb, sb, bf16);
-  DDLA_HIP_CHECK(hipGetLastError());
-  dim3 grid((unsigned)((int64_t)tm * tn)), block(256);
-  const int M = (int)m, N = (int)n, K = (int)k;
+  void* ws = nullptr;
+  const int K = (int)k, Kp = (K + 31) / 32 * 32;
+  DDLA_RETURN_IF(workspace(WS_SPLIT, (size_t)terms * (m + n) * Kp * 2, &ws));
+  const int M = (int)m, N = (int)n;
   const double mn = (tri ? 0.5 : 1.0) * (double)m * n;
-  ProfScope prof(bf16 ? "hgemm_mfma_bf16" : "hgemm_mfma_f16", 2.0 * mn * k * std::max(1, terms * (terms + 1) / 2),
-                 4.0 * ((double)m * k + (double)k * n + 2.0 * mn), s);
-#define HG(BF, TM, KC, T)                                                                  \
-  hipLaunchKernelGGL((hgemm_kernel<BF, TM, KC, T>), grid, block, 0, s, M, N, K, alpha, A, lda, \
-                     B, ldb, C, ldc, sa, sb, tm, tn)
-#define HG_T(BF, TM)                                         \
-  do {                                                       \
-    if (b_kcontig) {                                         \
-      if (tri) HG(BF, TM, true, 1); else HG(BF, TM, true, 0); \
-    } else {                                                 \
-      if (tri) HG(BF, TM, false, 1); else HG(BF, TM, false, 0); \
-    }                                                        \
+  ProfScope prof(bf16 ? "hgemm_mfma_bf16" : "hgemm_mfma_f16",
+                 2.0 * mn * k * std::max(1, terms * (terms + 1) / 2),
+                 4.0 * 2.0 * mn + 2.0 * terms * ((double)m + n) * k, s);
+#define SPLIT(BF, TM, KC, rows, src, ld, out, sc)                                            \
+  hipLaunchKernelGGL((split_kernel<BF, TM, KC>), dim3(ceil_div(rows, HB)), dim3(256), 0, s, \
+                     (int)(rows), K, Kp, src, ld, out, sc)
+#define HG(BF, TM, T)                                                                        \
+  do {                                                                                       \
+    typedef typename LowP<BF>::e E;                                                          \
+    E* ap = static_cast<E*>(ws);                                                             \
+    E* bp = ap + (size_t)TM * m * Kp;                                                        \
+    SPLIT(BF, TM, true, m, A, lda, ap, sa);                                                  \
+    if (b_kcontig) SPLIT(BF, TM, true, n, B, ldb, bp, sb);                                   \
+    else SPLIT(BF, TM, false, n, B, ldb, bp, sb);                                            \
+    hipLaunchKernelGGL((hgemm_kernel<BF, TM, T>), dim3((unsigned)((int64_t)tm * tn)),        \
+                       dim3(256), 0, s, M, N, Kp, alpha, ap, bp, C, ldc, sa, sb, tm, tn);    \
+  } while (0)
+#define HG_T(BF, TM)           \
+  do {                         \
+    if (tri) HG(BF, TM, 1);    \
+    else HG(BF, TM, 0);        \
   } while (0)
   if (bf16) {
-    if (terms <= 1) HG_T(true, 1); else if (terms == 2) HG_T(true, 2); else HG_T(true, 3);
+    if (terms == 1) HG_T(true, 1); else if (terms == 2) HG_T(true, 2); else HG_T(true, 3);
   } else {
-    if (terms <= 1) HG_T(false, 1); else if (terms == 2) HG_T(false, 2); else HG_T(false, 3);
+    if (terms == 1) HG_T(false, 1); else if (terms == 2) HG_T(false, 2); else HG_T(false, 3);
   }
 #undef HG_T
 #undef HG
+#undef SPLIT
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
 }
