@@ -312,21 +312,78 @@ __global__ __launch_bounds__(1024) void max_kernel(int64_t n, const double* __re
 }
 
 // per column j of the n x nrhs matrices: out[2j] = ||R_j||_inf, out[2j+1] = ||X_j||_inf
+// out[2j] = max_i |R[i][j]|, out[2j+1] = max_i |X[i][j]| (NaN counts as infinity) as the bit
+// patterns of non-negative doubles, which order like the values: column j = blockIdx.y,
+// a wave-level maximum first, then one atomicMax per wave.
 __global__ __launch_bounds__(256) void col_inf_norms_kernel(int64_t n, int64_t nrhs,
                                                             const double* __restrict__ R,
                                                             const double* __restrict__ X,
                                                             int64_t ldx,
                                                             unsigned long long* __restrict__ out) {
-  // non-negative doubles order like their bit patterns: atomicMax on the bits
-  const int64_t total = n * nrhs;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = t / nrhs, j = t % nrhs;
-    double r = fabs(R[t]), x = fabs(X[i * ldx + j]);
+  const int64_t j = blockIdx.y;
+  double rm = 0.0, xm = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double r = fabs(R[i * nrhs + j]), x = fabs(X[i * ldx + j]);
     if (r != r) r = INFINITY;
     if (x != x) x = INFINITY;
-    atomicMax(out + 2 * j, __double_as_longlong(r));
-    atomicMax(out + 2 * j + 1, __double_as_longlong(x));
+    rm = fmax(rm, r);
+    xm = fmax(xm, x);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    rm = fmax(rm, __shfl_xor(rm, off));
+    xm = fmax(xm, __shfl_xor(xm, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out + 2 * j, __double_as_longlong(rm));
+    atomicMax(out + 2 * j + 1, __double_as_longlong(xm));
+  }
+}
+
+// R[i][c] -= sum_k A[i][k] X[k][c] for NR right-hand sides: one wave per RPW rows of the
+// FP64 matrix, lanes along the rows (coalesced 512-byte reads; each X value serves RPW rows),
+// a butterfly sum at the end.  The refinement's residual reads A once per step, so this runs
+// at HBM speed.
+template <int NR>
+__global__ __launch_bounds__(256) void dgemv_resid_kernel(int64_t n, const double* __restrict__ A,
+                                                          int64_t lda, const double* __restrict__ X,
+                                                          int64_t ldx, double* __restrict__ R,
+                                                          int64_t ldr, int nrhs) {
+  constexpr int RPW = 4;
+  const int64_t row0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW;
+  if (row0 >= n) return;
+  const int lane = threadIdx.x & 63;
+  const double* a[RPW];
+#pragma unroll
+  for (int q = 0; q < RPW; ++q) a[q] = A + min(row0 + q, n - 1) * lda;
+  double acc[RPW][NR];
+#pragma unroll
+  for (int q = 0; q < RPW; ++q)
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[q][c] = 0.0;
+#pragma unroll 2
+  for (int64_t k = lane; k < n; k += 64) {
+    double xv[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) xv[c] = c < nrhs ? X[k * ldx + c] : 0.0;
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const double av = a[q][k];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[q][c] = fma(av, xv[c], acc[q][c]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RPW; ++q)
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      for (int off = 32; off > 0; off >>= 1) acc[q][c] += __shfl_xor(acc[q][c], off);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < RPW; ++q)
+#pragma unroll
+      for (int c = 0; c < NR; ++c)
+        if (c < nrhs && row0 + q < n) R[(row0 + q) * ldr + c] -= acc[q][c];
   }
 }
 
@@ -754,10 +811,22 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
       hipLaunchKernelGGL(update_x_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.D, X,
                          ldx, B, ldb, w.R);
     DDLA_HIP_CHECK(hipGetLastError());
-    DDLA_RETURN_IF(launch_dgemm(n, nrhs, n, -1.0, A, lda, true, X, ldx, true, 1.0, w.R, nrhs, 0, s));
+    if (nrhs <= 8) {
+      ProfScope prof("dgemv_resid_f64", 2.0 * n * n * nrhs, 8.0 * n * n, s);
+      const dim3 g((unsigned)ceil_div(n, 16));  // 4 waves x 4 rows per workgroup
+      if (nrhs == 1)
+        hipLaunchKernelGGL(dgemv_resid_kernel<1>, g, dim3(256), 0, s, n, A, lda, X, ldx, w.R,
+                           nrhs, (int)nrhs);
+      else
+        hipLaunchKernelGGL(dgemv_resid_kernel<8>, g, dim3(256), 0, s, n, A, lda, X, ldx, w.R,
+                           nrhs, (int)nrhs);
+    } else {
+      DDLA_RETURN_IF(launch_dgemm(n, nrhs, n, -1.0, A, lda, true, X, ldx, true, 1.0, w.R, nrhs, 0, s));
+    }
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, s, 2 * nrhs, w.norms);
-    hipLaunchKernelGGL(col_inf_norms_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.R, X,
-                       ldx, w.norms);
+    hipLaunchKernelGGL(col_inf_norms_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 64),
+                                                  (unsigned)nrhs),
+                       dim3(256), 0, s, n, nrhs, w.R, X, ldx, w.norms);
     DDLA_HIP_CHECK(hipGetLastError());
     DDLA_HIP_CHECK(hipMemcpyAsync(nh.data(), w.norms, nh.size() * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, s));
