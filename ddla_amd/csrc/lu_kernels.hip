@@ -28,6 +28,7 @@
 #include "ddla_internal.hpp"
 #include "wave_ops.hpp"
 
+#include <algorithm>
 #include <climits>
 
 namespace ddla {
@@ -441,19 +442,25 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
 }
 
 // Applies the (dst <- src) row list of a panel to columns [0, ncols) of A (rows relative to
-// row0), skipping columns [skip_lo, skip_hi).  Two phases per column through tmp so that a
-// source row is read before any destination write.
-template <typename T>
-__global__ void laswp_list_kernel(T* __restrict__ A, int64_t ld, int64_t row0, int64_t ncols,
-                                  int64_t skip_lo, int64_t skip_hi, const int* __restrict__ list,
-                                  const int* __restrict__ count, T* __restrict__ tmp) {
-  const int cnt = *count;
+// row0), skipping columns [skip_lo, skip_hi), in two launches so that every source row is read
+// before any destination row is written: gather the source rows into tmp, then scatter tmp
+// into the destination rows.  blockIdx.y is the list entry, x runs along the row (coalesced).
+template <typename T, bool GATHER>
+__global__ __launch_bounds__(256) void laswp_list_kernel(T* __restrict__ A, int64_t ld, int64_t row0,
+                                                         int64_t ncols, int64_t skip_lo,
+                                                         int64_t skip_hi,
+                                                         const int* __restrict__ list,
+                                                         const int* __restrict__ count,
+                                                         T* __restrict__ tmp) {
+  const int e = blockIdx.y;
+  if (e >= *count) return;
   const int64_t width = ncols - (skip_hi - skip_lo);
+  const int64_t row = row0 + list[2 * e + (GATHER ? 1 : 0)];
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < width;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = (t < skip_lo) ? t : t + (skip_hi - skip_lo);
-    for (int e = 0; e < cnt; ++e) tmp[(int64_t)e * width + t] = A[(row0 + list[2 * e + 1]) * ld + c];
-    for (int e = 0; e < cnt; ++e) A[(row0 + list[2 * e]) * ld + c] = tmp[(int64_t)e * width + t];
+    if (GATHER) tmp[(int64_t)e * width + t] = A[row * ld + c];
+    else A[row * ld + c] = tmp[(int64_t)e * width + t];
   }
 }
 
@@ -638,8 +645,11 @@ int launch_laswp_list(T* A, int64_t ld, int64_t row0, int64_t ncols, int64_t ski
   DDLA_RETURN_IF(workspace(WS_TEMP, (size_t)max_count * width * sizeof(T), &tmp));
   // bytes: every moved row (at most max_count) is read and written once
   ProfScope prof("laswp_list", 0.0, 2.0 * sizeof(T) * max_count * width, s);
-  hipLaunchKernelGGL(laswp_list_kernel<T>, dim3(grid_for(width)), dim3(256), 0, s, A, ld, row0,
-                     ncols, skip_lo, skip_hi, list, count, static_cast<T*>(tmp));
+  const dim3 grid((unsigned)std::min<int64_t>(ceil_div(width, 256), 64), (unsigned)max_count);
+  hipLaunchKernelGGL((laswp_list_kernel<T, true>), grid, dim3(256), 0, s, A, ld, row0, ncols,
+                     skip_lo, skip_hi, list, count, static_cast<T*>(tmp));
+  hipLaunchKernelGGL((laswp_list_kernel<T, false>), grid, dim3(256), 0, s, A, ld, row0, ncols,
+                     skip_lo, skip_hi, list, count, static_cast<T*>(tmp));
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
 }

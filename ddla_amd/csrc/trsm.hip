@@ -5,9 +5,8 @@
 // (DDLA/Factorizations/Cholesky.cs:156) and CholeskySolveUncheck (Cholesky.cs:210-226).
 //
 // Structure: the triangle is cut into 128-row diagonal blocks.  A diagonal block is
-// solved by trsm_slab_kernel: one workgroup per 64 right-hand-side columns keeps the
-// packed triangle and its 128 x 64 slab of B in LDS and substitutes row by row (lanes
-// run along the slab's columns, so every LDS access is conflict-free or a broadcast).
+// solved by trsm_slab_kernel: the packed triangle sits in LDS and each lane substitutes
+// down one right-hand-side column on its own (no barriers).
 // The off-diagonal coupling is a GEMM update on the matrix cores (launch_dgemm).
 #include "ddla_internal.hpp"
 
@@ -18,98 +17,63 @@ namespace {
 
 constexpr int TB = 128;  // diagonal block size
 
-// Solves op(T) X = B in place for one bs x bs triangle T (strides rs, cs) and the
-// bs x ncols block B.  LOWER selects forward substitution.  Non-unit diagonals divide.
+// Solves op(T) X = B in place for one bs x bs triangle T (strides rs, cs) and the bs x ncols
+// block B.  LOWER selects forward substitution; non-unit diagonals divide.
+//
+// Each lane owns one column of B and solves it alone, so there is no barrier after the
+// triangle is staged in LDS.  Everything runs in solve-order positions q (row q for LOWER,
+// row bs-1-q otherwise): M[qk][q] is the coupling of the already solved position qk into
+// position q > qk, so the couplings a step needs are contiguous (16-byte LDS reads, the
+// same address in every lane: broadcasts) and M[q][q] is the diagonal.  Positions go in
+// sub-blocks of SB: the sub-block's values sit in registers, take the contributions of the
+// positions already solved (read back from B, in order), then the substitution inside; each
+// x_q = (b_q - sum_k t_qk x_k) / t_qq is the FMA chain in k order of a row-by-row solve.
 template <typename T, bool LOWER, bool UNIT>
 __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
                                                         const T* __restrict__ A, int64_t rs,
                                                         int64_t cs, T* __restrict__ B,
                                                         int64_t ldb) {
-  __shared__ T Ls[TB * (TB + 1) / 2];
-  __shared__ T S[TB * 64];
+  constexpr int SB = 16;
+  __shared__ __attribute__((aligned(16))) T M[TB][TB];
   const int tid = threadIdx.x;
-  const int64_t c0 = (int64_t)blockIdx.x * 64;
-  const int cw = (int)min<int64_t>(64, ncols - c0);
-  // packed triangle: lower (r >= c) at r(r+1)/2 + c; upper (r <= c) stored as its
-  // transpose, at c(c+1)/2 + r
-  // one flat loop over the packed index so every lane's loads are independent
-  const int npk = bs * (bs + 1) / 2;
-  for (int p = tid; p < npk; p += 256) {
-    int r = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
-    while (r * (r + 1) / 2 > p) --r;
-    while ((r + 1) * (r + 2) / 2 <= p) ++r;
-    const int c = p - r * (r + 1) / 2;
-    Ls[p] = LOWER ? A[r * rs + c * cs] : A[c * rs + r * cs];
-  }
-  for (int idx = tid; idx < bs * 64; idx += 256) {
-    const int r = idx >> 6, cc = idx & 63;
-    S[idx] = (cc < cw) ? B[(int64_t)r * ldb + c0 + cc] : static_cast<T>(0);
+  auto row_of = [&](int q) { return LOWER ? q : bs - 1 - q; };
+  for (int p = tid; p < bs * bs; p += 256) {
+    const int qk = p / bs, q = p % bs;  // q runs along memory for the lower case
+    const int r = row_of(q), c = row_of(qk);
+    // element (r, c) of the stored triangle: lower r >= c, upper r <= c
+    M[qk][q] = (q >= qk) ? A[(int64_t)r * rs + (int64_t)c * cs] : T(0);
   }
   __syncthreads();
-  const int cc = tid & 63, g = tid >> 6;
-  // entry (r, i) of the triangle in packed form
-  auto tri = [&](int r, int i) -> T {
-    return LOWER ? Ls[r * (r + 1) / 2 + i] : Ls[i * (i + 1) / 2 + r];
-  };
-  // row r of the slab is owned by group g = r % 4; each step loads a batch of owned rows
-  // before storing them (one LDS round trip per batch instead of one per row)
-  constexpr int CH = 8;
-  if (LOWER) {
-    if (!UNIT && g == 0) S[cc] = S[cc] / tri(0, 0);
-    __syncthreads();
-    for (int i = 0; i < bs - 1; ++i) {
-      const T x = S[i * 64 + cc];
-      for (int k0 = 0; k0 < TB / 4; k0 += CH) {
-        T v[CH], l[CH];
+  const int64_t col = (int64_t)blockIdx.x * 256 + tid;
+  if (col >= ncols) return;
+  T* b = B + col;
+  constexpr int VW = 16 / sizeof(T);  // elements per 16-byte read
+  typedef T vec __attribute__((ext_vector_type(VW)));
+  for (int s0 = 0; s0 < bs; s0 += SB) {
+    T x[SB];
 #pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          const int r = g + 4 * (k0 + k);
-          v[k] = S[r * 64 + cc];
-          l[k] = (r > i && r < bs) ? tri(r, i) : T(0);
-        }
+    for (int j = 0; j < SB; ++j) x[j] = (s0 + j < bs) ? b[(int64_t)row_of(s0 + j) * ldb] : T(0);
+    // positions already solved, in order
+#pragma unroll 2
+    for (int qk = 0; qk < s0; ++qk) {
+      const T xk = b[(int64_t)row_of(qk) * ldb];
 #pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          const int r = g + 4 * (k0 + k);
-          if (r > i && r < bs) {
-            T w = fma(-l[k], x, v[k]);
-            if (!UNIT && r == i + 1) w = w / tri(r, r);
-            S[r * 64 + cc] = w;
-          }
-        }
+      for (int v = 0; v < SB / VW; ++v) {
+        const vec m = *reinterpret_cast<const vec*>(&M[qk][s0 + v * VW]);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) x[v * VW + e] = fma(-m[e], xk, x[v * VW + e]);
       }
-      __syncthreads();
     }
-  } else {
-    const int last = bs - 1;
-    if (!UNIT && g == 0) S[last * 64 + cc] = S[last * 64 + cc] / tri(last, last);
-    __syncthreads();
-    for (int i = last; i > 0; --i) {
-      const T x = S[i * 64 + cc];
-      for (int k0 = 0; k0 < TB / 4; k0 += CH) {
-        T v[CH], l[CH];
+    // inside the sub-block (entries past bs are zero couplings of zero values)
 #pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          const int r = g + 4 * (k0 + k);
-          v[k] = S[r * 64 + cc];
-          // upper: row r < i couples through element (r, i) = stored transpose
-          l[k] = (r < i) ? tri(r, i) : T(0);
-        }
+    for (int i = 0; i < SB; ++i) {
+      if (!UNIT && s0 + i < bs) x[i] = x[i] / M[s0 + i][s0 + i];
 #pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          const int r = g + 4 * (k0 + k);
-          if (r < i) {
-            T w = fma(-l[k], x, v[k]);
-            if (!UNIT && r == i - 1) w = w / tri(r, r);
-            S[r * 64 + cc] = w;
-          }
-        }
-      }
-      __syncthreads();
+      for (int j = i + 1; j < SB; ++j) x[j] = fma(-M[s0 + i][s0 + j], x[i], x[j]);
     }
-  }
-  for (int idx = tid; idx < bs * 64; idx += 256) {
-    const int r = idx >> 6, c = idx & 63;
-    if (c < cw) B[(int64_t)r * ldb + c0 + c] = S[idx];
+#pragma unroll
+    for (int j = 0; j < SB; ++j)
+      if (s0 + j < bs) b[(int64_t)row_of(s0 + j) * ldb] = x[j];
   }
 }
 
@@ -123,7 +87,7 @@ int launch_trsm_slab(bool lower, bool unit, int bs, int64_t ncols, const T* A, i
     set_error("trsm block %d > %d", bs, TB);
     return DDLA_ERR_INVALID_ARG;
   }
-  dim3 grid(ceil_div(ncols, 64)), block(256);
+  dim3 grid(ceil_div(ncols, 256)), block(256);
   ProfScope prof(sizeof(T) == 8 ? "trsm_slab_f64" : "trsm_slab_f32", (double)bs * bs * ncols,
                  sizeof(T) * (2.0 * bs * ncols + 0.5 * bs * bs), s);
   if (lower && unit)
