@@ -46,6 +46,10 @@ struct Context {
   int num_cus = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
+  // second stream of the library's own (look-ahead: trailing updates overlap the next panel),
+  // default priority like the main one; events order the two
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev[4] = {};
   // growable scratch buffers (device memory kept for the life of the context)
   void* ws[12] = {};
   size_t ws_bytes[12] = {};
@@ -54,6 +58,8 @@ struct Context {
 Context& ctx();
 int ensure_init();
 hipStream_t stream();
+// the auxiliary stream and its events (created on first use)
+int aux_stream(hipStream_t* s, hipEvent_t** ev);
 // Returns a device scratch buffer `slot` of at least `bytes` (contents undefined).
 int workspace(int slot, size_t bytes, void** out);
 enum WorkspaceSlot {
@@ -67,6 +73,8 @@ enum WorkspaceSlot {
   WS_SMALL = 7,   // small reduction outputs
   WS_DINV = 8,    // inverted diagonal blocks for the FP64 solves
   WS_SPLIT = 9,   // low-precision parts of the trailing-update operands
+  WS_SPLIT2 = 10, // ... for updates on the auxiliary stream
+  WS_TEMP2 = 11,  // row-interchange staging on the auxiliary stream
   WS_COUNT = 12,
 };
 
@@ -130,7 +138,7 @@ int launch_sgemm(int64_t m, int64_t n, int64_t k, float alpha, const float* A, i
 // rounded per 128-block with power-of-two scales; scale_ws holds >= tiles_m + tiles_n floats
 int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k,
                  float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                 int64_t ldc, float* scale_ws, hipStream_t s);
+                 int64_t ldc, float* scale_ws, hipStream_t s, int ws_slot = WS_SPLIT);
 
 // general strided copy / transpose helpers
 int launch_dcopy2d(int64_t m, int64_t n, const double* src, int64_t rs, int64_t cs, double* dst,
@@ -150,7 +158,7 @@ int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int
 template <typename T>
 int launch_laswp_list(T* A, int64_t ld, int64_t row0, int64_t ncols, int64_t skip_lo,
                       int64_t skip_hi, const int* list, const int* count, int max_count,
-                      hipStream_t s);
+                      hipStream_t s, int ws_slot = WS_TEMP);
 int launch_perm_compose(int n, int npanels, int nb, const int* lists, const int* counts, int* perm,
                         hipStream_t s);
 

@@ -638,11 +638,11 @@ int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int
 template <typename T>
 int launch_laswp_list(T* A, int64_t ld, int64_t row0, int64_t ncols, int64_t skip_lo,
                       int64_t skip_hi, const int* list, const int* count, int max_count,
-                      hipStream_t s) {
+                      hipStream_t s, int ws_slot) {
   const int64_t width = ncols - (skip_hi - skip_lo);
   if (width <= 0) return 0;
   void* tmp = nullptr;
-  DDLA_RETURN_IF(workspace(WS_TEMP, (size_t)max_count * width * sizeof(T), &tmp));
+  DDLA_RETURN_IF(workspace(ws_slot, (size_t)max_count * width * sizeof(T), &tmp));
   // bytes: every moved row (at most max_count) is read and written once
   ProfScope prof("laswp_list", 0.0, 2.0 * sizeof(T) * max_count * width, s);
   const dim3 grid((unsigned)std::min<int64_t>(ceil_div(width, 256), 64), (unsigned)max_count);
@@ -705,9 +705,9 @@ template int launch_getf2_panel_ex<double>(int64_t, int, double*, int64_t, int32
 template int launch_getf2_panel_ex<float>(int64_t, int, float*, int64_t, int32_t*, int32_t*, int,
                                           int*, int*, hipStream_t);
 template int launch_laswp_list<double>(double*, int64_t, int64_t, int64_t, int64_t, int64_t,
-                                       const int*, const int*, int, hipStream_t);
+                                       const int*, const int*, int, hipStream_t, int);
 template int launch_laswp_list<float>(float*, int64_t, int64_t, int64_t, int64_t, int64_t,
-                                      const int*, const int*, int, hipStream_t);
+                                      const int*, const int*, int, hipStream_t, int);
 template int launch_gather_rows<double>(int64_t, int64_t, const double*, int64_t, const int32_t*,
                                         double*, int64_t, bool, hipStream_t);
 template int launch_gather_rows<float>(int64_t, int64_t, const float*, int64_t, const int32_t*,

@@ -541,7 +541,7 @@ struct Timer {
 // (k contiguous); B k x n with n contiguous, or (b_kcontig) B(k, j) at B[j ldb + k]; k <= 128.
 int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k,
                  float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                 int64_t ldc, float* scale_ws, hipStream_t s) {
+                 int64_t ldc, float* scale_ws, hipStream_t s, int ws_slot) {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
   if (k > NBLK) {
     set_error("hgemm: k = %lld exceeds %d", (long long)k, NBLK);
@@ -553,7 +553,7 @@ int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64
   float* sb = scale_ws + tm;
   void* ws = nullptr;
   const int K = (int)k, Kp = (K + 31) / 32 * 32;
-  DDLA_RETURN_IF(workspace(WS_SPLIT, (size_t)terms * (m + n) * Kp * 2, &ws));
+  DDLA_RETURN_IF(workspace(ws_slot, (size_t)terms * (m + n) * Kp * 2, &ws));
   const int M = (int)m, N = (int)n;
   const double mn = (tri ? 0.5 : 1.0) * (double)m * n;
   ProfScope prof(bf16 ? "hgemm_mfma_bf16" : "hgemm_mfma_f16",
@@ -619,7 +619,7 @@ int mixed_ws(int64_t n, int64_t np, int64_t nrhs, MixedWs* w) {
   w->np = np;
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   const int64_t nb = (np + 127) / 128 + 1;
-  const size_t s_scales = al(2 * nb * sizeof(float));
+  const size_t s_scales = al(4 * nb * sizeof(float));  // two streams' update scales
   const size_t s_perm = al(np * sizeof(int));
   const size_t s_piv = al(np * sizeof(int32_t));
   const size_t s_D = al((size_t)np * nrhs * sizeof(float));
@@ -649,14 +649,54 @@ int mixed_ws(int64_t n, int64_t np, int64_t nrhs, MixedWs* w) {
 
 // FP32 LU with partial pivoting; trailing updates in prec (FP16/BF16 matrix cores, or
 // FP32 matrix cores for DDLA_PREC_FP32).  Fills w.perm.  Returns info via *info_host.
+// One trailing update of the mixed LU: U12 = L11^-1 A12 for columns [c0, c0 + nc) of the row
+// block, then A22 -= L21 U12 on those columns (FP32, or FP16 / BF16 matrix cores).
+int lu_update_cols(int prec, int terms, int64_t n, int64_t j, int b, int64_t c0, int64_t nc,
+                   float* a, int64_t lda, float* scales, hipStream_t s, int split_slot) {
+  if (nc <= 0) return 0;
+  float* panel = a + j * lda + j;
+  const int64_t rest = n - j - b;
+  float* u12 = a + j * lda + c0;
+  DDLA_RETURN_IF(launch_trsm_slab<float>(true, true, b, nc, panel, lda, 1, u12, lda, s));
+  if (rest <= 0) return 0;
+  float* c = a + (j + b) * lda + c0;
+  if (prec == DDLA_PREC_FP32)
+    return launch_sgemm(rest, nc, b, -1.f, panel + b * lda, lda, true, u12, lda, true, 1.f, c, lda,
+                        0, s);
+  return launch_hgemm(prec == DDLA_PREC_BF16, terms, false, 0, rest, nc, b, -1.f, panel + b * lda,
+                      lda, u12, lda, c, lda, scales, s, split_slot);
+}
+
+// Right-looking blocked LU with partial pivoting on the FP32 working copy (same block
+// structure as PLUDecBlock, LU.cs:289-342), with a look-ahead of one panel: after panel k, the
+// main stream interchanges and updates only the next panel's columns and factors that panel,
+// while the auxiliary stream interchanges and updates the remaining columns.  The panel is
+// latency-bound (one exchange between workgroups per column), so the bulk of the trailing
+// update runs in its shadow.  Ordering, with E0(k) = "panel k done" and E2(k) = "update of
+// the remaining columns after panel k done":
+//   main: panel(k); wait E2(k-1); swaps + update of [0, j) and the next block; panel(k+1) ...
+//   aux:  wait E0(k); swaps + update of the columns after the next block; record E2(k)
 int sgetrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
   hipStream_t s = stream();
+  hipStream_t s2;
+  hipEvent_t* ev;
+  DDLA_RETURN_IF(aux_stream(&s2, &ev));
   int32_t* dinfo;
   int *counts, *lists;
   DDLA_RETURN_IF(lu_bookkeeping(n, &dinfo, &counts, &lists));
   DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
   float* a = w.W;
   const int64_t lda = n;
+  const int64_t nb = (n + NBLK - 1) / NBLK + 1;
+  float* scales_aux = w.scales + 2 * nb;
+  {  // size the main stream's interchange staging once (its width grows with j)
+    void* t = nullptr;
+    DDLA_RETURN_IF(workspace(WS_TEMP, (size_t)2 * NBLK * n * sizeof(float), &t));
+  }
+  // the auxiliary stream starts after everything queued so far on the main stream
+  DDLA_HIP_CHECK(hipEventRecord(ev[0], s));
+  DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
+  bool pending = false;  // an update on the auxiliary stream that the main stream must await
   int k = 0;
   for (int64_t j = 0; j < n; j += NBLK, ++k) {
     const int b = (int)std::min<int64_t>(NBLK, n - j);
@@ -664,21 +704,32 @@ int sgetrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
     int* list_k = lists + (size_t)k * 4 * NBLK;
     DDLA_RETURN_IF(launch_getf2_panel_ex<float>(n - j, b, panel, lda, w.piv + j, dinfo, (int)j,
                                                 list_k, counts + k, s));
-    DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, n, j, j + b, list_k, counts + k,
-                                            2 * NBLK, s));
-    const int64_t rest = n - j - b;
-    if (rest > 0) {
-      DDLA_RETURN_IF(launch_trsm_slab<float>(true, true, b, rest, panel, lda, 1, panel + b, lda, s));
-      if (prec == DDLA_PREC_FP32) {
-        DDLA_RETURN_IF(launch_sgemm(rest, rest, b, -1.f, panel + b * lda, lda, true, panel + b, lda,
-                                    true, 1.f, panel + b * lda + b, lda, 0, s));
-      } else {
-        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, w.terms, false, 0, rest, rest, b, -1.f,
-                                    panel + b * lda, lda, panel + b, lda, panel + b * lda + b, lda,
-                                    w.scales, s));
-      }
+    const int64_t nxt = std::min<int64_t>(NBLK, n - j - b);  // the next panel's width
+    const int64_t far0 = j + b + nxt, far = n - far0;        // columns after the next panel
+    if (far > 0) {
+      DDLA_HIP_CHECK(hipEventRecord(ev[0], s));  // E0(k)
+    }
+    if (pending) {
+      DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));  // E2(k-1): next block fully updated
+      pending = false;
+    }
+    // main stream: swaps on [0, j) and the next block, its update
+    DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, j + b + nxt, j, j + b, list_k,
+                                            counts + k, 2 * NBLK, s));
+    DDLA_RETURN_IF(lu_update_cols(prec, w.terms, n, j, b, j + b, nxt, a, lda, w.scales, s,
+                                  WS_SPLIT));
+    // auxiliary stream: swaps and update of the far columns
+    if (far > 0) {
+      DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
+      DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda + far0, lda, 0, far, 0, 0, list_k,
+                                              counts + k, 2 * NBLK, s2, WS_TEMP2));
+      DDLA_RETURN_IF(lu_update_cols(prec, w.terms, n, j, b, far0, far, a, lda, scales_aux, s2,
+                                    WS_SPLIT2));
+      DDLA_HIP_CHECK(hipEventRecord(ev[1], s2));  // E2(k)
+      pending = true;
     }
   }
+  if (pending) DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));
   DDLA_RETURN_IF(launch_perm_compose((int)n, k, NBLK, lists, counts, w.perm, s));
   DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DDLA_HIP_CHECK(hipStreamSynchronize(s));

@@ -50,6 +50,16 @@ int init_device(int device) {
   return 0;
 }
 
+int aux_stream(hipStream_t* s, hipEvent_t** ev) {
+  if (!g_ctx.aux_stream) {
+    DDLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.aux_stream, hipStreamNonBlocking));
+    for (hipEvent_t& e : g_ctx.ev) DDLA_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  *s = g_ctx.aux_stream;
+  *ev = g_ctx.ev;
+  return 0;
+}
+
 int ensure_init() {
   if (g_ctx.ready) {
     DDLA_HIP_CHECK(hipSetDevice(g_ctx.device));
@@ -68,8 +78,9 @@ int workspace(int slot, size_t bytes, void** out) {
   if (bytes == 0) bytes = 256;
   if (g_ctx.ws_bytes[slot] < bytes) {
     if (g_ctx.ws[slot]) {
-      // earlier queued work may still use the old buffer
+      // earlier queued work (on either library stream) may still use the old buffer
       DDLA_HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+      if (g_ctx.aux_stream) DDLA_HIP_CHECK(hipStreamSynchronize(g_ctx.aux_stream));
       DDLA_HIP_CHECK(hipFree(g_ctx.ws[slot]));
       g_ctx.ws[slot] = nullptr;
       g_ctx.ws_bytes[slot] = 0;
@@ -200,6 +211,9 @@ int finalize() {
     g_ctx.ws_bytes[i] = 0;
   }
   if (g_ctx.own_stream) hipStreamDestroy(g_ctx.own_stream);
+  if (g_ctx.aux_stream) hipStreamDestroy(g_ctx.aux_stream);
+  for (hipEvent_t e : g_ctx.ev)
+    if (e) hipEventDestroy(e);
   g_ctx = Context{};
   return 0;
 }
