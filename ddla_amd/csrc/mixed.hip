@@ -886,7 +886,8 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
     for (int64_t j = 0; j < nrhs; ++j) {
       const double r = __builtin_bit_cast(double, nh[2 * j]);
       const double x = __builtin_bit_cast(double, nh[2 * j + 1]);
-      const double e = (x > 0.0 && normA > 0.0) ? r / (normA * x) : (r == 0.0 ? 0.0 : INFINITY);
+      double e = (x > 0.0 && normA > 0.0) ? r / (normA * x) : (r == 0.0 ? 0.0 : INFINITY);
+      if (!std::isfinite(r) || !std::isfinite(x) || std::isnan(e)) e = INFINITY;  // NaN/Inf: failed
       worst = std::max(worst, e);
     }
     berr = worst;
@@ -961,9 +962,9 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
     }
     if (finfo == 0 && nrhs <= 8) {
       // inverted diagonal blocks of the two triangular factors for the refinement solves
-      if (kind != CHOLESKY) {
-        DDLA_RETURN_IF(trsv_prepare<float>(true, true, false, np, w.W, np, w.dinv1, s));
-        DDLA_RETURN_IF(trsv_prepare<float>(false, false, false, np, w.W, np, w.dinv2, s));
+      if (kind != CHOLESKY) {  // w.np: the butterfly's padded order, or n after the fallback
+        DDLA_RETURN_IF(trsv_prepare<float>(true, true, false, w.np, w.W, w.np, w.dinv1, s));
+        DDLA_RETURN_IF(trsv_prepare<float>(false, false, false, w.np, w.W, w.np, w.dinv2, s));
       } else {
         DDLA_RETURN_IF(trsv_prepare<float>(true, false, false, n, w.W, n, w.dinv1, s));
         DDLA_RETURN_IF(trsv_prepare<float>(false, false, true, n, w.W, n, w.dinv2, s));

@@ -118,6 +118,91 @@ constexpr int solve_bs() {
   return sizeof(T) == 4 ? 128 : 64;
 }
 
+// Layout of the prepared data behind Dinv (trsv_prepare): the inverted 64 x 64 diagonal
+// blocks, then per block row kb of the solve (BS rows) Dfull_kb = inverse of the whole
+// diagonal block, then M_kb = Dfull_kb * op(T)[kb, kprev], kprev = the block solved just
+// before kb.  With them x_kb = Dfull_kb (b_kb - sum_{older blocks} T x) - M_kb x_kprev: the
+// part that waits for the latest neighbour is a single product.
+__host__ __device__ inline int64_t blocks_of(int64_t n, int64_t b) { return (n + b - 1) / b; }
+__host__ __device__ inline int64_t inv64_elems(int64_t n) {
+  return blocks_of(n, BS_INV) * BS_INV * BS_INV;
+}
+template <int BS>
+__host__ __device__ inline int64_t blocks_elems(int64_t n) { return blocks_of(n, BS) * BS * BS; }
+
+// Builds Dfull and M (see above) for one block row per workgroup, in LDS.
+template <typename T, int BS, bool LOWER, bool TRANS>
+__global__ __launch_bounds__(256) void trsv_combine_kernel(int64_t n, const T* __restrict__ A,
+                                                           int64_t lda, T* __restrict__ Dinv) {
+  constexpr int NSUB = BS / BS_INV;
+  __shared__ T S1[BS][BS + 1];  // Dfull
+  __shared__ T S2[BS][BS + 1];  // operand blocks
+  const int nblk = (int)blocks_of(n, BS);
+  const int kb = blockIdx.x;
+  const int64_t r0 = (int64_t)kb * BS;
+  const int tid = threadIdx.x;
+  T* Dfull = Dinv + inv64_elems(n) + (int64_t)kb * BS * BS;
+  T* Mb = Dinv + inv64_elems(n) + blocks_elems<BS>(n) + (int64_t)kb * BS * BS;
+  const int nb64 = (int)blocks_of(n, BS_INV);
+  auto opT = [&](int64_t r, int64_t c) -> T {  // element (r, c) of op(T), zero outside n
+    if (r >= n || c >= n) return T(0);
+    return TRANS ? A[c * lda + r] : A[r * lda + c];
+  };
+  // inverted 64-blocks on the diagonal (identity beyond n)
+  for (int idx = tid; idx < BS * BS; idx += 256) {
+    const int r = idx / BS, c = idx % BS;
+    const int sr = r / BS_INV, sc = c / BS_INV;
+    T v = T(0);
+    if (sr == sc) {
+      const int64_t b64 = (int64_t)kb * NSUB + sr;
+      v = b64 < nb64 ? Dinv[b64 * BS_INV * BS_INV + (r % BS_INV) * BS_INV + (c % BS_INV)]
+                     : T(r == c ? 1 : 0);
+    }
+    S1[r][c] = v;
+  }
+  if (NSUB == 2) {
+    // LOWER: X = -D1 L10 D0 (rows 64.., cols ..64); UPPER: Y = -D0 U01 D1 (rows ..64, cols 64..)
+    constexpr int H = BS_INV;
+    for (int idx = tid; idx < H * H; idx += 256) {
+      const int r = idx / H, c = idx % H;
+      S2[r][c] = LOWER ? opT(r0 + H + r, r0 + c) : opT(r0 + r, r0 + H + c);
+    }
+    __syncthreads();
+    // P = LOWER ? L10 D0 : U01 D1  -> S2[.][H..2H)
+    for (int idx = tid; idx < H * H; idx += 256) {
+      const int r = idx / H, c = idx % H;
+      T acc = T(0);
+      for (int k = 0; k < H; ++k) acc = fma(S2[r][k], LOWER ? S1[k][c] : S1[H + k][H + c], acc);
+      S2[r][H + c] = acc;
+    }
+    __syncthreads();
+    // off-diagonal quarter = -(LOWER ? D1 P : D0 P)
+    for (int idx = tid; idx < H * H; idx += 256) {
+      const int r = idx / H, c = idx % H;
+      T acc = T(0);
+      for (int k = 0; k < H; ++k) acc = fma(LOWER ? S1[H + r][H + k] : S1[r][k], S2[k][H + c], acc);
+      if (LOWER) S1[H + r][c] = -acc;
+      else S1[r][H + c] = -acc;
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < BS * BS; idx += 256) Dfull[idx] = S1[idx / BS][idx % BS];
+  const int kprev = LOWER ? kb - 1 : kb + 1;
+  if (kprev < 0 || kprev >= nblk) return;  // uniform
+  const int64_t c0 = (int64_t)kprev * BS;
+  for (int idx = tid; idx < BS * BS; idx += 256) {
+    const int r = idx / BS, c = idx % BS;
+    S2[r][c] = opT(r0 + r, c0 + c);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < BS * BS; idx += 256) {
+    const int r = idx / BS, c = idx % BS;
+    T acc = T(0);
+    for (int k = 0; k < BS; ++k) acc = fma(S1[r][k], S2[k][c], acc);
+    Mb[idx] = acc;
+  }
+}
+
 template <typename T, int BS, int NR, bool LOWER, bool TRANS>
 __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T* __restrict__ A,
                                                    int64_t lda, const T* __restrict__ Dinv,
@@ -126,24 +211,27 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
   constexpr int Q = 256 / BS;       // threads per row of a block product (k split)
   constexpr int KQ = BS / Q;        // k range per thread
   constexpr int E = BS * BS / 256;  // staged block elements per thread
-  constexpr int NSUB = BS / BS_INV; // inverted sub-blocks on the diagonal
   constexpr int XPT = (BS * NR + 255) / 256;  // X / B words per thread
   constexpr int NW = Words<T>::N;
   __shared__ T Ts[BS][BS + 1];
+  __shared__ T Ms[BS][BS + 1];
   __shared__ T xs[BS][NR];
-  __shared__ T red[4][BS][NR];
+  __shared__ T ys[BS][NR];
+  __shared__ T red[Q][BS][NR];
   const int nblk = (int)((n + BS - 1) / BS);
   const int G = gridDim.x;
   const int tid = threadIdx.x;
   const int row = tid % BS, q = tid / BS;
+  const T* Dfull = Dinv + inv64_elems(n);
+  const T* Mall = Dfull + blocks_elems<BS>(n);
 
   for (int jb = blockIdx.x; jb < nblk; jb += G) {
     const int kb = LOWER ? jb : nblk - 1 - jb;  // dependency order
     const int64_t r0 = (int64_t)kb * BS;
     const int rows = (int)min<int64_t>(BS, n - r0);
-    // Blocks in dependency order: jc < jb are the off-diagonal blocks, jc == jb the diagonal
-    // one.  Block jc + 1 (and the X it multiplies) is loaded while block jc is used, so the
-    // stream of a long block row runs at the memory system's pace, not one latency per block.
+    // Blocks in dependency order jc < jb - 1 stream through Ts (block jc + 1 and the X it
+    // multiplies are loaded while block jc is used, so a long block row runs at the memory
+    // system's pace), then Dfull; the block kprev = jb - 1 enters through M.
     T stage[E];
     auto load_block = [&](int jc) {
       const int kc = LOWER ? jc : nblk - 1 - jc;
@@ -159,12 +247,16 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
         stage[e] = ok ? (TRANS ? A[(c0 + i) * lda + r0 + j] : A[(r0 + i) * lda + c0 + j]) : T(0);
       }
     };
-    auto store_block = [&]() {
+    auto load_dfull = [&]() {
+#pragma unroll
+      for (int e = 0; e < E; ++e) stage[e] = Dfull[(int64_t)kb * BS * BS + tid + 256 * e];
+    };
+    auto store_block = [&](bool tr) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int idx = tid + 256 * e;
         const int i = idx / BS, j = idx % BS;
-        if (TRANS) Ts[j][i] = stage[e];
+        if (tr) Ts[j][i] = stage[e];
         else Ts[i][j] = stage[e];
       }
     };
@@ -213,17 +305,21 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
         xs[k][c] = v;
       }
     };
+    // sum over the Q k-ranges: red[q][r][c] -> dst[r][c] (optionally minus)
+    auto product = [&](const T (&acc)[NR]) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) red[q][row][c] = acc[c];
+    };
 
-    // the diagonal step's operands do not depend on X: fetch them first (the inverted
-    // sub-blocks of this thread's 64 x 16 slice, and B)
-    const int sr = tid & 63, sq = tid >> 6;  // 64 rows x 4 k-quarters for the 64 x 64 products
-    T dreg[NSUB][16];
+    // operands that do not depend on X: M (to LDS, through the staging registers), and B
+    if (jb > 0) {
 #pragma unroll
-    for (int sb = 0; sb < NSUB; ++sb) {
-      const bool okb = sb * BS_INV < rows;
-      const T* D = Dinv + ((int64_t)kb * NSUB + sb) * BS_INV * BS_INV + sr * BS_INV + sq * 16;
+      for (int e = 0; e < E; ++e) stage[e] = Mall[(int64_t)kb * BS * BS + tid + 256 * e];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) dreg[sb][k] = okb ? D[k] : T(0);
+      for (int e = 0; e < E; ++e) {
+        const int idx = tid + 256 * e;
+        Ms[idx / BS][idx % BS] = stage[e];
+      }
     }
     T breg[XPT];
 #pragma unroll
@@ -235,28 +331,29 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
     T part[NR];
 #pragma unroll
     for (int c = 0; c < NR; ++c) part[c] = T(0);
+    const int nstream = jb > 0 ? jb - 1 : 0;
     if (jb > 0) load_x(0);
-    load_block(0);
-    for (int jc = 0; jc < jb; ++jc) {
+    if (nstream > 0) load_block(0);
+    else load_dfull();
+    for (int jc = 0; jc < nstream; ++jc) {
       __syncthreads();  // the previous block's product is done with Ts / xs
-      store_block();
+      store_block(TRANS);
       use_x(jc);
       if (jc + 1 < jb) load_x(jc + 1);
-      load_block(jc + 1);  // the next off-diagonal block, or the diagonal block
+      if (jc + 1 < nstream) load_block(jc + 1);
+      else load_dfull();
       __syncthreads();
-#pragma unroll 16
+#pragma unroll 4
       for (int k = q * KQ; k < q * KQ + KQ; ++k) {
         const T a = Ts[row][k];
 #pragma unroll
         for (int c = 0; c < NR; ++c) part[c] = fma(a, xs[k][c], part[c]);
       }
     }
-    // diagonal block: rhs = B - sum, then sub-block substitution with the inverted 64 x 64
-    // diagonal sub-blocks and the block's own off-diagonal sub-blocks
+    // y = Dfull (b - sum)
     __syncthreads();
-    store_block();
-#pragma unroll
-    for (int c = 0; c < NR; ++c) red[q][row][c] = part[c];
+    store_block(false);
+    product(part);
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < XPT; ++t) {
@@ -270,58 +367,56 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
       }
     }
     __syncthreads();
+    T acc[NR];
 #pragma unroll
-    for (int ss = 0; ss < NSUB; ++ss) {
-      const int sb = LOWER ? ss : NSUB - 1 - ss;
-      if (sb * BS_INV >= rows) continue;  // sub-block beyond n (uniform)
-      // x_sb = Dinv_sb rhs_sb
-      T acc[NR];
+    for (int c = 0; c < NR; ++c) acc[c] = T(0);
+#pragma unroll 4
+    for (int k = q * KQ; k < q * KQ + KQ; ++k) {
+      const T d = Ts[row][k];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = fma(d, xs[k][c], acc[c]);
+    }
+    product(acc);
+    __syncthreads();
+    for (int idx = tid; idx < BS * NR; idx += 256) {
+      const int r = idx / NR, c = idx % NR;
+      T sum = T(0);
+#pragma unroll
+      for (int qq = 0; qq < Q; ++qq) sum += red[qq][r][c];
+      ys[r][c] = sum;
+    }
+    // x = y - M x_kprev: the only step that waits for the block solved just before
+    if (jb > 0) {
+      use_x(jb - 1);
+      __syncthreads();
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = T(0);
+#pragma unroll 4
+      for (int k = q * KQ; k < q * KQ + KQ; ++k) {
+        const T m = Ms[row][k];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const T d = dreg[sb][k];
-#pragma unroll
-        for (int c = 0; c < NR; ++c) acc[c] = fma(d, xs[sb * BS_INV + sq * 16 + k][c], acc[c]);
+        for (int c = 0; c < NR; ++c) acc[c] = fma(m, xs[k][c], acc[c]);
       }
-#pragma unroll
-      for (int c = 0; c < NR; ++c) red[sq][sr][c] = acc[c];
+      product(acc);
       __syncthreads();
-      for (int idx = tid; idx < BS_INV * NR; idx += 256) {
+      for (int idx = tid; idx < BS * NR; idx += 256) {
         const int r = idx / NR, c = idx % NR;
-        xs[sb * BS_INV + r][c] = red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
-      }
-      __syncthreads();
-      // the sub-blocks still to solve: rhs_t -= T[t][sb] x_sb
-      for (int tt = ss + 1; tt < NSUB; ++tt) {
-        const int tb = LOWER ? tt : NSUB - 1 - tt;
+        T sum = T(0);
 #pragma unroll
-        for (int c = 0; c < NR; ++c) acc[c] = T(0);
-#pragma unroll
-        for (int k = sq * 16; k < sq * 16 + 16; ++k) {
-          const T a = Ts[tb * BS_INV + sr][sb * BS_INV + k];
-#pragma unroll
-          for (int c = 0; c < NR; ++c) acc[c] = fma(a, xs[sb * BS_INV + k][c], acc[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < NR; ++c) red[sq][sr][c] = acc[c];
-        __syncthreads();
-        for (int idx = tid; idx < BS_INV * NR; idx += 256) {
-          const int r = idx / NR, c = idx % NR;
-          xs[tb * BS_INV + r][c] -= red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
-        }
-        __syncthreads();
+        for (int qq = 0; qq < Q; ++qq) sum += red[qq][r][c];
+        ys[r][c] -= sum;
       }
     }
+    __syncthreads();
     for (int idx = tid; idx < BS * NR; idx += 256) {
       const int r = idx / NR, c = idx % NR;
       if (r < rows && c < nrhs) {
-        const T x = xs[r][c];
+        const T x = ys[r][c];
         publish<T>(XW + ((r0 + r) * nrhs + c) * NW, x, tag);
         B[(r0 + r) * ldb + c] = x;
       }
     }
-    __syncthreads();  // xs / Ts are reused by the next block row of this workgroup
+    __syncthreads();  // the LDS is reused by the next block row of this workgroup
   }
 }
 
@@ -365,10 +460,24 @@ int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64
   }
 #undef TD
   DDLA_HIP_CHECK(hipGetLastError());
+  constexpr int BS = solve_bs<T>();
+  const int nbb = ceil_div(n, BS);
+#define TC(L, TR) \
+  hipLaunchKernelGGL((trsv_combine_kernel<T, BS, L, TR>), dim3(nbb), dim3(256), 0, s, n, A, lda, Dinv)
+  if (lower) {
+    if (trans) TC(true, true); else TC(true, false);
+  } else {
+    if (trans) TC(false, true); else TC(false, false);
+  }
+#undef TC
+  DDLA_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-size_t trsv_dinv_elems(int64_t n) { return (size_t)ceil_div(n, BS_INV) * BS_INV * BS_INV; }
+// inverted 64-blocks + Dfull + M (sized for the larger FP32 solve block)
+size_t trsv_dinv_elems(int64_t n) {
+  return (size_t)(inv64_elems(n) + 2 * blocks_elems<128>(n));
+}
 
 // op(T) X = B in place, nrhs <= 8, with Dinv from trsv_prepare (same lower/trans).
 template <typename T>
