@@ -213,6 +213,10 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
   constexpr int E = BS * BS / 256;  // staged block elements per thread
   constexpr int XPT = (BS * NR + 255) / 256;  // X / B words per thread
   constexpr int NW = Words<T>::N;
+  constexpr int VW = 16 / sizeof(T);  // elements per 16-byte load
+  constexpr int EV = E / VW;
+  typedef T vecT __attribute__((ext_vector_type(VW)));
+  const bool avec = lda % VW == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
   __shared__ T Ts[BS][BS + 1];
   __shared__ T Ms[BS][BS + 1];
   __shared__ T xs[BS][NR];
@@ -233,31 +237,51 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
     // multiplies are loaded while block jc is used, so a long block row runs at the memory
     // system's pace), then Dfull; the block kprev = jb - 1 enters through M.
     T stage[E];
+    // Staging: element e of vector v of this thread is (i, j) = (idx / (BS/VW), (idx % (BS/VW))
+    // VW + e) with idx = tid + 256 v, j running along memory; 16-byte loads wherever the
+    // row is aligned (few wide loads keep the address units from limiting the stream).
     auto load_block = [&](int jc) {
       const int kc = LOWER ? jc : nblk - 1 - jc;
       const int64_t c0 = (int64_t)kc * BS;
       const int cols = (int)min<int64_t>(BS, n - c0);
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int idx = tid + 256 * e;
-        const int i = idx / BS, j = idx % BS;  // j runs along memory
+      for (int v = 0; v < EV; ++v) {
+        const int idx = tid + 256 * v;
+        const int i = idx / (BS / VW), j0 = (idx % (BS / VW)) * VW;
         // !TRANS: op(T)(r0 + i, c0 + j) = A[(r0 + i) lda + c0 + j]
         //  TRANS: op(T)(r0 + j, c0 + i) = A[(c0 + i) lda + r0 + j]
-        const bool ok = TRANS ? (j < rows && i < cols) : (i < rows && j < cols);
-        stage[e] = ok ? (TRANS ? A[(c0 + i) * lda + r0 + j] : A[(r0 + i) * lda + c0 + j]) : T(0);
+        const bool rowok = TRANS ? (i < cols) : (i < rows);
+        const int lim = TRANS ? rows : cols;
+        const T* p = TRANS ? A + (c0 + i) * lda + r0 + j0 : A + (r0 + i) * lda + c0 + j0;
+        if (avec && rowok && j0 + VW <= lim) {
+          const vecT x = *reinterpret_cast<const vecT*>(p);
+#pragma unroll
+          for (int e = 0; e < VW; ++e) stage[v * VW + e] = x[e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < VW; ++e) stage[v * VW + e] = (rowok && j0 + e < lim) ? p[e] : T(0);
+        }
       }
     };
-    auto load_dfull = [&]() {
+    auto load_dense = [&](const T* src) {  // a BS x BS row-major block (16-byte aligned)
 #pragma unroll
-      for (int e = 0; e < E; ++e) stage[e] = Dfull[(int64_t)kb * BS * BS + tid + 256 * e];
+      for (int v = 0; v < EV; ++v) {
+        const vecT x = *reinterpret_cast<const vecT*>(src + (int64_t)(tid + 256 * v) * VW);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) stage[v * VW + e] = x[e];
+      }
     };
-    auto store_block = [&](bool tr) {
+    auto load_dfull = [&]() { load_dense(Dfull + (int64_t)kb * BS * BS); };
+    auto store_block = [&](bool tr, T (*dst)[BS + 1]) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int idx = tid + 256 * e;
-        const int i = idx / BS, j = idx % BS;
-        if (tr) Ts[j][i] = stage[e];
-        else Ts[i][j] = stage[e];
+      for (int v = 0; v < EV; ++v) {
+        const int idx = tid + 256 * v;
+        const int i = idx / (BS / VW), j0 = (idx % (BS / VW)) * VW;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+          if (tr) dst[j0 + e][i] = stage[v * VW + e];
+          else dst[i][j0 + e] = stage[v * VW + e];
+        }
       }
     };
     u64 xw[XPT][NW];
@@ -313,13 +337,8 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
 
     // operands that do not depend on X: M (to LDS, through the staging registers), and B
     if (jb > 0) {
-#pragma unroll
-      for (int e = 0; e < E; ++e) stage[e] = Mall[(int64_t)kb * BS * BS + tid + 256 * e];
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int idx = tid + 256 * e;
-        Ms[idx / BS][idx % BS] = stage[e];
-      }
+      load_dense(Mall + (int64_t)kb * BS * BS);
+      store_block(false, Ms);
     }
     T breg[XPT];
 #pragma unroll
@@ -337,7 +356,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
     else load_dfull();
     for (int jc = 0; jc < nstream; ++jc) {
       __syncthreads();  // the previous block's product is done with Ts / xs
-      store_block(TRANS);
+      store_block(TRANS, Ts);
       use_x(jc);
       if (jc + 1 < jb) load_x(jc + 1);
       if (jc + 1 < nstream) load_block(jc + 1);
@@ -352,7 +371,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
     }
     // y = Dfull (b - sum)
     __syncthreads();
-    store_block(false);
+    store_block(false, Ts);
     product(part);
     __syncthreads();
 #pragma unroll
