@@ -258,12 +258,13 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--prec", choices=["fp16", "bf16", "fp32"], default="fp16")
-    ap.add_argument("--pivoting", choices=["butterfly", "partial"], default="partial")
+    ap.add_argument("--pivoting", choices=["partial", "tournament", "butterfly"],
+                    default="partial")
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--compare-butterfly", action="store_true",
-                    help="also time the butterfly (pivot-free) variant of the mixed solver")
+    ap.add_argument("--compare", default="",
+                    help="comma list of other pivoting rules of the mixed solver timed once each")
     ap.add_argument("--sweep-kappa", default="1e2,1e4",
                     help="extra condition numbers timed (one solve each) besides --kappa")
     ap.add_argument("--seed", type=int, default=0xDD1A0003)
@@ -292,26 +293,28 @@ def main():
     X = ddla.DeviceMatrix.wrap(X_t.data_ptr(), n, nrhs, nrhs, X_t)
     torch.cuda.synchronize()
 
-    pivoting = {"butterfly": ddla.PIVOT_BUTTERFLY, "partial": ddla.PIVOT_PARTIAL}[args.pivoting]
+    piv_codes = {"partial": ddla.PIVOT_PARTIAL, "tournament": ddla.PIVOT_TOURNAMENT,
+                 "butterfly": ddla.PIVOT_BUTTERFLY}
+    pivoting = piv_codes[args.pivoting]
 
     def solve(piv):
         return ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14, pivoting=piv)
 
-    # ---- optionally, the other pivoting choice of the mixed solver
-    other_out = None
-    if args.compare_butterfly:
-        other = ddla.PIVOT_PARTIAL if pivoting == ddla.PIVOT_BUTTERFLY else ddla.PIVOT_BUTTERFLY
-        solve(other)
+    # ---- optionally, other pivoting rules of the mixed solver (one timed solve each)
+    other_out = []
+    for name in [x for x in args.compare.split(",") if x and x != args.pivoting]:
+        solve(piv_codes[name])
         ddla.synchronize()
         t0 = time.perf_counter()
-        oinfo = solve(other)
+        oinfo = solve(piv_codes[name])
         ddla.synchronize()
         t_other = time.perf_counter() - t0
-        other_out = {"pivoting": "partial" if other == ddla.PIVOT_PARTIAL else "butterfly",
-                     "ms_per_solve": t_other * 1e3, "status": oinfo.status, "iters": oinfo.iters,
-                     "fallback": oinfo.fallback, "backward_error": oinfo.backward_error}
-        log(f"mixed ({other_out['pivoting']}): {t_other * 1e3:.1f} ms status={oinfo.status} "
-            f"iters={oinfo.iters} fallback={oinfo.fallback}")
+        other_out.append({"pivoting": name, "ms_per_solve": t_other * 1e3, "status": oinfo.status,
+                          "iters": oinfo.iters, "fallback": oinfo.fallback,
+                          "backward_error": oinfo.backward_error,
+                          "factor_ms": oinfo.factor_ms, "refine_ms": oinfo.refine_ms})
+        log(f"mixed ({name}): {t_other * 1e3:.1f} ms status={oinfo.status} "
+            f"iters={oinfo.iters} fallback={oinfo.fallback} factor={oinfo.factor_ms:.1f}ms")
 
     # ---- mixed precision LU + IR (the measured step)
     for w in range(args.warmup):
@@ -437,7 +440,7 @@ def main():
                   "backward_error": be_mixed, "forward_error_vs_xtrue": fe_mixed,
                   "forward_error_vs_fp64_lu": fe_vs_fp64,
                   "factor_ms": last.factor_ms, "refine_ms": last.refine_ms},
-        "mixed_other_pivoting": other_out,
+        "mixed_other_pivoting": other_out or None,
         "kappa_sweep": sweep,
         "fp64_lu": {"ms_per_solve": t_fp64 * 1e3, "solves_per_s": 1.0 / t_fp64,
                     "backward_error": be_fp64,

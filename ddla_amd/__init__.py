@@ -24,7 +24,8 @@ __all__ = [
     "CholDecompose", "CholeskySolve", "LinalgException", "gesv_mpir", "posv_mpir", "init",
     "synchronize", "NO_TRANSPOSE", "TRANSPOSE", "LOWER", "UPPER", "UNIT_DIAG", "NONUNIT_DIAG",
     "LEFT", "RIGHT", "PREC_FP64", "PREC_FP32", "PREC_FP16", "PREC_BF16", "DDLAError",
-    "BatchPlan", "batch_factor", "BATCH_LU", "BATCH_CHOL",
+    "BatchPlan", "batch_factor", "BATCH_LU", "BATCH_CHOL", "sgetrf_mp", "PIVOT_PARTIAL",
+    "PIVOT_BUTTERFLY", "PIVOT_TOURNAMENT",
 ]
 
 
@@ -420,7 +421,7 @@ class Cholesky:
         x = b if inplace else np.array(_as_f64(b), copy=True)
         CholeskySolve(self.uplo, self.matrix, x)
         return x
-PIVOT_PARTIAL, PIVOT_BUTTERFLY = 0, 1
+PIVOT_PARTIAL, PIVOT_BUTTERFLY, PIVOT_TOURNAMENT = 0, 1, 2
 
 
 def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_iters, gmres,
@@ -439,8 +440,9 @@ def gesv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16,
     """Mixed-precision LU + iterative refinement to FP64 accuracy (no reference counterpart).
 
     terms: low-precision parts per operand of the trailing updates (0 = library default).
-    pivoting: PIVOT_PARTIAL, or PIVOT_BUTTERFLY (random butterfly transform + LU without
-    pivoting, falling back to partial pivoting if it does not refine)."""
+    pivoting: PIVOT_PARTIAL; PIVOT_BUTTERFLY (random butterfly transform + LU without
+    pivoting); or PIVOT_TOURNAMENT (pivot rows of each 128-column panel chosen by a reduction
+    tree, as in CALU).  The last two fall back to partial pivoting if they do not refine."""
     return _mpir(_lib().ddla_dgesv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms,
                  pivoting)
 
@@ -449,6 +451,23 @@ def posv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16,
               max_iters=50, gmres=0, terms=0) -> MpirInfo:
     """Mixed-precision Cholesky + iterative refinement (no reference counterpart)."""
     return _mpir(_lib().ddla_dposv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms)
+
+
+def sgetrf_mp(A, prec=PREC_FP32, pivoting=PIVOT_TOURNAMENT):
+    """The mixed solver's FP32 LU factor on its own (no reference counterpart).
+
+    A: square float32 host array (not modified).  Returns (LU, perm, info): unit L below the
+    diagonal and U on and above it, row i of LU belonging to row perm[i] of A; info > 0 is the
+    first zero pivot column + 1."""
+    a = np.ascontiguousarray(A, dtype=np.float32)
+    n = a.shape[0]
+    assert a.shape == (n, n)
+    buf, perm = _DevBuf(a.nbytes), _DevBuf(4 * n)
+    buf.upload(a)
+    st = _lib().ddla_sgetrf_mp_dev(n, buf.ptr, n, prec, pivoting, perm.ptr)
+    if st < 0:
+        check(st)
+    return (buf.download(np.float32, n * n).reshape(n, n), perm.download(np.int32, n), st)
 
 
 # ------------------------------------------------------------ batched factorizations ---

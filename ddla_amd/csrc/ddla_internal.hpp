@@ -51,8 +51,8 @@ struct Context {
   hipStream_t aux_stream = nullptr;
   hipEvent_t ev[4] = {};
   // growable scratch buffers (device memory kept for the life of the context)
-  void* ws[12] = {};
-  size_t ws_bytes[12] = {};
+  void* ws[16] = {};
+  size_t ws_bytes[16] = {};
 };
 
 Context& ctx();
@@ -75,7 +75,8 @@ enum WorkspaceSlot {
   WS_SPLIT = 9,   // low-precision parts of the trailing-update operands
   WS_SPLIT2 = 10, // ... for updates on the auxiliary stream
   WS_TEMP2 = 11,  // row-interchange staging on the auxiliary stream
-  WS_COUNT = 12,
+  WS_TSLU = 12,   // tournament-pivoting tree (candidate sets, arrival words)
+  WS_COUNT = 13,
 };
 
 // Optional per-kernel timing (ddla_profile_*): while enabled, a ProfScope around a launch
@@ -161,6 +162,11 @@ int launch_laswp_list(T* A, int64_t ld, int64_t row0, int64_t ncols, int64_t ski
                       hipStream_t s, int ws_slot = WS_TEMP);
 int launch_perm_compose(int n, int npanels, int nb, const int* lists, const int* counts, int* perm,
                         hipStream_t s);
+// Tournament-pivoting panel (tslu.hip, FP32): same outputs as the partial-pivoting panel
+// (factored panel, *info, (dst, src) list) except relative pivots; nb must be 128.
+bool tslu_supported(int64_t m, int nb);
+int launch_tslu_panel(int64_t m, int nb, float* P, int64_t ld, int32_t* info, int info_base,
+                      int* list_out, int* count_out, hipStream_t s);
 
 // triangular block solves (trsm.hip): op(T) X = B for one bs x bs triangle (bs <= 128)
 template <typename T>

@@ -668,7 +668,9 @@ int lu_update_cols(int prec, int terms, int64_t n, int64_t j, int b, int64_t c0,
 }
 
 // Right-looking blocked LU with partial pivoting on the FP32 working copy (same block
-// structure as PLUDecBlock, LU.cs:289-342), with a look-ahead of one panel: after panel k, the
+// structure as PLUDecBlock, LU.cs:289-342), or with tournament pivoting in the 128-wide
+// panels (tslu.hip; a narrower last panel pivots column by column), with a look-ahead of one
+// panel: after panel k, the
 // main stream interchanges and updates only the next panel's columns and factors that panel,
 // while the auxiliary stream interchanges and updates the remaining columns.  The panel is
 // latency-bound (one exchange between workgroups per column), so the bulk of the trailing
@@ -676,7 +678,8 @@ int lu_update_cols(int prec, int terms, int64_t n, int64_t j, int b, int64_t c0,
 // the remaining columns after panel k done":
 //   main: panel(k); wait E2(k-1); swaps + update of [0, j) and the next block; panel(k+1) ...
 //   aux:  wait E0(k); swaps + update of the columns after the next block; record E2(k)
-int sgetrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
+int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, MixedWs& w,
+                 int32_t* info_host) {
   hipStream_t s = stream();
   hipStream_t s2;
   hipEvent_t* ev;
@@ -685,8 +688,6 @@ int sgetrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
   int *counts, *lists;
   DDLA_RETURN_IF(lu_bookkeeping(n, &dinfo, &counts, &lists));
   DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
-  float* a = w.W;
-  const int64_t lda = n;
   const int64_t nb = (n + NBLK - 1) / NBLK + 1;
   float* scales_aux = w.scales + 2 * nb;
   {  // size the main stream's interchange staging once (its width grows with j)
@@ -702,8 +703,11 @@ int sgetrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
     const int b = (int)std::min<int64_t>(NBLK, n - j);
     float* panel = a + j * lda + j;
     int* list_k = lists + (size_t)k * 4 * NBLK;
-    DDLA_RETURN_IF(launch_getf2_panel_ex<float>(n - j, b, panel, lda, w.piv + j, dinfo, (int)j,
-                                                list_k, counts + k, s));
+    if (tournament && tslu_supported(n - j, b))
+      DDLA_RETURN_IF(launch_tslu_panel(n - j, b, panel, lda, dinfo, (int)j, list_k, counts + k, s));
+    else
+      DDLA_RETURN_IF(launch_getf2_panel_ex<float>(n - j, b, panel, lda, w.piv + j, dinfo, (int)j,
+                                                  list_k, counts + k, s));
     const int64_t nxt = std::min<int64_t>(NBLK, n - j - b);  // the next panel's width
     const int64_t far0 = j + b + nxt, far = n - far0;        // columns after the next panel
     if (far > 0) {
@@ -812,7 +816,7 @@ int norm_inf(int64_t n, const double* A, int64_t lda, MixedWs& w, double* host) 
 }
 
 // Classic refinement. lu: solve with P, L (unit) and U; else L L^T.
-enum FactorKind { LU_PIVOT = 0, LU_BUTTERFLY = 1, CHOLESKY = 2 };
+enum FactorKind { LU_PIVOT = 0, LU_BUTTERFLY = 1, CHOLESKY = 2, LU_TOURNAMENT = 3 };
 
 // Classic refinement: X += M^-1 R, R = B - A X, with M the FP32 factor of kind `kind`.
 // Stops at backward error <= tol, after max_iters steps, or when three steps in a row fail
@@ -840,7 +844,8 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
                          RU, w.D);
     else
       hipLaunchKernelGGL(gather_d2s_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.R,
-                         nrhs, kind == LU_PIVOT ? w.perm : nullptr, w.D);
+                         nrhs, kind == LU_PIVOT || kind == LU_TOURNAMENT ? w.perm : nullptr,
+                         w.D);
     DDLA_HIP_CHECK(hipGetLastError());
     const bool lu = kind != CHOLESKY;
     if (nrhs <= 8) {
@@ -919,7 +924,8 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
     return DDLA_ERR_UNSUPPORTED;
   }
   FactorKind kind = lu ? LU_PIVOT : CHOLESKY;
-  if (lu && opts && opts->pivoting == 1) kind = LU_BUTTERFLY;
+  if (lu && opts && opts->pivoting == DDLA_PIVOT_BUTTERFLY) kind = LU_BUTTERFLY;
+  if (lu && opts && opts->pivoting == DDLA_PIVOT_TOURNAMENT) kind = LU_TOURNAMENT;
   const int64_t np = (kind == LU_BUTTERFLY) ? (n + 3) / 4 * 4 : n;
   MixedWs w;
   DDLA_RETURN_IF(mixed_ws(n, np, nrhs, &w));
@@ -947,8 +953,8 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
     } else {
       hipLaunchKernelGGL(d2s_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, n, n, a, lda, w.W, n);
       DDLA_HIP_CHECK(hipGetLastError());
-      if (kind == LU_PIVOT) {
-        DDLA_RETURN_IF(sgetrf_mixed(n, prec, w, &finfo));
+      if (kind == LU_PIVOT || kind == LU_TOURNAMENT) {
+        DDLA_RETURN_IF(sgetrf_mixed(n, prec, kind == LU_TOURNAMENT, w.W, n, w, &finfo));
       } else {
         DDLA_RETURN_IF(spotrf_mixed(n, prec, w, &finfo));
         if (finfo > 0 && prec != DDLA_PREC_FP32) {
@@ -977,10 +983,11 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
                             &berr));
       refine_ms += t.stop_ms(s);
     }
-    // a pivot-free factor that broke down or does not refine: redo with partial pivoting
-    if (kind == LU_BUTTERFLY && (finfo != 0 || !(berr <= tol))) {
+    // a butterfly or tournament factor that broke down or does not refine: redo with partial
+    // pivoting
+    if ((kind == LU_BUTTERFLY || kind == LU_TOURNAMENT) && (finfo != 0 || !(berr <= tol))) {
+      out.fallback = kind == LU_BUTTERFLY ? 2 : 3;
       kind = LU_PIVOT;
-      out.fallback = 2;
       w.np = n;
       continue;
     }
@@ -1001,6 +1008,20 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
   return out.status;
 }
 
+// FP32 LU of a device matrix in place with the mixed solver's trailing updates; perm[i] =
+// source row of row i.
+int sgetrf_mp(int64_t n, float* a, int64_t lda, int prec, int pivoting, int32_t* perm) {
+  MixedWs w;
+  DDLA_RETURN_IF(mixed_ws(n, n, 1, &w));
+  w.terms = prec == DDLA_PREC_BF16 ? 3 : 2;
+  int32_t finfo = 0;
+  DDLA_RETURN_IF(sgetrf_mixed(n, prec, pivoting == DDLA_PIVOT_TOURNAMENT, a, lda, w, &finfo));
+  DDLA_HIP_CHECK(hipMemcpyAsync(perm, w.perm, n * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                stream()));
+  DDLA_HIP_CHECK(hipStreamSynchronize(stream()));
+  return finfo;
+}
+
 }  // namespace
 
 }  // namespace ddla
@@ -1018,6 +1039,19 @@ int ddla_dgesv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc
   if (n == 0 || nrhs == 0) return 0;
   DDLA_ARG_CHECK(n <= 32768, "n = %lld above the supported 32768", (long long)n);
   return mpir(true, n, nrhs, a, lda, b, ldb, x, ldx, opts, info);
+}
+
+int ddla_sgetrf_mp_dev(ddla_dim_t n, float* a, ddla_inc_t lda, int32_t prec, int32_t pivoting,
+                       int32_t* perm) {
+  DDLA_RETURN_IF(ensure_init());
+  DDLA_ARG_CHECK(n >= 0 && lda >= n, "bad sgetrf arguments");
+  DDLA_ARG_CHECK(prec == DDLA_PREC_FP32 || prec == DDLA_PREC_FP16 || prec == DDLA_PREC_BF16,
+                 "trailing-update precision %d not supported", (int)prec);
+  DDLA_ARG_CHECK(pivoting == DDLA_PIVOT_PARTIAL || pivoting == DDLA_PIVOT_TOURNAMENT,
+                 "pivoting %d not supported by the FP32 factor", (int)pivoting);
+  if (n == 0) return 0;
+  DDLA_ARG_CHECK(n <= 32768, "n = %lld above the supported 32768", (long long)n);
+  return sgetrf_mp(n, a, lda, prec, pivoting, perm);
 }
 
 int ddla_dposv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,

@@ -191,6 +191,10 @@ int ddla_damaxv_dev(ddla_dim_t n, const double* x, ddla_inc_t incx, ddla_dim_t* 
 /* No counterpart in the reference: factor in a lower precision on the matrix cores, then
  * recover FP64 accuracy by iterative refinement on the GPU (residual, correction and
  * convergence test in FP64). */
+#define DDLA_PIVOT_PARTIAL 0
+#define DDLA_PIVOT_BUTTERFLY 1
+#define DDLA_PIVOT_TOURNAMENT 2
+
 typedef struct ddla_mpir_opts {
   int32_t factor_prec;   /* DDLA_PREC_FP16 / _BF16 / _FP32 / _FP64 */
   int32_t max_iters;     /* refinement steps before giving up (default 50 if <= 0) */
@@ -199,10 +203,14 @@ typedef struct ddla_mpir_opts {
   int32_t terms;         /* low-precision parts per FP32 operand in the trailing updates:
                             1 = one FP16/BF16 rounding, 2 or 3 = split x = h + m (+ l) with
                             the cross products summed; 0 = default (FP16: 2, BF16: 3) */
-  int32_t pivoting;      /* LU only: 0 = partial pivoting (the FP64 path's pivot rule);
-                            1 = two-sided random butterfly transform, then LU without pivoting
-                            (no global pivot search); falls back to partial pivoting when the
-                            pivot-free factor breaks down or does not refine (fallback = 2) */
+  int32_t pivoting;      /* LU only (DDLA_PIVOT_*): PARTIAL = partial pivoting (the FP64
+                            path's pivot rule); BUTTERFLY = two-sided random butterfly
+                            transform, then LU without pivoting (no global pivot search);
+                            TOURNAMENT = tournament pivoting in each 128-column panel (pivot
+                            rows chosen by a reduction tree of small partial-pivoting
+                            eliminations, as in CALU).  BUTTERFLY and TOURNAMENT fall back to
+                            partial pivoting when their factor breaks down or does not refine
+                            (fallback = 2 / 3) */
   int32_t reserved;
 } ddla_mpir_opts;
 
@@ -212,7 +220,7 @@ typedef struct ddla_mpir_info {
   int32_t inner_iters;
   int32_t fallback;      /* 1: the low-precision Cholesky factor was not positive definite
                             and an FP32 factor was used; 2: the butterfly LU fell back to
-                            partial pivoting */
+                            partial pivoting; 3: the tournament-pivoting LU did */
   double backward_error; /* max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf) */
   double factor_ms;      /* GPU time of the factorization */
   double refine_ms;      /* GPU time of the refinement */
@@ -228,6 +236,14 @@ int ddla_dgesv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc
 int ddla_dposv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
                         const double* b, ddla_inc_t ldb, double* x, ddla_inc_t ldx,
                         const ddla_mpir_opts* opts, ddla_mpir_info* info);
+/* The low-precision factor of ddla_dgesv_mpir_dev on its own: LU of a row-major n x n FP32
+ * device matrix in place (unit L below the diagonal, U on and above), trailing updates in
+ * prec (DDLA_PREC_FP32 / _FP16 / _BF16), pivoting DDLA_PIVOT_PARTIAL or _TOURNAMENT.
+ * perm (device, n entries): row i of the factored matrix is row perm[i] of the input.
+ * Returns > 0 (first zero pivot column + 1) for a singular factor.  No reference
+ * counterpart (a preconditioner for callers running their own refinement). */
+int ddla_sgetrf_mp_dev(ddla_dim_t n, float* a, ddla_inc_t lda, int32_t prec, int32_t pivoting,
+                       int32_t* perm);
 
 /* ------------------------------------------------ batched uneven factorizations ----- */
 /* No counterpart in the reference (DDLA factors one matrix per call: LU.PLUDecompose,

@@ -19,6 +19,8 @@
 //   TRSM      operation of DDLA/BLAS/3m.cs:287-314 (facade -> bli_dtrsm), blocked like
 //             DDLA/BLAS/Managed/3m/TrSM.cs:62-195 (KC-row blocks, GEMM update + TRSV)
 //   SYRK      operation of DDLA/BLAS/3m.cs:118-147 (facade -> bli_dsyrk)
+//   TSLU      no reference counterpart: the FP32 tournament-pivoting panel of the mixed LU
+//             (CALU, Grigori-Demmel-Xiang 2011), in the GPU kernel's operation order
 //
 // Bit-faithfulness: for alpha=1, beta=0 the GEMM result of every element is
 //   c = (((0 + s_0) + s_1) + ... ) with s_p = fma-chain over k in KC-block p, from 0.0,
@@ -483,6 +485,95 @@ uint64_t splitmix(uint64_t x) {
   return x;
 }
 
+// ------------------------------------------------------------ tournament pivoting -------
+// No counterpart in the reference (its LU pivots column by column, LU.cs:344-374): the FP32
+// tournament-pivoting panel of the mixed-precision LU, i.e. CALU's tournament (Grigori,
+// Demmel, Xiang, SIAM J. Matrix Anal. Appl. 32, 2011) over stacks of 256 rows, restated
+// serially.  The elimination of a stack follows the GPU kernel's operation order (columns in
+// blocks of 16; each block's U12 by forward substitution over the block's pivot rows, then
+// the rank-16 update of the remaining rows; every update one fmaf, index ascending; the
+// multiplier is the value times the pivot's reciprocal), so selection and factored rows can
+// be compared bit for bit.
+namespace tourn {
+
+constexpr int ROWS = 256, NB = 128, CB = 16;
+
+// GEPP on the cnt x NB row-major stack S in place (rows not moved).  order: stack rows in pivot
+// order (min(cnt, NB) of them).  Returns the first zero-pivot column + 1, or 0.
+int stack_gepp(std::vector<float>& S, int cnt, std::vector<int>& order) {
+  order.clear();
+  if (cnt <= 0) return 0;
+  const int npiv = std::min(cnt, NB);
+  order.assign(npiv, 0);
+  std::vector<char> active(cnt, 1);
+  int zero = 0;
+  auto at = [&](int r, int c) -> float& { return S[(size_t)r * NB + c]; };
+  for (int j0 = 0; j0 < npiv; j0 += CB) {
+    for (int i = 0; i < CB && j0 + i < npiv; ++i) {
+      const int j = j0 + i;
+      float best = -2.f;
+      int p = -1;
+      for (int t = 0; t < cnt; ++t) {  // largest key, first row among equal keys; NaN -> -1
+        const float key = active[t] ? std::fmax(std::fabs(at(t, j)), -1.f) : -2.f;
+        if (p < 0 || key > best) {
+          best = key;
+          p = t;
+        }
+      }
+      order[j] = p;
+      active[p] = 0;
+      if (!(best > 0.f) && zero == 0) zero = j + 1;
+      const float piv = at(p, j);
+      const float rinv = piv != 0.f ? 1.f / piv : 0.f;
+      for (int t = 0; t < cnt; ++t) {
+        if (!active[t]) continue;
+        const float l = at(t, j) * rinv;
+        at(t, j) = l;
+        for (int c = j + 1; c < j0 + CB; ++c) at(t, c) = std::fmaf(-l, at(p, c), at(t, c));
+      }
+    }
+    const int c1 = j0 + CB;
+    if (c1 >= npiv) break;
+    std::vector<float> U12((size_t)(NB - c1) * CB);
+    for (int c = c1; c < NB; ++c) {
+      float u[CB];
+      for (int i = 0; i < CB; ++i) {
+        const int pr = order[j0 + i];
+        float x = at(pr, c);
+        for (int k = 0; k < i; ++k) x = std::fmaf(-at(pr, j0 + k), u[k], x);
+        u[i] = x;
+        at(pr, c) = x;
+        U12[(size_t)(c - c1) * CB + i] = x;
+      }
+    }
+    for (int t = 0; t < cnt; ++t) {
+      if (!active[t]) continue;
+      for (int c = c1; c < NB; ++c) {
+        float x = at(t, c);
+        for (int i = 0; i < CB; ++i) x = std::fmaf(-at(t, j0 + i), U12[(size_t)(c - c1) * CB + i], x);
+        at(t, c) = x;
+      }
+    }
+  }
+  return zero;
+}
+
+// Stack of the panel rows `rows` (original values), eliminated; winners (panel rows, pivot
+// order) returned, factored stack left in S.
+std::vector<int> reduce(const float* P, int64_t ld, const std::vector<int>& rows,
+                        std::vector<float>& S, std::vector<int>& order, int* zero) {
+  const int cnt = (int)rows.size();
+  S.assign((size_t)cnt * NB, 0.f);
+  for (int r = 0; r < cnt; ++r)
+    for (int c = 0; c < NB; ++c) S[(size_t)r * NB + c] = P[(int64_t)rows[r] * ld + c];
+  *zero = stack_gepp(S, cnt, order);
+  std::vector<int> win(order.size());
+  for (size_t i = 0; i < order.size(); ++i) win[i] = rows[order[i]];
+  return win;
+}
+
+}  // namespace tourn
+
 }  // namespace
 
 extern "C" {
@@ -598,6 +689,70 @@ int orc_dpotrs(int uplo, int64_t n, int64_t nrhs, const double* a, int64_t rsa, 
     trsm(kLeft, kUpper, kNoTrans, kNonUnit, 1.0, A, B);
   }
   return 0;
+}
+
+// Tournament-pivoting panel (FP32, m >= 128 rows of 128 columns, row-major, ld): leaves of
+// 256 consecutive rows, a binary tree over them (left set stacked above right set; the last
+// node of an odd level moves up unchanged).  Outputs: sel[128] = panel rows in pivot order,
+// top[128 x 128] = their factored rows (L11 \ U11), the (dst, src) row list of the panel's
+// interchanges (selected rows to the top; displaced top rows, ascending, into the vacated rows
+// in pivot order) and its length.  Returns the root's first zero-pivot column + 1, or 0.
+int orc_stslu(int64_t m, const float* P, int64_t ld, int32_t* sel, float* top, int32_t* list,
+              int32_t* count) {
+  using namespace tourn;
+  if (m < NB) return -1;
+  const int leaves = (int)((m + ROWS - 1) / ROWS);
+  std::vector<std::vector<int>> sets(leaves);
+  std::vector<float> S;
+  std::vector<int> order;
+  int zero = 0;
+  for (int l = 0; l < leaves; ++l) {
+    std::vector<int> rows;
+    for (int64_t r = (int64_t)l * ROWS; r < std::min<int64_t>(m, (int64_t)(l + 1) * ROWS); ++r)
+      rows.push_back((int)r);
+    sets[l] = reduce(P, ld, rows, S, order, &zero);
+  }
+  while (sets.size() > 1) {
+    std::vector<std::vector<int>> up;
+    for (size_t i = 0; i < sets.size(); i += 2) {
+      if (i + 1 == sets.size()) {
+        up.push_back(sets[i]);
+        continue;
+      }
+      std::vector<int> rows = sets[i];
+      rows.insert(rows.end(), sets[i + 1].begin(), sets[i + 1].end());
+      up.push_back(reduce(P, ld, rows, S, order, &zero));
+    }
+    sets.swap(up);
+  }
+  // S / order / zero are the root's (the last reduction performed)
+  const std::vector<int>& win = sets[0];
+  for (int i = 0; i < NB; ++i) {
+    sel[i] = win[i];
+    for (int c = 0; c < NB; ++c) top[(size_t)i * NB + c] = S[(size_t)order[i] * NB + c];
+  }
+  std::vector<char> is_top_sel(NB, 0);
+  for (int i = 0; i < NB; ++i)
+    if (win[i] < NB) is_top_sel[win[i]] = 1;
+  std::vector<int> disp, vac;
+  for (int p = 0; p < NB; ++p)
+    if (!is_top_sel[p]) disp.push_back(p);
+  for (int i = 0; i < NB; ++i)
+    if (win[i] >= NB) vac.push_back(win[i]);
+  int cnt = 0;
+  for (int i = 0; i < NB; ++i)
+    if (win[i] != i) {
+      list[2 * cnt] = i;
+      list[2 * cnt + 1] = win[i];
+      ++cnt;
+    }
+  for (size_t k = 0; k < disp.size(); ++k) {
+    list[2 * cnt] = vac[k];
+    list[2 * cnt + 1] = disp[k];
+    ++cnt;
+  }
+  *count = cnt;
+  return zero;
 }
 
 }  // extern "C"

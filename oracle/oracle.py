@@ -52,6 +52,9 @@ def lib():
         L.orc_dgetrf.restype = i64
         L.orc_dgetf2.argtypes = [i64, i64, dp, i64, i64, ip]
         L.orc_dgetf2.restype = i64
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.orc_stslu.argtypes = [i64, fp, i64, ip, fp, ip, ip]
+        L.orc_stslu.restype = i
         L.orc_dgetrf_nopiv.argtypes = [i64, dp, i64, i64]
         L.orc_dgetrf_nopiv.restype = i64
         L.orc_dlaswp.argtypes = [i, i, i64, i64, dp, i64, i64, ip, i64]
@@ -149,6 +152,24 @@ def getf2(P):
     piv = np.zeros(min(m, nb), dtype=np.int32)
     info = lib().orc_dgetf2(m, nb, _dp(P), *_strides(P), _ip(piv))
     return int(info), piv
+
+
+def tslu(P):
+    """Tournament-pivoting panel (FP32; no reference counterpart, CALU's tournament over stacks
+    of 256 rows).  P: m x 128 float32, m >= 128 (not modified).  Returns (zero, sel, top,
+    pairs): zero = first zero-pivot column + 1 or 0, sel = panel rows in pivot order, top =
+    their factored rows (L11 \\ U11), pairs = (dst, src) row moves."""
+    P = np.ascontiguousarray(P, dtype=np.float32)
+    m, nb = P.shape
+    assert nb == 128 and m >= 128
+    sel = np.zeros(128, dtype=np.int32)
+    top = np.zeros((128, 128), dtype=np.float32)
+    lst = np.zeros(512, dtype=np.int32)
+    cnt = np.zeros(1, dtype=np.int32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    zero = lib().orc_stslu(m, P.ctypes.data_as(fp), nb, _ip(sel), top.ctypes.data_as(fp),
+                           _ip(lst), _ip(cnt))
+    return int(zero), sel, top, lst[: 2 * int(cnt[0])].reshape(-1, 2)
 
 
 def getrf_nopiv(A):

@@ -284,3 +284,111 @@ def test_uniform_stream_closed_form():
         assert x[i] == (z >> 11) * 2.0**-52 - 1.0
     y = O.uniform(100000, 7)
     assert -1.0 <= y.min() and y.max() < 1.0 and abs(y.mean()) < 0.01
+
+
+# ------------------------------------------------------------------ tournament pivoting ---
+def designed_panel(m, seed, off=0.0):
+    """m x 128 panel whose 128 pivot rows are known: rows e_j (1 in column j, `off`-sized
+    entries after column j: one unit row of a unit lower trapezoid) at random places, every
+    other row zero, so partial pivoting and the tournament must pick e_0, e_1, ... in order."""
+    rng = np.random.default_rng(seed)
+    P = np.zeros((m, 128), dtype=np.float32)
+    where = rng.permutation(m)[:128]
+    for j, r in enumerate(where):
+        P[r, j] = 1.0
+        P[r, j + 1:] = off * rng.uniform(-1, 1, 127 - j)
+    return P, where
+
+
+def check_tslu_structure(P, zero, sel, top, pairs):
+    m = P.shape[0]
+    assert len(set(sel.tolist())) == 128 and sel.min() >= 0 and sel.max() < m
+    # L11 U11 reproduces the selected rows (FP32 elimination)
+    L = np.tril(top.astype(np.float64), -1) + np.eye(128)
+    U = np.triu(top.astype(np.float64))
+    A = P[sel].astype(np.float64)
+    assert np.linalg.norm(L @ U - A) <= 1e-5 * np.linalg.norm(A)
+    # the pairs are one permutation that brings sel to the top
+    src = np.arange(m)
+    new = src.copy()
+    new[pairs[:, 0]] = src[pairs[:, 1]]
+    assert sorted(new.tolist()) == list(range(m))
+    np.testing.assert_array_equal(new[:128], sel)
+
+
+@pytest.mark.parametrize("m", [128, 200, 256, 700, 1300, 2100])
+def test_tslu_designed_pivot_order(oracle, m):
+    # every leaf keeps the unit rows it holds and every merge passes them on (they hold 1
+    # where all other candidates hold 0), so the root eliminates them in column order;
+    # 1300 / 2100 rows: trees with odd levels (6, 3, 2, 1 and 9, 5, 3, 2, 1 nodes)
+    P, where = designed_panel(m, m)
+    zero, sel, top, pairs = oracle.tslu(P)
+    assert zero == 0
+    np.testing.assert_array_equal(sel, where)
+    check_tslu_structure(P, zero, sel, top, pairs)
+    # partial pivoting (the reference's panel rule) picks the same rows
+    _, piv = oracle.getf2(P.astype(np.float64).copy())
+    order = np.arange(m)
+    for i, p in enumerate(piv):
+        order[[i, i + p]] = order[[i + p, i]]
+    np.testing.assert_array_equal(order[:128], where)
+
+
+@pytest.mark.parametrize("m", [150, 256])
+def test_tslu_single_stack_is_partial_pivoting(oracle, m):
+    # one stack: the tournament is GEPP of the panel.  P = L0 U0 with L0 = unit rows at known
+    # places and |l| < 0.45 elsewhere: eliminating with a unit row changes no later column
+    # of L0, so each unit row wins its column by more than a factor of two, and right
+    # multiplication by the upper triangular U0 keeps that choice.  The reference panel
+    # (partial pivoting) picks the same rows and its factored top equals the tournament's
+    # to FP32 accuracy.
+    L0, where = designed_panel(m, 7 + m)
+    rng = np.random.default_rng(m)
+    rest = np.setdiff1d(np.arange(m), where)
+    L0 = L0.astype(np.float64)
+    L0[rest] = 0.45 * rng.uniform(-1, 1, (rest.size, 128))
+    U0 = np.triu(rng.uniform(-1, 1, (128, 128)), 1) + np.diag(rng.uniform(1, 2, 128))
+    P = (L0 @ U0).astype(np.float32)
+    zero, sel, top, pairs = oracle.tslu(P)
+    assert zero == 0
+    np.testing.assert_array_equal(sel, where)
+    ref = P.astype(np.float64)
+    _, piv = oracle.getf2(ref)
+    order = np.arange(m)
+    for i, p in enumerate(piv):
+        order[[i, i + p]] = order[[i + p, i]]
+    np.testing.assert_array_equal(order[:128], where)
+    np.testing.assert_allclose(top, ref[:128], rtol=0, atol=1e-5 * np.abs(ref[:128]).max())
+    check_tslu_structure(P, zero, sel, top, pairs)
+
+
+@pytest.mark.parametrize("m", [300, 1024, 4000])
+def test_tslu_first_pivot_is_column_max_and_structure(oracle, m):
+    # the global arg-max of column 0 wins its stack and every merge: the first pivot equals
+    # partial pivoting's; the rest is checked structurally
+    P = np.random.default_rng(m).uniform(-1, 1, (m, 128)).astype(np.float32)
+    zero, sel, top, pairs = oracle.tslu(P)
+    assert zero == 0
+    assert sel[0] == int(np.argmax(np.abs(P[:, 0])))
+    check_tslu_structure(P, zero, sel, top, pairs)
+    # multipliers of the selected rows stay bounded (pivot growth close to GEPP's)
+    assert np.abs(np.tril(top, -1)).max() <= 1.0 + 1e-6
+
+
+def test_tslu_dominant_top_block_keeps_order(oracle):
+    # a strongly diagonally dominant top block over tiny rows: no interchange at all
+    rng = np.random.default_rng(5)
+    m = 1500
+    P = (1e-3 * rng.uniform(-1, 1, (m, 128))).astype(np.float32)
+    P[:128] += (10 * np.eye(128) + 0.05 * rng.uniform(-1, 1, (128, 128))).astype(np.float32)
+    zero, sel, top, pairs = oracle.tslu(P)
+    np.testing.assert_array_equal(sel, np.arange(128))
+    assert pairs.shape[0] == 0
+    check_tslu_structure(P, zero, sel, top, pairs)
+
+
+def test_tslu_zero_column_reports_info(oracle):
+    P = np.random.default_rng(2).uniform(-1, 1, (600, 128)).astype(np.float32)
+    P[:, 40] = 0.0
+    zero, sel, top, pairs = oracle.tslu(P)
+    assert zero == 41
