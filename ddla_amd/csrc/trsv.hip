@@ -121,8 +121,9 @@ constexpr int solve_bs() {
 // Layout of the prepared data behind Dinv (trsv_prepare): the inverted 64 x 64 diagonal
 // blocks, then per block row kb of the solve (BS rows) Dfull_kb = inverse of the whole
 // diagonal block, then M_kb = Dfull_kb * op(T)[kb, kprev], kprev = the block solved just
-// before kb.  With them x_kb = Dfull_kb (b_kb - sum_{older blocks} T x) - M_kb x_kprev: the
-// part that waits for the latest neighbour is a single product.
+// before kb, then N_kb = Dfull_kb * op(T)[kb, kprev2], kprev2 = the one before kprev.  With
+// them x_kb = Dfull_kb (b_kb - sum_{older blocks} T x) - N_kb x_kprev2 - M_kb x_kprev: each
+// of the two latest neighbours enters through a single product.
 __host__ __device__ inline int64_t blocks_of(int64_t n, int64_t b) { return (n + b - 1) / b; }
 __host__ __device__ inline int64_t inv64_elems(int64_t n) {
   return blocks_of(n, BS_INV) * BS_INV * BS_INV;
@@ -201,13 +202,30 @@ __global__ __launch_bounds__(256) void trsv_combine_kernel(int64_t n, const T* _
     for (int k = 0; k < BS; ++k) acc = fma(S1[r][k], S2[k][c], acc);
     Mb[idx] = acc;
   }
+  const int kprev2 = LOWER ? kb - 2 : kb + 2;
+  if (kprev2 < 0 || kprev2 >= nblk) return;  // uniform
+  T* Nb = Mb + blocks_elems<BS>(n);
+  const int64_t c2 = (int64_t)kprev2 * BS;
+  __syncthreads();  // S2 is rewritten
+  for (int idx = tid; idx < BS * BS; idx += 256) {
+    const int r = idx / BS, c = idx % BS;
+    S2[r][c] = opT(r0 + r, c2 + c);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < BS * BS; idx += 256) {
+    const int r = idx / BS, c = idx % BS;
+    T acc = T(0);
+    for (int k = 0; k < BS; ++k) acc = fma(S1[r][k], S2[k][c], acc);
+    Nb[idx] = acc;
+  }
 }
 
 template <typename T, int BS, int NR, bool LOWER, bool TRANS>
 __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T* __restrict__ A,
                                                    int64_t lda, const T* __restrict__ Dinv,
                                                    T* __restrict__ B, int64_t ldb,
-                                                   u64* __restrict__ XW, unsigned tag) {
+                                                   u64* __restrict__ XW, unsigned tag,
+                                                   int split) {
   constexpr int Q = 256 / BS;       // threads per row of a block product (k split)
   constexpr int KQ = BS / Q;        // k range per thread
   constexpr int E = BS * BS / 256;  // staged block elements per thread
@@ -228,8 +246,14 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
   const int row = tid % BS, q = tid / BS;
   const T* Dfull = Dinv + inv64_elems(n);
   const T* Mall = Dfull + blocks_elems<BS>(n);
+  const T* Nall = Mall + blocks_elems<BS>(n);
 
-  for (int jb = blockIdx.x; jb < nblk; jb += G) {
+  // split (one block row per workgroup, 2 nblk workgroups): workgroup nblk + jb helps block
+  // row jb by streaming its odd blocks and publishing the partial sums (HW words)
+  const bool helper = split && (int)blockIdx.x >= nblk;
+  u64* HW = XW + n * nrhs * 2;
+  for (int jb = helper ? (int)blockIdx.x - nblk : (int)blockIdx.x; jb < nblk;
+       jb += split ? nblk : G) {
     const int kb = LOWER ? jb : nblk - 1 - jb;  // dependency order
     const int64_t r0 = (int64_t)kb * BS;
     const int rows = (int)min<int64_t>(BS, n - r0);
@@ -335,10 +359,22 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
       for (int c = 0; c < NR; ++c) red[q][row][c] = acc[c];
     };
 
-    // operands that do not depend on X: M (to LDS, through the staging registers), and B
+    // operands that do not depend on X: M (to LDS, through the staging registers), this
+    // thread's k-range of N's row (registers), and B
     if (jb > 0) {
       load_dense(Mall + (int64_t)kb * BS * BS);
       store_block(false, Ms);
+    }
+    T nreg[KQ];
+    if (jb > 1) {
+      const T* np = Nall + (int64_t)kb * BS * BS + (int64_t)row * BS + q * KQ;
+      constexpr int NV = KQ * sizeof(T) / 16;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const vecT x = *reinterpret_cast<const vecT*>(np + v * VW);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) nreg[v * VW + e] = x[e];
+      }
     }
     T breg[XPT];
 #pragma unroll
@@ -350,17 +386,23 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
     T part[NR];
 #pragma unroll
     for (int c = 0; c < NR; ++c) part[c] = T(0);
-    const int nstream = jb > 0 ? jb - 1 : 0;
-    if (jb > 0) load_x(0);
-    if (nstream > 0) load_block(0);
-    else load_dfull();
-    for (int jc = 0; jc < nstream; ++jc) {
+    const int nstream = jb > 1 ? jb - 2 : 0;
+    const int jstep = split ? 2 : 1, jfirst = helper ? 1 : 0;
+    const bool helped = split && nstream / 2 > 0;  // the helper streams a block of this row
+    if (jfirst < nstream) {
+      load_x(jfirst);
+      load_block(jfirst);
+    } else if (!helper) {
+      load_dfull();
+    }
+    for (int jc = jfirst; jc < nstream; jc += jstep) {
+      const int jn = jc + jstep;
       __syncthreads();  // the previous block's product is done with Ts / xs
       store_block(TRANS, Ts);
+      if (jn < nstream) load_block(jn);  // the next block's loads overlap the wait for x
+      else if (!helper) load_dfull();
       use_x(jc);
-      if (jc + 1 < jb) load_x(jc + 1);
-      if (jc + 1 < nstream) load_block(jc + 1);
-      else load_dfull();
+      if (jn < nstream) load_x(jn);
       __syncthreads();
 #pragma unroll 4
       for (int k = q * KQ; k < q * KQ + KQ; ++k) {
@@ -368,6 +410,22 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
 #pragma unroll
         for (int c = 0; c < NR; ++c) part[c] = fma(a, xs[k][c], part[c]);
       }
+    }
+    if (helper) {  // partial sums of the odd blocks to the row's workgroup
+      __syncthreads();
+      product(part);
+      __syncthreads();
+      if (nstream / 2 > 0) {
+        for (int idx = tid; idx < BS * NR; idx += 256) {
+          const int r = idx / NR, c = idx % NR;
+          T sum = T(0);
+#pragma unroll
+          for (int qq = 0; qq < Q; ++qq) sum += red[qq][r][c];
+          if (r < rows && c < nrhs) publish<T>(HW + ((r0 + r) * nrhs + c) * NW, sum, tag);
+        }
+      }
+      __syncthreads();
+      continue;
     }
     // y = Dfull (b - sum)
     __syncthreads();
@@ -382,6 +440,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
         T sum = T(0);
 #pragma unroll
         for (int qq = 0; qq < Q; ++qq) sum += red[qq][r][c];
+        if (helped && c < nrhs && r < rows) sum += await<T>(HW + ((r0 + r) * nrhs + c) * NW, tag);
         xs[r][c] = (c < nrhs && r < rows) ? breg[t] - sum : T(0);
       }
     }
@@ -404,8 +463,32 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
       for (int qq = 0; qq < Q; ++qq) sum += red[qq][r][c];
       ys[r][c] = sum;
     }
+    // y -= N x_kprev2 (the block solved two steps before)
+    if (jb > 1) {
+      load_x(jb - 2);  // use_x only re-reads words that do not carry this call's tag yet
+      use_x(jb - 2);
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = T(0);
+#pragma unroll
+      for (int kk = 0; kk < KQ; ++kk) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] = fma(nreg[kk], xs[q * KQ + kk][c], acc[c]);
+      }
+      product(acc);
+      __syncthreads();
+      for (int idx = tid; idx < BS * NR; idx += 256) {
+        const int r = idx / NR, c = idx % NR;
+        T sum = T(0);
+#pragma unroll
+        for (int qq = 0; qq < Q; ++qq) sum += red[qq][r][c];
+        ys[r][c] -= sum;
+      }
+      __syncthreads();  // xs is rewritten
+    }
     // x = y - M x_kprev: the only step that waits for the block solved just before
     if (jb > 0) {
+      load_x(jb - 1);
       use_x(jb - 1);
       __syncthreads();
 #pragma unroll
@@ -495,7 +578,7 @@ int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64
 
 // inverted 64-blocks + Dfull + M (sized for the larger FP32 solve block)
 size_t trsv_dinv_elems(int64_t n) {
-  return (size_t)(inv64_elems(n) + 2 * blocks_elems<128>(n));
+  return (size_t)(inv64_elems(n) + 3 * blocks_elems<128>(n));
 }
 
 // op(T) X = B in place, nrhs <= 8, with Dinv from trsv_prepare (same lower/trans).
@@ -509,10 +592,14 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
   }
   static unsigned tag = 0;
   void* ws = nullptr;
-  DDLA_RETURN_IF(workspace(WS_GMRES, (size_t)n * nrhs * 2 * sizeof(u64), &ws));
+  // X words, then the helpers' partial-sum words (2 words per FP64 value)
+  const size_t wbytes = (size_t)n * nrhs * 4 * sizeof(u64);
+  DDLA_RETURN_IF(workspace(WS_GMRES, wbytes, &ws));
   static void* zeroed = nullptr;
-  if (zeroed != ws) {
-    DDLA_HIP_CHECK(hipMemsetAsync(ws, 0, (size_t)n * nrhs * 2 * sizeof(u64), s));
+  static size_t zeroed_bytes = 0;
+  if (zeroed != ws || zeroed_bytes < wbytes) {
+    DDLA_HIP_CHECK(hipMemsetAsync(ws, 0, wbytes, s));
+    zeroed_bytes = wbytes;
     zeroed = ws;
     tag = 0;
   }
@@ -526,10 +613,15 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
   int nrhs_ = nrhs;
   int64_t lda_ = lda, ldb_ = ldb;
   unsigned tag_ = tag;
-  void* args[] = {&n_, &nrhs_, &A, &lda_, &Dinv, &B, &ldb_, &XW, &tag_};
+  int split_ = 0;
+  void* args[] = {&n_, &nrhs_, &A, &lda_, &Dinv, &B, &ldb_, &XW, &tag_, &split_};
+  // a helper workgroup per block row when twice the rows fit co-resident: the longest rows
+  // stream their blocks with two workgroups
 #define TS(NR, L, TR)                                                                    \
   do {                                                                                   \
-    const int G = std::min(nblk, max_trsv_groups<T, L, TR>());                           \
+    const int maxg = max_trsv_groups<T, L, TR>();                                        \
+    split_ = 2 * nblk <= maxg ? 1 : 0;                                                   \
+    const int G = split_ ? 2 * nblk : std::min(nblk, maxg);                              \
     DDLA_HIP_CHECK(hipLaunchCooperativeKernel(                                           \
         reinterpret_cast<const void*>(trsv_kernel<T, solve_bs<T>(), NR, L, TR>), dim3(G), \
         dim3(256), args, 0, s));                                                         \
