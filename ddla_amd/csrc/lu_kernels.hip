@@ -362,35 +362,26 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
           better(key, row, __builtin_bit_cast(double, ((a[j] >> 32) << 32) | (b[j] >> 32)),
                  static_cast<int>(c[j] >> 32));
       }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double k2 = __shfl_xor(key, off);
-        const int r2 = __shfl_xor(row, off);
-        better(key, row, k2, r2);
-      }
-      if (lane == 0) {
-        sk[1][0] = key;
-        sr[1][0] = row;
-      }
-    }
-    __syncthreads();  // also: every lane is done with last step's prow / drow
-    key = sk[1][0];
-    row = sr[1][0];
-    row = __builtin_amdgcn_readfirstlane(row);
-    const int wwin = row / R;
-    const bool own_p = (row != i) && (row >= r0 && row < r0 + R);
-    {
+      wave_argmax_dpp(key, row);  // DPP across the lanes (every lane ends with the winner)
+      // the winner's row (and the diagonal row for the workgroup that gives its row up),
+      // read by this wave right away, to LDS for the others
+      row = __builtin_amdgcn_readfirstlane(row);
+      const int wwin = row / R;
+      const bool own_p = (row != i) && (row >= r0 && row < r0 + R);
       const u64* cw = sy.cand + ((size_t)par * WMAX + wwin) * NBX * 2;
       const u64* dg = sy.diag + (size_t)par * NBX * 2;
-      for (int c = tid; c < NBX; c += NT) {
+      for (int c = lane; c < NBX; c += 64) {
         if (own_p && !own_i) {
           wait_elem2<T>(cw + 2 * c, dg + 2 * c, tag, prow[c], drow[c]);
         } else {
           prow[c] = wait_elem<T>(cw + 2 * c, tag);
         }
       }
+      if (lane == 0) sr[1][0] = row;
     }
-    __syncthreads();
+    __syncthreads();  // prow / drow / the winner in LDS
+    row = __builtin_amdgcn_readfirstlane(sr[1][0]);
+    const bool own_p = (row != i) && (row >= r0 && row < r0 + R);
     const T pv = prow[i];
     if (w == 0 && tid == 0) piv[i] = row - i;
     const bool stop = (pv == static_cast<T>(0));
