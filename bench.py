@@ -250,6 +250,105 @@ def cpu_baseline(n_target, n_sample, nrhs):
                        f"to n={n_target}")}
 
 
+def timed(fn):
+    fn()
+    import ddla_amd as ddla
+    ddla.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    ddla.synchronize()
+    return r, time.perf_counter() - t0
+
+
+def bench_extras(torch, ddla, A, A_t, n, args, t_fp64):
+    """BF16 trailing updates and 64 right-hand sides on the headline matrix; then the mixed
+    Cholesky configuration: SPD A = Q diag(l) Q^T, l geometric from 1 to 1e-5, FP16 and BF16
+    factors, next to the plain FP64 Cholesky on the GPU."""
+    out = []
+    dev = torch.device("cuda")
+
+    def rhs(nrhs, seed):
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        xt = torch.rand(n, nrhs, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        return xt, (A_t @ xt).contiguous()
+
+    def rec(name, info, dt, xt, X_t, extra=None):
+        fe = float(((X_t - xt).abs().amax() / xt.abs().amax()))
+        e = {"config": name, "ms_per_solve": dt * 1e3, "iters": info.iters, "status": info.status,
+             "fallback": info.fallback, "backward_error": info.backward_error,
+             "forward_error_vs_xtrue": fe, "factor_ms": info.factor_ms,
+             "refine_ms": info.refine_ms}
+        if extra:
+            e.update(extra)
+        out.append(e)
+        log(f"{name}: {dt * 1e3:.1f} ms iters={info.iters} berr={info.backward_error:.2e} "
+            f"fallback={info.fallback}")
+
+    for prec_name, nrhs in (("bf16", 1), ("fp16", 64)):
+        xt, b_t = rhs(nrhs, args.seed + 7 + nrhs)
+        X_t = torch.zeros(n, nrhs, dtype=torch.float64, device=dev)
+        B = ddla.DeviceMatrix.wrap(b_t.data_ptr(), n, nrhs, nrhs, b_t)
+        X = ddla.DeviceMatrix.wrap(X_t.data_ptr(), n, nrhs, nrhs, X_t)
+        prec = ddla.PREC_BF16 if prec_name == "bf16" else ddla.PREC_FP16
+        info, dt = timed(lambda: ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14))
+        rec(f"lu {prec_name} kappa={args.kappa:g} nrhs={nrhs}", info, dt, xt, X_t,
+            {"speedup_vs_fp64_lu_1rhs": t_fp64 / dt})
+        del B, X, X_t, b_t, xt
+
+    # Cholesky: one QR of a Gaussian
+    g = torch.Generator(device=torch.device(args.qr_device))
+    g.manual_seed(args.seed + 2)
+    Q, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=args.qr_device,
+                                       generator=g))
+    Q = Q.to(dev)
+    lam = 10.0 ** (-5.0 * torch.arange(n, dtype=torch.float64, device=dev) / (n - 1))
+    S_t = ((Q * lam) @ Q.T.contiguous())
+    S_t = ((S_t + S_t.T) * 0.5).contiguous()
+    del Q
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed + 3)
+    xt = torch.rand(n, 1, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    b_t = (S_t @ xt).contiguous()
+    S = ddla.DeviceMatrix.wrap(S_t.data_ptr(), n, n, n, S_t)
+    B = ddla.DeviceMatrix.wrap(b_t.data_ptr(), n, 1, 1, b_t)
+    # the plain FP64 GPU Cholesky + solve on the same system
+    W_t = torch.empty_like(S_t)
+    Xf_t = torch.empty_like(b_t)
+    W = ddla.DeviceMatrix.wrap(W_t.data_ptr(), n, n, n, W_t)
+    Xf = ddla.DeviceMatrix.wrap(Xf_t.data_ptr(), n, 1, 1, Xf_t)
+
+    def chol64():
+        W_t.copy_(S_t)
+        Xf_t.copy_(b_t)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ddla.CholDecompose(ddla.LOWER, W)
+        ddla.CholeskySolve(ddla.LOWER, W, Xf)
+        ddla.synchronize()
+        return time.perf_counter() - t0
+
+    chol64()
+    t_chol64 = chol64()
+    be64 = backward_error(torch, S_t, Xf_t, b_t)
+    del W, W_t
+    out.append({"config": "cholesky fp64 (plain GPU)", "ms_per_solve": t_chol64 * 1e3,
+                "backward_error": be64})
+    log(f"cholesky fp64: {t_chol64 * 1e3:.1f} ms berr={be64:.2e}")
+    for prec_name in ("fp16", "bf16"):
+        X_t = torch.zeros(n, 1, dtype=torch.float64, device=dev)
+        X = ddla.DeviceMatrix.wrap(X_t.data_ptr(), n, 1, 1, X_t)
+        prec = ddla.PREC_BF16 if prec_name == "bf16" else ddla.PREC_FP16
+        info, dt = timed(lambda: ddla.posv_mpir(S, B, X, prec=prec, tol=1e-14))
+        fe64 = float(((X_t - Xf_t).abs().amax() / Xf_t.abs().amax()))
+        rec(f"cholesky {prec_name} lambda 1..1e-5 nrhs=1", info, dt, xt, X_t,
+            {"speedup_vs_fp64_cholesky": t_chol64 / dt, "forward_error_vs_fp64_cholesky": fe64})
+        del X, X_t
+    del S, S_t, B, b_t, Xf, Xf_t
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +371,9 @@ def main():
                     help="problems in the batched-factorization measurement (0 skips it)")
     ap.add_argument("--qr-device", choices=["cuda", "cpu"], default="cuda",
                     help="where the test matrix's QR factors are computed")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other mixed configurations (BF16, 64 right-hand sides, "
+                         "Cholesky), one timed solve each")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -387,6 +489,10 @@ def main():
     breakdown64 = {k: {"launches": v["count"] // nfp, "ms_per_solve": v["ms"] / nfp}
                    for k, v in sorted(prof64.items(), key=lambda kv: -kv[1]["ms"])}
 
+    # ---- the other mixed configurations at this kappa, one timed solve each: BF16 updates,
+    #      64 right-hand sides, and the Cholesky + refinement configuration
+    extras = None if args.no_extras else bench_extras(torch, ddla, A, A_t, n, args, t_fp64)
+
     # ---- the other condition numbers of the configuration, one timed solve each
     sweep = []
     for ks in [x for x in args.sweep_kappa.split(",") if x.strip()]:
@@ -442,6 +548,7 @@ def main():
                   "factor_ms": last.factor_ms, "refine_ms": last.refine_ms},
         "mixed_other_pivoting": other_out or None,
         "kappa_sweep": sweep,
+        "extra_configs": extras,
         "fp64_lu": {"ms_per_solve": t_fp64 * 1e3, "solves_per_s": 1.0 / t_fp64,
                     "backward_error": be_fp64,
                     "gflops": (2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs) / t_fp64 / 1e9},

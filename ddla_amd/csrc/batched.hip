@@ -221,11 +221,11 @@ __device__ void lu_problem(int n, T* __restrict__ M, int32_t* __restrict__ P,
   }
 }
 
-// Lower Cholesky of one n x n row-major problem: a11 = sqrt(a11), a21 *= 1/a11,
+// Lower Cholesky of one n x n row-major problem (row stride ld): a11 = sqrt(a11), a21 *= 1/a11,
 // A22 -= a21 a21^T on the lower triangle (Cholesky.cs:127-140); only the lower triangle is
 // written.  *info = first non-positive pivot + 1.
 template <typename T, int NB>
-__device__ void chol_problem(int n, T* __restrict__ M, int32_t* __restrict__ info,
+__device__ void chol_problem(int n, T* __restrict__ M, int64_t ld, int32_t* __restrict__ info,
                              Smem<T>& sm) {
   const int tid = threadIdx.x;
   for (int j0 = 0; j0 < n; j0 += NB) {
@@ -234,7 +234,7 @@ __device__ void chol_problem(int n, T* __restrict__ M, int32_t* __restrict__ inf
     T v[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c)
-      v[c] = (own && c < jb) ? M[(int64_t)(j0 + tid) * n + j0 + c] : T(0);
+      v[c] = (own && c < jb) ? M[(int64_t)(j0 + tid) * ld + j0 + c] : T(0);
     if (tid == 0) sm.dsh = v[0];
     __syncthreads();
     bool stop = false;
@@ -268,7 +268,7 @@ __device__ void chol_problem(int n, T* __restrict__ M, int32_t* __restrict__ inf
     if (own) {
 #pragma unroll
       for (int c = 0; c < NB; ++c)
-        if (c < jb && c <= tid) M[(int64_t)(j0 + tid) * n + j0 + c] = v[c];
+        if (c < jb && c <= tid) M[(int64_t)(j0 + tid) * ld + j0 + c] = v[c];
 #pragma unroll
       for (int c = 0; c < NB; ++c) sm.Pn[tid][c] = v[c];
     }
@@ -280,7 +280,7 @@ __device__ void chol_problem(int n, T* __restrict__ M, int32_t* __restrict__ inf
         T x[CH];
 #pragma unroll
         for (int c = 0; c < CH; ++c)
-          x[c] = (c0 + c <= row) ? M[(int64_t)row * n + c0 + c] : T(0);
+          x[c] = (c0 + c <= row) ? M[(int64_t)row * ld + c0 + c] : T(0);
         const int lc0 = c0 - j0;  // panel row of column c0
 #pragma unroll 2
         for (int k = 0; k < jb; ++k) {
@@ -291,7 +291,7 @@ __device__ void chol_problem(int n, T* __restrict__ M, int32_t* __restrict__ inf
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c)
-          if (c0 + c <= row) M[(int64_t)row * n + c0 + c] = x[c];
+          if (c0 + c <= row) M[(int64_t)row * ld + c0 + c] = x[c];
       }
     }
     __syncthreads();
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs<T> args) {
     if (args.kind[p] == DDLA_BATCH_LU)
       lu_problem<T, NB>(n, M, args.piv + args.poff[p], args.info + p, sm);
     else
-      chol_problem<T, NB>(n, M, args.info + p, sm);
+      chol_problem<T, NB>(n, M, n, args.info + p, sm);
     __syncthreads();  // the next problem reuses the LDS
   }
 }
@@ -323,6 +323,23 @@ const void* batch_fn() {
 const void* batch_kernel_for(int prec, int nb) {
   if (prec == DDLA_PREC_FP64) return nb == 16 ? batch_fn<double, 16>() : batch_fn<double, 32>();
   return nb == 16 ? batch_fn<float, 16>() : batch_fn<float, 32>();
+}
+
+// The diagonal block of the blocked Cholesky (CholeskyLowerUnblock on it,
+// DDLA/Factorizations/Cholesky.cs:127-140), factored by one workgroup with the batched
+// kernels' blocked in-register scheme; *info = info_base + first bad pivot + 1 (if 0).
+template <typename T>
+__global__ __launch_bounds__(NT) void potf2_block_kernel(int nb, T* __restrict__ A, int64_t lda,
+                                                         int32_t* __restrict__ info,
+                                                         int info_base) {
+  __shared__ Smem<T> sm;
+  __shared__ int32_t local;
+  if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  if (threadIdx.x == 0) local = 0;
+  __syncthreads();
+  chol_problem<T, 16>(nb, A, lda, &local, sm);
+  __syncthreads();
+  if (threadIdx.x == 0 && local != 0) atomicCAS(info, 0, info_base + local);
 }
 
 // relative cost of one problem (used only to order and balance the schedule)
@@ -613,5 +630,21 @@ int batch_describe(const BatchPlan* p, char* buf, size_t len) {
            (long long)p->count, p->nb, p->wg_per_cu, p->G, p->flops, p->bytes, t.c_str());
   return 0;
 }
+
+template <typename T>
+int launch_potf2_block(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s) {
+  if (nb <= 0) return 0;
+  if (nb > NT) {
+    set_error("potf2 block %d > %d", nb, NT);
+    return DDLA_ERR_INVALID_ARG;
+  }
+  ProfScope prof(sizeof(T) == 8 ? "potf2_f64" : "potf2_f32", (double)nb * nb * nb / 3.0,
+                 sizeof(T) * (double)nb * nb, s);
+  hipLaunchKernelGGL(potf2_block_kernel<T>, dim3(1), dim3(NT), 0, s, nb, A, lda, info, info_base);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+template int launch_potf2_block<double>(int, double*, int64_t, int32_t*, int, hipStream_t);
+template int launch_potf2_block<float>(int, float*, int64_t, int32_t*, int, hipStream_t);
 
 }  // namespace ddla

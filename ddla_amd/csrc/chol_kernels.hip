@@ -1,83 +1,21 @@
 // Kernels of the blocked Cholesky (lower).  Reference: CholeskyLowerBlock /
 // CholeskyLowerUnblock / Sqrt (DDLA/Factorizations/Cholesky.cs:127-160,228-236).
 //
-//   potf2_kernel      one workgroup factors the nb x nb diagonal block in LDS
-//                     (sqrt, scale by the reciprocal like InvScaled, lower rank-1 update)
-//   trsm_rlt_kernel   A21 <- A21 * L11^-T, one workgroup per 64 rows of A21, the lower
-//                     triangle of L11 packed in LDS and the slab stored transposed so lanes
-//                     run along rows
+//   potf2             the nb x nb diagonal block: one workgroup, the batched kernels' blocked
+//                     in-register scheme (batched.hip)
+//   trsm_rlt_kernel   A21 <- A21 * L11^-T (or * U11^-1), four lanes per row of A21 with the
+//                     row in registers, the triangle in LDS
 //   transpose_kernel  in-place transpose of a square matrix (the upper factorization is the
 //                     lower one of the transposed storage)
 #include "ddla_internal.hpp"
+
+#include <utility>
 
 namespace ddla {
 namespace {
 
 constexpr int NB = 128;
 
-template <typename T>
-__global__ __launch_bounds__(256) void potf2_kernel(int nb, T* __restrict__ A, int64_t lda,
-                                                    int32_t* __restrict__ info, int info_base) {
-  __shared__ T t[NB * (NB + 1)];
-  __shared__ T s_rinv;
-  __shared__ int s_stop;
-  constexpr int LD = NB + 1;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int r = idx / nb, c = idx % nb;
-    if (c <= r) t[r * LD + c] = A[(int64_t)r * lda + c];
-  }
-  if (tid == 0) s_stop = 0;
-  __syncthreads();
-  for (int i = 0; i < nb; ++i) {
-    if (tid == 0) {
-      const T a = t[i * LD + i];
-      if (!(a > static_cast<T>(0))) {
-        s_stop = 1;
-        atomicCAS(info, 0, info_base + i + 1);
-      } else {
-        const T d = sqrt(a);
-        t[i * LD + i] = d;
-        s_rinv = static_cast<T>(1) / d;
-      }
-    }
-    __syncthreads();
-    if (s_stop) break;
-    const T rinv = s_rinv;
-    for (int r = i + 1 + tid; r < nb; r += 256) t[r * LD + i] *= rinv;
-    __syncthreads();
-    // lower rank-1 update; lane -> columns c0, c1, wave -> rows wave + 4k, batches of rows
-    // whose LDS loads are issued before their stores
-    const int c0 = i + 1 + lane, c1 = c0 + 64;
-    const T w0 = c0 < nb ? t[c0 * LD + i] : T(0), w1 = c1 < nb ? t[c1 * LD + i] : T(0);
-    constexpr int CH = 8;
-    for (int k0 = 0; k0 < NB / 4; k0 += CH) {
-      T x[CH], a0[CH], a1[CH];
-#pragma unroll
-      for (int k = 0; k < CH; ++k) {
-        const T* tr = t + (wave + 4 * (k0 + k)) * LD;
-        x[k] = tr[i];
-        a0[k] = tr[c0 < NB ? c0 : 0];
-        a1[k] = tr[c1 < NB ? c1 : 0];
-      }
-#pragma unroll
-      for (int k = 0; k < CH; ++k) {
-        const int r = wave + 4 * (k0 + k);
-        if (r > i && r < nb) {
-          T* tr = t + r * LD;
-          if (c0 <= r) tr[c0] = fma(w0, -x[k], a0[k]);
-          if (c1 <= r) tr[c1] = fma(w1, -x[k], a1[k]);
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int r = idx / nb, c = idx % nb;
-    if (c <= r) A[(int64_t)r * lda + c] = t[r * LD + c];
-  }
-}
 
 // LU without pivoting of one nb x nb block in LDS (LUDecUnblock, LU.cs:188-203): for each
 // column i, a21 *= 1/a11 (InvScal multiplies by the reciprocal) and A22 -= a21 a12^T.
@@ -139,63 +77,95 @@ __global__ __launch_bounds__(256) void getf2_nopiv_kernel(int nb, T* __restrict_
   }
 }
 
-// B (nrows x nb) <- B * L^-T, L lower nb x nb (non-unit).  Row b of B solves L x^T = b^T.
-// TL: L is given by its transpose, i.e. L[r][c] is read from U[c][r] (B <- B * U^-1 for an
-// upper U stored at the same address).
+// B (nrows x nb) <- B * U^-1 for an upper nb x nb U (non-unit), nb <= 128: TL = U stored as
+// it is (U[k][c] at L[k ldl + c]); otherwise U = L^T for the lower L stored at L (B <- B L^-T).
+// Four lanes per row of B (a DPP quad): lane q holds the column groups 4 (q + 4 t) .. + 3,
+// t < 8, in registers.  Column k is finished by its owner (x = a_k times the reciprocal of
+// u_kk, the inverted-diagonal form of BLIS' TRSM kernels), broadcast to the
+// quad, and every lane subtracts x U[k][c] from its columns c > k with one fma each (U's
+// strictly upper part in LDS, zero beyond nb; 16-byte broadcast reads).  Per row this is the
+// right-looking substitution, k ascending.
+template <int K, typename T>
+__device__ __forceinline__ T quad_bcast(T x) {
+  constexpr int o = (K >> 2) & 3;  // owner lane of column K within the quad
+  constexpr int ctrl = o | (o << 2) | (o << 4) | (o << 6);
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), ctrl, 0xf, 0xf, false);
+    return __builtin_bit_cast(T, (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                     static_cast<unsigned>(lo));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), ctrl, 0xf,
+                                                          0xf, false));
+  }
+}
+
+template <typename T>
+using Quad4 = T __attribute__((ext_vector_type(4)));
+
+template <int K, typename T>
+__device__ __forceinline__ void rlt_step(Quad4<T> (&a)[8], const Quad4<T> (*U)[NB / 4],
+                                         const T* rinv, int q) {
+  constexpr int TT = K / 16, E = K % 4;
+  T x = T(0);
+  if (q == ((K >> 2) & 3)) {
+    x = a[TT][E] * rinv[K];
+    a[TT][E] = x;
+  }
+  x = quad_bcast<K>(x);
+#pragma unroll
+  for (int t = TT; t < 8; ++t) {  // groups that can hold a column > K
+    const Quad4<T> u = U[K][q + 4 * t];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) a[t][e] = fma(-x, u[e], a[t][e]);
+  }
+  // keep each step's LDS reads next to their use (hoisting them across steps only raises
+  // register pressure)
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <typename T, int... Ks>
+__device__ __forceinline__ void rlt_steps(std::integer_sequence<int, Ks...>, Quad4<T> (&a)[8],
+                                          const Quad4<T> (*U)[NB / 4], const T* rinv, int q) {
+  (rlt_step<Ks>(a, U, rinv, q), ...);
+}
+
 template <typename T, bool TL>
 __global__ __launch_bounds__(256) void trsm_rlt_kernel(int nb, int64_t nrows,
                                                        const T* __restrict__ L, int64_t ldl,
                                                        T* __restrict__ B, int64_t ldb) {
-  __shared__ T Ls[NB * (NB + 1) / 2];
-  constexpr int SL = 65;   // padded row of the transposed slab
-  __shared__ T S[NB * SL];  // transposed slab: S[c*SL + r]
+  __shared__ Quad4<T> U[NB][NB / 4];  // strictly upper part of U, zero beyond nb
+  __shared__ T rinv[NB];  // 1 / u_kk (1 beyond nb)
   const int tid = threadIdx.x;
-  const int64_t r0 = (int64_t)blockIdx.x * 64;
-  const int rh = (int)min<int64_t>(64, nrows - r0);
-  const int npk = nb * (nb + 1) / 2;
-  for (int p = tid; p < npk; p += 256) {  // flat packed index: independent loads
-    int r = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
-    while (r * (r + 1) / 2 > p) --r;
-    while ((r + 1) * (r + 2) / 2 <= p) ++r;
-    const int c = p - r * (r + 1) / 2;
-    Ls[p] = TL ? L[(int64_t)c * ldl + r] : L[(int64_t)r * ldl + c];
-  }
-  for (int idx = tid; idx < 64 * nb; idx += 256) {
-    const int r = idx / nb, c = idx % nb;
-    S[c * SL + r] = (r < rh) ? B[(r0 + r) * ldb + c] : static_cast<T>(0);
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int r = e / NB, c = e % NB;
+    T x = T(0);
+    if (r < nb && c < nb && c >= r) x = TL ? L[(int64_t)r * ldl + c] : L[(int64_t)c * ldl + r];
+    reinterpret_cast<T*>(&U[0][0])[r * NB + c] = c > r ? x : T(0);
+    if (c == r) rinv[r] = r < nb ? T(1) / x : T(1);
   }
   __syncthreads();
-  const int rr = tid & 63, g = tid >> 6;
-  if (g == 0) S[rr] = S[rr] / Ls[0];
-  __syncthreads();
-  // column c of the slab (c = g + 4k) is owned by group g; each step loads a batch of
-  // owned columns before storing them (one LDS round trip per batch)
-  for (int i = 0; i < nb - 1; ++i) {
-    const T x = S[i * SL + rr];
-    constexpr int CH = 8;
-    for (int k0 = 0; k0 < NB / 4; k0 += CH) {
-      T v[CH], l[CH];
+  const int q = tid & 3;
+  const int64_t row = (int64_t)blockIdx.x * 64 + (tid >> 2);
+  const bool own = row < nrows;
+  T* br = B + (own ? row : nrows - 1) * ldb;
+  Quad4<T> a[8];
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
-        const int c = g + 4 * (k0 + k);
-        v[k] = S[c * SL + rr];
-        l[k] = (c > i && c < nb) ? Ls[c * (c + 1) / 2 + i] : T(0);
-      }
+  for (int t = 0; t < 8; ++t) {
+    const int c = 4 * (q + 4 * t);
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
-        const int c = g + 4 * (k0 + k);
-        if (c > i && c < nb) {
-          T w = fma(-l[k], x, v[k]);
-          if (c == i + 1) w = w / Ls[c * (c + 1) / 2 + c];
-          S[c * SL + rr] = w;
-        }
-      }
+    for (int e = 0; e < 4; ++e) a[t][e] = c + e < nb ? br[c + e] : T(0);
+  }
+  rlt_steps(std::make_integer_sequence<int, NB>{}, a, U, rinv, q);
+  if (own) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int c = 4 * (q + 4 * t);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (c + e < nb) br[c + e] = a[t][e];
     }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < 64 * nb; idx += 256) {
-    const int r = idx / nb, c = idx % nb;
-    if (r < rh) B[(r0 + r) * ldb + c] = S[c * SL + r];
   }
 }
 
@@ -226,16 +196,11 @@ __global__ __launch_bounds__(256) void transpose_kernel(int64_t n, double* __res
 
 template <typename T>
 int launch_potf2_ex(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s) {
-  if (nb <= 0) return 0;
   if (nb > NB) {
     set_error("potf2 block %d > %d", nb, NB);
     return DDLA_ERR_INVALID_ARG;
   }
-  ProfScope prof(sizeof(T) == 8 ? "potf2_f64" : "potf2_f32", (double)nb * nb * nb / 3.0,
-                 sizeof(T) * (double)nb * nb, s);
-  hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, nb, A, lda, info, info_base);
-  DDLA_HIP_CHECK(hipGetLastError());
-  return 0;
+  return launch_potf2_block<T>(nb, A, lda, info, info_base, s);
 }
 
 template <typename T>

@@ -16,7 +16,7 @@
 //     panel's L11 \ U11.  The root moves the displaced top rows into the places the selected
 //     rows leave, writes L11 \ U11 on top and emits the (dst <- src) row list that the rest of
 //     the matrix receives (same list format as the partial-pivoting panel);
-//   * then L21 = A21 U11^-1 for the rows below (one parallel triangular solve).
+//   * then L21 = A21 U11^-1 for the rows below (the panel TRSM of the Cholesky kernels).
 // Pivot growth stays close to partial pivoting's in practice (the paper's experiments); the
 // refinement recovers FP64 accuracy either way, and a zero pivot in U11 reports info.
 //
@@ -29,7 +29,6 @@
 
 #include <algorithm>
 #include <climits>
-#include <utility>
 
 namespace ddla {
 
@@ -401,88 +400,6 @@ __global__ __launch_bounds__(TS_ROWS) void tslu_tree_kernel(TsluArgs a) {
   }
 }
 
-// L21 = A21 U11^-1 for the rows below the top block, U11 = upper triangle of the top block.
-// Four lanes per row (a DPP quad); lane q owns the column groups 4 (q + 4 t) .. + 3, t < 8,
-// in registers.  Column k is finished by its owner (x = a_k / u_kk as a multiply by the
-// reciprocal), broadcast to the quad, and every lane subtracts x U11[k][c] from its columns
-// c > k (U11 in LDS, strictly upper part; one 16-byte broadcast read per group).
-template <int K>
-__device__ __forceinline__ float quad_bcast(float x) {
-  constexpr int o = (K >> 2) & 3;  // owner lane of column K within the quad
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x),
-                                                            o | (o << 2) | (o << 4) | (o << 6),
-                                                            0xf, 0xf, false));
-}
-
-template <int K>
-__device__ __forceinline__ void l21_step(float4 (&a)[8], const float4 (*U)[TS_NB / 4],
-                                         const float* rinv, int q) {
-  constexpr int T = K / 16, E = K % 4;
-  float x = 0.f;
-  if (q == ((K >> 2) & 3)) {
-    float& ak = E == 0 ? a[T].x : E == 1 ? a[T].y : E == 2 ? a[T].z : a[T].w;
-    x = ak * rinv[K];
-    ak = x;
-  }
-  x = quad_bcast<K>(x);
-#pragma unroll
-  for (int t = T; t < 8; ++t) {  // groups that can hold a column > K
-    const float4 u = U[K][q + 4 * t];
-    a[t].x = fmaf(-x, u.x, a[t].x);
-    a[t].y = fmaf(-x, u.y, a[t].y);
-    a[t].z = fmaf(-x, u.z, a[t].z);
-    a[t].w = fmaf(-x, u.w, a[t].w);
-  }
-}
-
-template <int... Ks>
-__device__ __forceinline__ void l21_steps(std::integer_sequence<int, Ks...>, float4 (&a)[8],
-                                          const float4 (*U)[TS_NB / 4], const float* rinv,
-                                          int q) {
-  (l21_step<Ks>(a, U, rinv, q), ...);
-}
-
-__global__ __launch_bounds__(256) void tslu_l21_kernel(int rows, float* __restrict__ B,
-                                                       int64_t ld, const float* __restrict__ Ut,
-                                                       bool vec) {
-  __shared__ float4 U[TS_NB][TS_NB / 4];  // strictly upper part of U11
-  __shared__ float rinv[TS_NB];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < TS_NB * TS_NB; e += 256) {
-    const int r = e / TS_NB, c = e % TS_NB;
-    const float x = Ut[(int64_t)r * ld + c];
-    reinterpret_cast<float*>(&U[r][0])[c] = c > r ? x : 0.f;
-    if (c == r) rinv[r] = x != 0.f ? 1.f / x : 0.f;
-  }
-  __syncthreads();
-  const int q = tid & 3;
-  const int64_t row = (int64_t)blockIdx.x * 64 + (tid >> 2);
-  const bool own = row < rows;
-  float* br = B + (own ? row : (int64_t)rows - 1) * ld;
-  float4 a[8];
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const int c = 4 * (q + 4 * t);
-    if (vec) a[t] = *reinterpret_cast<const float4*>(br + c);
-    else a[t] = make_float4(br[c], br[c + 1], br[c + 2], br[c + 3]);
-  }
-  l21_steps(std::make_integer_sequence<int, TS_NB>{}, a, U, rinv, q);
-  if (own) {
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int c = 4 * (q + 4 * t);
-      if (vec) {
-        *reinterpret_cast<float4*>(br + c) = a[t];
-      } else {
-        br[c] = a[t].x;
-        br[c + 1] = a[t].y;
-        br[c + 2] = a[t].z;
-        br[c + 3] = a[t].w;
-      }
-    }
-  }
-}
-
 unsigned g_tag = 0;
 void* g_zeroed = nullptr;
 size_t g_zeroed_bytes = 0;
@@ -528,15 +445,8 @@ int launch_tslu_panel(int64_t m, int nb, float* P, int64_t ld, int32_t* info, in
     hipLaunchKernelGGL(tslu_tree_kernel, dim3(leaves), dim3(TS_ROWS), 0, s, a);
     DDLA_HIP_CHECK(hipGetLastError());
   }
-  if (m > TS_NB) {  // L21 = A21 U11^-1 below the top block
-    const int64_t rows = m - TS_NB;
-    const bool vec = ld % 4 == 0 && (reinterpret_cast<uintptr_t>(P) & 15) == 0;
-    ProfScope prof("tslu_l21_f32", (double)rows * TS_NB * TS_NB,
-                   8.0 * rows * TS_NB + 4.0 * TS_NB * TS_NB, s);
-    hipLaunchKernelGGL(tslu_l21_kernel, dim3((unsigned)ceil_div(rows, 64)), dim3(256), 0, s,
-                       (int)rows, P + TS_NB * ld, ld, P, vec);
-    DDLA_HIP_CHECK(hipGetLastError());
-  }
+  // L21 = A21 U11^-1 below the top block
+  DDLA_RETURN_IF(launch_trsm_rlt_panel<float>(TS_NB, m - TS_NB, P, ld, P + TS_NB * ld, ld, s, true));
   return 0;
 }
 
