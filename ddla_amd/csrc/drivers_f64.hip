@@ -149,27 +149,54 @@ int dgetrs_nopiv_internal(bool trans, int64_t n, int64_t nrhs, const double* a, 
   return 0;
 }
 
+// Blocked lower Cholesky (CholeskyLowerBlock, Cholesky.cs:142-160) with a look-ahead of one
+// block column: after L21 of step k, the main stream updates only the next block column and
+// goes on to its diagonal block and L21, while the auxiliary stream updates the rest of the
+// trailing triangle.  E0(k) = L21 of step k ready; E2(k) = far update of step k done, awaited
+// before the main stream touches columns that update also covers.  Each element takes the
+// same products as without look-ahead (the split only decides which launch computes it).
 int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host) {
   hipStream_t s = stream();
+  hipStream_t s2;
+  hipEvent_t* ev;
+  DDLA_RETURN_IF(aux_stream(&s2, &ev));
   int32_t* dinfo;
   int *counts, *lists;
   DDLA_RETURN_IF(lu_bookkeeping(1, &dinfo, &counts, &lists));
   DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
   if (!lower) DDLA_RETURN_IF(launch_dtranspose_inplace(n, a, lda, s));
+  DDLA_HIP_CHECK(hipEventRecord(ev[0], s));
+  DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
+  bool pending = false;
   for (int64_t j = 0; j < n; j += NBLK) {
     const int b = (int)std::min<int64_t>(NBLK, n - j);
     double* a11 = a + j * lda + j;
     DDLA_RETURN_IF(launch_potf2_ex<double>(b, a11, lda, dinfo, (int)j, s));
     const int64_t rest = n - j - b;
-    if (rest > 0) {
-      double* a21 = a11 + b * lda;
-      // A21 = A21 * L11^-T (Cholesky.cs:156-157)
-      DDLA_RETURN_IF(launch_trsm_rlt_panel<double>(b, rest, a11, lda, a21, lda, s, false));
-      // lower(A22) -= A21 * A21^T (Cholesky.cs:158)
-      DDLA_RETURN_IF(launch_dgemm(rest, rest, b, -1.0, a21, lda, true, a21, lda, false, 1.0,
-                                  a21 + b, lda, 1, s));
+    if (rest <= 0) break;
+    double* a21 = a11 + b * lda;
+    // A21 = A21 * L11^-T (Cholesky.cs:156-157)
+    DDLA_RETURN_IF(launch_trsm_rlt_panel<double>(b, rest, a11, lda, a21, lda, s, false));
+    // lower(A22) -= A21 * A21^T (Cholesky.cs:158): the next block column here, the rest on
+    // the auxiliary stream
+    const int64_t nxt = std::min<int64_t>(NBLK, rest), far = rest - nxt;
+    if (far > 0) DDLA_HIP_CHECK(hipEventRecord(ev[0], s));  // E0(k)
+    if (pending) {
+      DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));  // E2(k-1)
+      pending = false;
+    }
+    DDLA_RETURN_IF(launch_dgemm(rest, nxt, b, -1.0, a21, lda, true, a21, lda, false, 1.0, a21 + b,
+                                lda, 1, s));
+    if (far > 0) {
+      DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
+      double* af = a21 + nxt * lda;
+      DDLA_RETURN_IF(launch_dgemm(far, far, b, -1.0, af, lda, true, af, lda, false, 1.0,
+                                  af + b + nxt, lda, 1, s2));
+      DDLA_HIP_CHECK(hipEventRecord(ev[1], s2));  // E2(k)
+      pending = true;
     }
   }
+  if (pending) DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));
   if (!lower) DDLA_RETURN_IF(launch_dtranspose_inplace(n, a, lda, s));
   return read_info(dinfo, info_host);
 }

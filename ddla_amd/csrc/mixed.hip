@@ -773,32 +773,57 @@ int sgetrf_nopiv_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
   return 0;
 }
 
-// FP32 lower Cholesky; trailing SYRK in prec.
+// FP32 lower Cholesky; trailing SYRK in prec, with the look-ahead of dpotrf_internal: the
+// next block column is updated on the main stream, the rest of the trailing triangle on the
+// auxiliary stream (its own split workspace and scales), in the shadow of the next diagonal
+// block and L21.
 int spotrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
   hipStream_t s = stream();
+  hipStream_t s2;
+  hipEvent_t* ev;
+  DDLA_RETURN_IF(aux_stream(&s2, &ev));
   int32_t* dinfo;
   int *counts, *lists;
   DDLA_RETURN_IF(lu_bookkeeping(1, &dinfo, &counts, &lists));
   DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
   float* a = w.W;
   const int64_t lda = n;
+  const int64_t nb = (n + NBLK - 1) / NBLK + 1;
+  float* scales_aux = w.scales + 2 * nb;
+  auto syrk = [&](int64_t m, int64_t nc, int b, const float* A, float* C, float* sc,
+                  hipStream_t st, int slot) -> int {
+    if (prec == DDLA_PREC_FP32)
+      return launch_sgemm(m, nc, b, -1.f, A, lda, true, A, lda, false, 1.f, C, lda, 1, st);
+    return launch_hgemm(prec == DDLA_PREC_BF16, w.terms, true, 1, m, nc, b, -1.f, A, lda, A, lda,
+                        C, lda, sc, st, slot);
+  };
+  DDLA_HIP_CHECK(hipEventRecord(ev[0], s));
+  DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
+  bool pending = false;
   for (int64_t j = 0; j < n; j += NBLK) {
     const int b = (int)std::min<int64_t>(NBLK, n - j);
     float* a11 = a + j * lda + j;
     DDLA_RETURN_IF(launch_potf2_ex<float>(b, a11, lda, dinfo, (int)j, s));
     const int64_t rest = n - j - b;
-    if (rest > 0) {
-      float* a21 = a11 + b * lda;
-      DDLA_RETURN_IF(launch_trsm_rlt_panel<float>(b, rest, a11, lda, a21, lda, s, false));
-      if (prec == DDLA_PREC_FP32) {
-        DDLA_RETURN_IF(launch_sgemm(rest, rest, b, -1.f, a21, lda, true, a21, lda, false, 1.f,
-                                    a21 + b, lda, 1, s));
-      } else {
-        DDLA_RETURN_IF(launch_hgemm(prec == DDLA_PREC_BF16, w.terms, true, 1, rest, rest, b, -1.f, a21, lda,
-                                    a21, lda, a21 + b, lda, w.scales, s));
-      }
+    if (rest <= 0) break;
+    float* a21 = a11 + b * lda;
+    DDLA_RETURN_IF(launch_trsm_rlt_panel<float>(b, rest, a11, lda, a21, lda, s, false));
+    const int64_t nxt = std::min<int64_t>(NBLK, rest), far = rest - nxt;
+    if (far > 0) DDLA_HIP_CHECK(hipEventRecord(ev[0], s));  // E0(k)
+    if (pending) {
+      DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));  // E2(k-1)
+      pending = false;
+    }
+    DDLA_RETURN_IF(syrk(rest, nxt, b, a21, a21 + b, w.scales, s, WS_SPLIT));
+    if (far > 0) {
+      DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
+      float* af = a21 + nxt * lda;
+      DDLA_RETURN_IF(syrk(far, far, b, af, af + b + nxt, scales_aux, s2, WS_SPLIT2));
+      DDLA_HIP_CHECK(hipEventRecord(ev[1], s2));  // E2(k)
+      pending = true;
     }
   }
+  if (pending) DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));
   DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DDLA_HIP_CHECK(hipStreamSynchronize(s));
   return 0;
@@ -848,16 +873,12 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
                          w.D);
     DDLA_HIP_CHECK(hipGetLastError());
     const bool lu = kind != CHOLESKY;
-    if (nrhs <= 8) {
-      // persistent triangular solves with the inverted diagonal blocks (trsv.hip)
-      DDLA_RETURN_IF(trsv_solve<float>(true, false, np, (int)nrhs, w.W, np, w.dinv1, w.D, nrhs, s));
-      DDLA_RETURN_IF(trsv_solve<float>(false, !lu, np, (int)nrhs, w.W, np, w.dinv2, w.D, nrhs, s));
-    } else if (lu) {
-      DDLA_RETURN_IF(trsm_left<float>(true, true, false, np, nrhs, w.W, np, w.D, nrhs, s));
-      DDLA_RETURN_IF(trsm_left<float>(false, false, false, np, nrhs, w.W, np, w.D, nrhs, s));
-    } else {
-      DDLA_RETURN_IF(trsm_left<float>(true, false, false, np, nrhs, w.W, np, w.D, nrhs, s));
-      DDLA_RETURN_IF(trsm_left<float>(true, false, true, np, nrhs, w.W, np, w.D, nrhs, s));
+    // persistent triangular solves with the inverted diagonal blocks (trsv.hip), up to 8
+    // right-hand sides per launch
+    for (int64_t c0 = 0; c0 < nrhs; c0 += 8) {
+      const int nc = (int)std::min<int64_t>(8, nrhs - c0);
+      DDLA_RETURN_IF(trsv_solve<float>(true, false, np, nc, w.W, np, w.dinv1, w.D + c0, nrhs, s));
+      DDLA_RETURN_IF(trsv_solve<float>(false, !lu, np, nc, w.W, np, w.dinv2, w.D + c0, nrhs, s));
     }
     // X += D (FP64, through V for the butterfly form); R = B - A X (FP64)
     if (kind == LU_BUTTERFLY)
@@ -966,7 +987,7 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
         }
       }
     }
-    if (finfo == 0 && nrhs <= 8) {
+    if (finfo == 0) {
       // inverted diagonal blocks of the two triangular factors for the refinement solves
       if (kind != CHOLESKY) {  // w.np: the butterfly's padded order, or n after the fallback
         DDLA_RETURN_IF(trsv_prepare<float>(true, true, false, w.np, w.W, w.np, w.dinv1, s));

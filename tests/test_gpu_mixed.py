@@ -112,3 +112,19 @@ def test_gesv_mpir_rejects_singular(ddla):
     A[:, 100] = 0.0
     info, _, _ = run(ddla, ddla.gesv_mpir, A, 1, 2)
     assert info.status == 101
+
+
+@pytest.mark.parametrize("fn", ["gesv", "posv"])
+def test_mpir_many_rhs(ddla, oracle, fn):
+    # more right-hand sides than one persistent solve takes (chunks of 8, 8 and 4)
+    n, nrhs, kappa = 1024, 20, 1e4
+    spd = fn == "posv"
+    A = conditioned(n, kappa, 41, spd=spd)
+    if spd:
+        A = (A + A.T) / 2
+    info, X, B = run(ddla, ddla.gesv_mpir if fn == "gesv" else ddla.posv_mpir, A, nrhs, 2)
+    assert info.status == 0 and info.backward_error <= 1e-14
+    for j in range(nrhs):
+        r = B[:, j] - A @ X[:, j]
+        be = np.linalg.norm(r, np.inf) / (np.linalg.norm(A, np.inf) * np.linalg.norm(X[:, j], np.inf))
+        assert be <= 2e-14, j
