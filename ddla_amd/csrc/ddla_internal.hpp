@@ -186,6 +186,9 @@ int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64
 template <typename T>
 int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t lda, const T* Dinv,
                T* B, int64_t ldb, hipStream_t s);
+// the same solve for many right-hand sides (FP32): GEMMs on the prepared diagonal inverses
+int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float* A, int64_t lda,
+                       const float* Dinv, float* B, int64_t ldb, hipStream_t s);
 
 // Cholesky pieces (chol_kernels.hip)
 template <typename T>

@@ -873,12 +873,14 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
                          w.D);
     DDLA_HIP_CHECK(hipGetLastError());
     const bool lu = kind != CHOLESKY;
-    // persistent triangular solves with the inverted diagonal blocks (trsv.hip), up to 8
-    // right-hand sides per launch
-    for (int64_t c0 = 0; c0 < nrhs; c0 += 8) {
-      const int nc = (int)std::min<int64_t>(8, nrhs - c0);
-      DDLA_RETURN_IF(trsv_solve<float>(true, false, np, nc, w.W, np, w.dinv1, w.D + c0, nrhs, s));
-      DDLA_RETURN_IF(trsv_solve<float>(false, !lu, np, nc, w.W, np, w.dinv2, w.D + c0, nrhs, s));
+    // triangular solves with the inverted diagonal blocks (trsv.hip): the persistent
+    // latency-bound solve for up to 8 right-hand sides, GEMMs on the same blocks beyond
+    if (nrhs <= 8) {
+      DDLA_RETURN_IF(trsv_solve<float>(true, false, np, (int)nrhs, w.W, np, w.dinv1, w.D, nrhs, s));
+      DDLA_RETURN_IF(trsv_solve<float>(false, !lu, np, (int)nrhs, w.W, np, w.dinv2, w.D, nrhs, s));
+    } else {
+      DDLA_RETURN_IF(trsv_solve_blocked(true, false, np, (int)nrhs, w.W, np, w.dinv1, w.D, nrhs, s));
+      DDLA_RETURN_IF(trsv_solve_blocked(false, !lu, np, (int)nrhs, w.W, np, w.dinv2, w.D, nrhs, s));
     }
     // X += D (FP64, through V for the butterfly form); R = B - A X (FP64)
     if (kind == LU_BUTTERFLY)

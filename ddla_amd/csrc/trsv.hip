@@ -576,7 +576,38 @@ int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64
   return 0;
 }
 
-// inverted 64-blocks + Dfull + M (sized for the larger FP32 solve block)
+// op(T) X = B for many right-hand sides (FP32), level-3 form on the prepared blocks: per
+// block row in dependency order, X_kb = Dfull_kb B_kb (a 128 x 128 x nrhs product), then the
+// block rows still to come take B -= op(T)[later, kb] X_kb (one GEMM).  Same (lower, trans)
+// convention and prepared data as trsv_solve.
+int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float* A, int64_t lda,
+                       const float* Dinv, float* B, int64_t ldb, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0) return 0;
+  constexpr int BS = solve_bs<float>();
+  const int64_t nblk = blocks_of(n, BS);
+  const float* Dfull = Dinv + inv64_elems(n);
+  void* t = nullptr;
+  DDLA_RETURN_IF(workspace(WS_TEMP2, (size_t)BS * nrhs * sizeof(float), &t));
+  float* X = static_cast<float*>(t);
+  for (int64_t jb = 0; jb < nblk; ++jb) {
+    const int64_t kb = lower ? jb : nblk - 1 - jb;
+    const int64_t r0 = kb * BS;
+    const int rows = (int)std::min<int64_t>(BS, n - r0);
+    DDLA_RETURN_IF(launch_sgemm(rows, nrhs, rows, 1.f, Dfull + kb * BS * BS, BS, true,
+                                B + r0 * ldb, ldb, true, 0.f, X, nrhs, 0, s));
+    DDLA_HIP_CHECK(hipMemcpy2DAsync(B + r0 * ldb, ldb * sizeof(float), X, nrhs * sizeof(float),
+                                    nrhs * sizeof(float), rows, hipMemcpyDeviceToDevice, s));
+    // op(T)[r][c] is A[r lda + c], or A[c lda + r] when trans
+    const int64_t lo = lower ? r0 + rows : 0, m = lower ? n - r0 - rows : r0;
+    if (m <= 0) continue;
+    const float* M = trans ? A + r0 * lda + lo : A + lo * lda + r0;
+    DDLA_RETURN_IF(launch_sgemm(m, nrhs, rows, -1.f, M, lda, !trans, X, nrhs, true, 1.f,
+                                B + lo * ldb, ldb, 0, s));
+  }
+  return 0;
+}
+
+// inverted 64-blocks + Dfull + M + N (sized for the larger FP32 solve block)
 size_t trsv_dinv_elems(int64_t n) {
   return (size_t)(inv64_elems(n) + 3 * blocks_elems<128>(n));
 }
