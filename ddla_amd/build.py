@@ -37,6 +37,8 @@ CFLAGS = [
 # elements stay independent registers.
 FILE_FLAGS = {
     "lu_kernels.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
+    # same for the Cholesky diagonal-block kernel's per-lane row of the 32-column block
+    "chol_kernels.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
     # the batched kernels unroll every column step of a panel (register-indexed rows), which
     # exceeds the default size limit for fully unrolled loops
     "batched.hip": ["-mllvm", "-pragma-unroll-threshold=200000"],

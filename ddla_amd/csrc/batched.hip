@@ -325,23 +325,6 @@ const void* batch_kernel_for(int prec, int nb) {
   return nb == 16 ? batch_fn<float, 16>() : batch_fn<float, 32>();
 }
 
-// The diagonal block of the blocked Cholesky (CholeskyLowerUnblock on it,
-// DDLA/Factorizations/Cholesky.cs:127-140), factored by one workgroup with the batched
-// kernels' blocked in-register scheme; *info = info_base + first bad pivot + 1 (if 0).
-template <typename T>
-__global__ __launch_bounds__(NT) void potf2_block_kernel(int nb, T* __restrict__ A, int64_t lda,
-                                                         int32_t* __restrict__ info,
-                                                         int info_base) {
-  __shared__ Smem<T> sm;
-  __shared__ int32_t local;
-  if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-  if (threadIdx.x == 0) local = 0;
-  __syncthreads();
-  chol_problem<T, 16>(nb, A, lda, &local, sm);
-  __syncthreads();
-  if (threadIdx.x == 0 && local != 0) atomicCAS(info, 0, info_base + local);
-}
-
 // relative cost of one problem (used only to order and balance the schedule)
 double problem_cost(int n) { return (double)n * 8.0 + (double)n * n * n / 1024.0; }
 
@@ -630,21 +613,5 @@ int batch_describe(const BatchPlan* p, char* buf, size_t len) {
            (long long)p->count, p->nb, p->wg_per_cu, p->G, p->flops, p->bytes, t.c_str());
   return 0;
 }
-
-template <typename T>
-int launch_potf2_block(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s) {
-  if (nb <= 0) return 0;
-  if (nb > NT) {
-    set_error("potf2 block %d > %d", nb, NT);
-    return DDLA_ERR_INVALID_ARG;
-  }
-  ProfScope prof(sizeof(T) == 8 ? "potf2_f64" : "potf2_f32", (double)nb * nb * nb / 3.0,
-                 sizeof(T) * (double)nb * nb, s);
-  hipLaunchKernelGGL(potf2_block_kernel<T>, dim3(1), dim3(NT), 0, s, nb, A, lda, info, info_base);
-  DDLA_HIP_CHECK(hipGetLastError());
-  return 0;
-}
-template int launch_potf2_block<double>(int, double*, int64_t, int32_t*, int, hipStream_t);
-template int launch_potf2_block<float>(int, float*, int64_t, int32_t*, int, hipStream_t);
 
 }  // namespace ddla

@@ -1,8 +1,8 @@
 // Kernels of the blocked Cholesky (lower).  Reference: CholeskyLowerBlock /
 // CholeskyLowerUnblock / Sqrt (DDLA/Factorizations/Cholesky.cs:127-160,228-236).
 //
-//   potf2             the nb x nb diagonal block: one workgroup, the batched kernels' blocked
-//                     in-register scheme (batched.hip)
+//   potf2_rec_kernel  the nb x nb diagonal block in LDS, in 32-column blocks: the diagonal
+//                     32 x 32 in one wave's registers, substitution and rank-32 update by all
 //   trsm_rlt_kernel   A21 <- A21 * L11^-T (or * U11^-1), four lanes per row of A21 with the
 //                     row in registers, the triangle in LDS
 //   transpose_kernel  in-place transpose of a square matrix (the upper factorization is the
@@ -74,6 +74,133 @@ __global__ __launch_bounds__(256) void getf2_nopiv_kernel(int nb, T* __restrict_
   for (int idx = tid; idx < nb * nb; idx += 256) {
     const int r = idx / nb, c = idx % nb;
     A[(int64_t)r * lda + c] = t[r * LD + c];
+  }
+}
+
+// Lower Cholesky of one nb x nb diagonal block (nb <= 128) in LDS, blocked by 32 columns
+// (CholeskyLowerUnblock, Cholesky.cs:127-140, reorganised; every element takes the same fmas in
+// the same column order, and the same scaling by the reciprocal of the pivot, as the unblocked
+// loop):
+//   (a) the 32 x 32 diagonal block, in wave 0's registers (lane r = row r), pivot columns
+//       broadcast with v_readlane: no barrier per column;
+//   (b) the rows below it: row-wise substitution against the factored block (one thread per
+//       row, the block in LDS);
+//   (c) the trailing lower triangle takes the block's rank-32 update (one element per thread
+//       at a time, fma over the block's columns in order).
+// *info = info_base + first non-positive pivot + 1 (if *info was 0); the block is then left as
+// it stands.
+template <typename T>
+__device__ __forceinline__ T rl(T x, int lane) {
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void potf2_rec_kernel(int nb, T* __restrict__ A, int64_t lda,
+                                                        int32_t* __restrict__ info, int info_base) {
+  constexpr int CB = 32;
+  __shared__ T S[NB][NB + 1];
+  __shared__ T piv_s[NB], piv_r[NB];  // sqrt(a_kk) and its reciprocal
+  __shared__ int stop;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  for (int e = tid; e < nb * nb; e += 256) {
+    const int r = e / nb, c = e % nb;
+    if (c <= r) S[r][c] = A[(int64_t)r * lda + c];
+  }
+  if (tid == 0) stop = 0;
+  __syncthreads();
+  for (int j0 = 0; j0 < nb; j0 += CB) {
+    const int bs = min(CB, nb - j0);
+    // (a) diagonal block in wave 0
+    if (wave == 0) {
+      T v[CB];
+#pragma unroll
+      for (int c = 0; c < CB; ++c) v[c] = (lane < bs && c <= lane && c < bs) ? S[j0 + lane][j0 + c] : T(0);
+      int st = 0;
+#pragma unroll
+      for (int k = 0; k < CB; ++k) {
+        if (k < bs && st == 0) {  // uniform
+          const T d = rl(v[k], k);
+          if (!(d > T(0))) {
+            st = j0 + k + 1;
+          } else {
+            const T sq = sqrt(d);
+            const T rinv = T(1) / sq;
+            if (lane == k) v[k] = sq;
+            else if (lane > k && sq != T(1)) v[k] *= rinv;
+            if (lane == 0) {
+              piv_s[j0 + k] = sq;
+              piv_r[j0 + k] = rinv;
+            }
+            const T xr = -v[k];
+#pragma unroll
+            for (int c = k + 1; c < CB; ++c) {
+              const T lck = rl(v[k], c);  // row c's multiplier for column k
+              if (lane >= c && lane < bs) v[c] = fma(lck, xr, v[c]);
+            }
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if (st == 0) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+          if (lane < bs && c <= lane) S[j0 + lane][j0 + c] = v[c];
+      } else if (lane == 0) {
+        stop = st;
+      }
+    }
+    __syncthreads();
+    if (stop) break;  // uniform
+    const int i0 = j0 + CB, m2 = nb - i0;  // rows below the block
+    if (m2 <= 0) break;
+    // (b) rows below: x_k = a_k / l_kk (reciprocal), then x_c -= l_ck x_k for c > k
+    if (tid < m2) {
+      const int i = i0 + tid;
+      T x[CB];
+#pragma unroll
+      for (int c = 0; c < CB; ++c) x[c] = S[i][j0 + c];
+#pragma unroll
+      for (int k = 0; k < CB; ++k) {
+        x[k] *= piv_r[j0 + k];  // exact when l_kk = 1 (reciprocal 1)
+        const T xr = -x[k];
+#pragma unroll
+        for (int c = k + 1; c < CB; ++c) x[c] = fma(S[j0 + c][j0 + k], xr, x[c]);
+        // keep each step's LDS reads next to their use (the compiler would otherwise issue
+        // all 496 of them up front)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int c = 0; c < CB; ++c) S[i][j0 + c] = x[c];
+    }
+    __syncthreads();
+    // (c) trailing lower triangle: a_ic -= sum_p l_ip l_cp, p over the block in order
+    for (int e = tid; e < m2 * m2; e += 256) {
+      const int ii = e / m2, cc = e % m2;
+      if (cc > ii) continue;
+      const int i = i0 + ii, c = i0 + cc;
+      T x = S[i][c];
+#pragma unroll
+      for (int p = 0; p < CB; ++p) x = fma(S[c][j0 + p], -S[i][j0 + p], x);
+      S[i][c] = x;
+    }
+    __syncthreads();
+  }
+  if (stop) {
+    if (tid == 0) atomicCAS(info, 0, info_base + stop);
+    return;
+  }
+  for (int e = tid; e < nb * nb; e += 256) {
+    const int r = e / nb, c = e % nb;
+    if (c <= r) A[(int64_t)r * lda + c] = S[r][c];
   }
 }
 
@@ -200,7 +327,12 @@ int launch_potf2_ex(int nb, T* A, int64_t lda, int32_t* info, int info_base, hip
     set_error("potf2 block %d > %d", nb, NB);
     return DDLA_ERR_INVALID_ARG;
   }
-  return launch_potf2_block<T>(nb, A, lda, info, info_base, s);
+  if (nb <= 0) return 0;
+  ProfScope prof(sizeof(T) == 8 ? "potf2_f64" : "potf2_f32", (double)nb * nb * nb / 3.0,
+                 sizeof(T) * (double)nb * nb, s);
+  hipLaunchKernelGGL(potf2_rec_kernel<T>, dim3(1), dim3(256), 0, s, nb, A, lda, info, info_base);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 template <typename T>

@@ -193,9 +193,6 @@ int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float*
 // Cholesky pieces (chol_kernels.hip)
 template <typename T>
 int launch_potf2_ex(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s);
-// the same diagonal-block factorization with the batched kernels' scheme (batched.hip)
-template <typename T>
-int launch_potf2_block(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s);
 // B <- B L^-T for lower L (l_transposed: B <- B U^-1 for the upper U stored at L)
 template <typename T>
 int launch_trsm_rlt_panel(int nb, int64_t nrows, const T* L, int64_t ldl, T* B, int64_t ldb,

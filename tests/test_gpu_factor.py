@@ -216,3 +216,18 @@ def test_chol_class_mirror(ddla):
     b = rng.uniform(-1, 1, n)
     x = ddla.Cholesky(S).Solve(b)
     assert backward_error(S, x, b) < 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 64, 100, 128])
+def test_cholesky_single_block_matches_unblocked_exactly(ddla, oracle, n):
+    # one diagonal block: the blocked-in-LDS kernel performs the unblocked loop's operations
+    # (same fmas in the same column order, scaling by the reciprocal), so the factor is
+    # bit-identical to the reference's CholeskyLowerUnblock restated in the oracle
+    G = oracle.uniform_matrix(n, n, 77 + n)
+    S = G @ G.T + n * np.eye(n)
+    So = S.copy()
+    assert oracle.potrf(So) == 0
+    A = S.copy()
+    ddla.CholDecompose(ddla.LOWER, A)
+    np.testing.assert_array_equal(np.tril(A), np.tril(So))
+    np.testing.assert_array_equal(np.triu(A, 1), np.triu(S, 1))
