@@ -29,9 +29,18 @@ __global__ __launch_bounds__(256) void getf2_nopiv_kernel(int nb, T* __restrict_
   constexpr int LD = NB + 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int r = idx / nb, c = idx % nb;
-    t[r * LD + c] = A[(int64_t)r * lda + c];
+  for (int b0 = 0; b0 < NB * NB / 256; b0 += 16) {  // 16 reads in flight per thread
+    T v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int idx = (b0 + i) * 256 + tid;
+      v[i] = A[idx < nb * nb ? (int64_t)(idx / nb) * lda + idx % nb : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int idx = (b0 + i) * 256 + tid;
+      if (idx < nb * nb) t[(idx / nb) * LD + idx % nb] = v[i];
+    }
   }
   __syncthreads();
   bool stop = false;
@@ -110,9 +119,21 @@ __global__ __launch_bounds__(256) void potf2_rec_kernel(int nb, T* __restrict__ 
   __shared__ int stop;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-  for (int e = tid; e < nb * nb; e += 256) {
-    const int r = e / nb, c = e % nb;
-    if (c <= r) S[r][c] = A[(int64_t)r * lda + c];
+  // fixed trip count, unrolled: a thread's reads are in flight together
+  for (int b0 = 0; b0 < NB * NB / 256; b0 += 16) {  // 16 reads in flight per thread
+    T v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = (b0 + i) * 256 + tid;
+      const int r = e / nb, c = e % nb;
+      v[i] = A[(e < nb * nb && c <= r) ? (int64_t)r * lda + c : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = (b0 + i) * 256 + tid;
+      const int r = e / nb, c = e % nb;
+      if (e < nb * nb && c <= r) S[r][c] = v[i];
+    }
   }
   if (tid == 0) stop = 0;
   __syncthreads();
@@ -265,13 +286,23 @@ __global__ __launch_bounds__(256) void trsm_rlt_kernel(int nb, int64_t nrows,
   __shared__ Quad4<T> U[NB][NB / 4];  // strictly upper part of U, zero beyond nb
   __shared__ T rinv[NB];  // 1 / u_kk (1 beyond nb)
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int r = e / NB, c = e % NB;
-    T x = T(0);
-    if (r < nb && c < nb && c >= r) x = TL ? L[(int64_t)r * ldl + c] : L[(int64_t)c * ldl + r];
-    reinterpret_cast<T*>(&U[0][0])[r * NB + c] = c > r ? x : T(0);
-    if (c == r) rinv[r] = r < nb ? T(1) / x : T(1);
+  const T dv = tid < nb ? L[(int64_t)tid * ldl + tid] : T(1);  // divided after the staging
+  // batches of 16 branch-free reads per thread in flight together
+  for (int b0 = 0; b0 < NB * NB / 256; b0 += 16) {
+    T v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = (b0 + i) * 256 + tid, r = e / NB, c = e % NB;
+      const bool in = r < nb && c < nb && c > r;
+      v[i] = L[in ? (TL ? (int64_t)r * ldl + c : (int64_t)c * ldl + r) : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = (b0 + i) * 256 + tid, r = e / NB, c = e % NB;
+      reinterpret_cast<T*>(&U[0][0])[e] = (r < nb && c < nb && c > r) ? v[i] : T(0);
+    }
   }
+  if (tid < NB) rinv[tid] = T(1) / dv;
   __syncthreads();
   const int q = tid & 3;
   const int64_t row = (int64_t)blockIdx.x * 64 + (tid >> 2);

@@ -76,7 +76,8 @@ enum WorkspaceSlot {
   WS_SPLIT2 = 10, // ... for updates on the auxiliary stream
   WS_TEMP2 = 11,  // row-interchange staging on the auxiliary stream
   WS_TSLU = 12,   // tournament-pivoting tree (candidate sets, arrival words)
-  WS_COUNT = 13,
+  WS_SPLITP = 13, // split panel operand shared by the updates of one panel (two in flight)
+  WS_COUNT = 14,
 };
 
 // Optional per-kernel timing (ddla_profile_*): while enabled, a ProfScope around a launch
@@ -141,6 +142,19 @@ int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64
                  float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
                  int64_t ldc, float* scale_ws, hipStream_t s, int ws_slot = WS_SPLIT);
 
+// the same in two parts, for an operand shared by several updates: split once, multiply many
+// times.  Low-precision parts of an operand: element (r, k) of part t at
+// parts[t plane + r Kp + k] (Kp = k rounded up to 32), one scale per 128 rows.
+struct SplitOp {
+  const void* parts;
+  int64_t plane;
+  const float* scales;
+};
+int launch_split(bool bf16, int terms, bool kcontig, int64_t rows, int64_t k, const float* src,
+                 int64_t ld, void* out, float* scales, hipStream_t s);
+int launch_hgemm_parts(bool bf16, int terms, int tri, int64_t m, int64_t n, int64_t k,
+                       float alpha, SplitOp a, SplitOp b, float* C, int64_t ldc, hipStream_t s);
+
 // general strided copy / transpose helpers
 int launch_dcopy2d(int64_t m, int64_t n, const double* src, int64_t rs, int64_t cs, double* dst,
                    int64_t ld, hipStream_t s);
@@ -156,10 +170,12 @@ int launch_dscale2d(int64_t m, int64_t n, double beta, double* c, int64_t rs, in
 template <typename T>
 int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
                           int info_base, int* list_out, int* count_out, hipStream_t s);
+// applies such a list (at most 2 * 128 entries) to columns [0, ncols) minus
+// [skip_lo, skip_hi) of the rows from row0 on
 template <typename T>
 int launch_laswp_list(T* A, int64_t ld, int64_t row0, int64_t ncols, int64_t skip_lo,
                       int64_t skip_hi, const int* list, const int* count, int max_count,
-                      hipStream_t s, int ws_slot = WS_TEMP);
+                      hipStream_t s);
 int launch_perm_compose(int n, int npanels, int nb, const int* lists, const int* counts, int* perm,
                         hipStream_t s);
 // Tournament-pivoting panel (tslu.hip, FP32): same outputs as the partial-pivoting panel

@@ -23,8 +23,8 @@
 // i.e. after every reader of step i finished, so a slot is never overwritten while read.
 // Every row carries one extra value: the panel-relative index of the row whose
 // trailing data belongs there.  At the end each workgroup emits (dst, src) pairs for
-// moved rows, so the interchanges for the rest of the matrix are one parallel gather
-// (laswp_list_kernel) instead of nb sequential swaps.
+// moved rows, so the interchanges for the rest of the matrix are one parallel gather and
+// scatter (laswp_list_kernel) instead of nb sequential swaps.
 #include "ddla_internal.hpp"
 #include "wave_ops.hpp"
 
@@ -433,25 +433,48 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
 }
 
 // Applies the (dst <- src) row list of a panel to columns [0, ncols) of A (rows relative to
-// row0), skipping columns [skip_lo, skip_hi), in two launches so that every source row is read
-// before any destination row is written: gather the source rows into tmp, then scatter tmp
-// into the destination rows.  blockIdx.y is the list entry, x runs along the row (coalesced).
-template <typename T, bool GATHER>
-__global__ __launch_bounds__(256) void laswp_list_kernel(T* __restrict__ A, int64_t ld, int64_t row0,
-                                                         int64_t ncols, int64_t skip_lo,
-                                                         int64_t skip_hi,
-                                                         const int* __restrict__ list,
-                                                         const int* __restrict__ count,
-                                                         T* __restrict__ tmp) {
-  const int e = blockIdx.y;
-  if (e >= *count) return;
+// row0), skipping columns [skip_lo, skip_hi).  The moved rows form a permutation among
+// themselves, so each workgroup owns LW columns outright: it reads every source row of its
+// columns into LDS, then writes them to their destinations, and no other workgroup touches
+// those columns.  The list is staged in LDS first; the reads of a thread (one per 64 rows
+// moved) are all in flight together.  One launch, no global staging.
+template <typename T>
+__global__ __launch_bounds__(1024) void laswp_list_kernel(T* __restrict__ A, int64_t ld,
+                                                          int64_t row0, int64_t ncols,
+                                                          int64_t skip_lo, int64_t skip_hi,
+                                                          const int* __restrict__ list,
+                                                          const int* __restrict__ count) {
+  constexpr int LW = 256 / sizeof(T);  // columns per workgroup: 256-byte row segments
+  constexpr int RP = 1024 / LW;        // rows per pass
+  constexpr int MAXE = 2 * NB_MAX;     // moved rows of one panel
+  __shared__ T buf[MAXE][LW];
+  __shared__ int lst[2 * MAXE];
+  const int tid = threadIdx.x;
+  const int cnt = min(*count, MAXE);
+  if (tid < 2 * cnt) lst[tid] = list[tid];
+  __syncthreads();
   const int64_t width = ncols - (skip_hi - skip_lo);
-  const int64_t row = row0 + list[2 * e + (GATHER ? 1 : 0)];
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < width;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = (t < skip_lo) ? t : t + (skip_hi - skip_lo);
-    if (GATHER) tmp[(int64_t)e * width + t] = A[row * ld + c];
-    else A[row * ld + c] = tmp[(int64_t)e * width + t];
+  const int lc = tid % LW, rs = tid / LW;
+  const int64_t t = (int64_t)blockIdx.x * LW + lc;
+  const bool ok = t < width;
+  const int64_t c = t < skip_lo ? t : t + (skip_hi - skip_lo);
+  T v[MAXE / RP];
+#pragma unroll
+  for (int q = 0; q < MAXE / RP; ++q) {  // branch-free: all reads in flight together
+    const int e = rs + q * RP;
+    const bool in = ok && e < cnt;
+    v[q] = A[in ? (row0 + lst[2 * e + 1]) * ld + c : row0 * ld];
+  }
+#pragma unroll
+  for (int q = 0; q < MAXE / RP; ++q) {
+    const int e = rs + q * RP;
+    if (ok && e < cnt) buf[e][lc] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXE / RP; ++q) {
+    const int e = rs + q * RP;
+    if (ok && e < cnt) A[(row0 + lst[2 * e]) * ld + c] = buf[e][lc];
   }
 }
 
@@ -543,10 +566,16 @@ const void* panel_kernel() {
 template <typename T, int R>
 int launch_panel_r(int m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
                    const PanelSync& sy, unsigned base, int info_base, hipStream_t s) {
+  // The W workgroups wait on each other, so all of them must become resident: W is at most
+  // the number that fit on the device at once (fits()), and every other kernel that may run
+  // beside the panel finishes without waiting on it, so the workgroups not yet placed get a
+  // CU as soon as such a kernel's workgroups retire.  A plain launch suffices; the
+  // cooperative launch costs about 30 us of dispatch overhead per panel on top.
   const int W = ceil_div(m, R);
-  void* args[] = {&m, &nb, &P, &ld, &piv, &info, const_cast<PanelSync*>(&sy), &base, &info_base};
-  DDLA_HIP_CHECK(hipLaunchCooperativeKernel(panel_kernel<T, R>(), dim3(W),
-                                            dim3(64 * panel_waves<T, R>()), args, 0, s));
+  hipLaunchKernelGGL((getf2_panel_kernel<T, R, panel_waves<T, R>()>), dim3(W),
+                     dim3(64 * panel_waves<T, R>()), 0, s, m, nb, P, ld, piv, info, sy, base,
+                     info_base);
+  DDLA_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
@@ -629,18 +658,18 @@ int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int
 template <typename T>
 int launch_laswp_list(T* A, int64_t ld, int64_t row0, int64_t ncols, int64_t skip_lo,
                       int64_t skip_hi, const int* list, const int* count, int max_count,
-                      hipStream_t s, int ws_slot) {
+                      hipStream_t s) {
   const int64_t width = ncols - (skip_hi - skip_lo);
   if (width <= 0) return 0;
-  void* tmp = nullptr;
-  DDLA_RETURN_IF(workspace(ws_slot, (size_t)max_count * width * sizeof(T), &tmp));
+  if (max_count > 2 * NB_MAX) {
+    set_error("row list of %d entries exceeds %d", max_count, 2 * NB_MAX);
+    return DDLA_ERR_INVALID_ARG;
+  }
   // bytes: every moved row (at most max_count) is read and written once
   ProfScope prof("laswp_list", 0.0, 2.0 * sizeof(T) * max_count * width, s);
-  const dim3 grid((unsigned)std::min<int64_t>(ceil_div(width, 256), 64), (unsigned)max_count);
-  hipLaunchKernelGGL((laswp_list_kernel<T, true>), grid, dim3(256), 0, s, A, ld, row0, ncols,
-                     skip_lo, skip_hi, list, count, static_cast<T*>(tmp));
-  hipLaunchKernelGGL((laswp_list_kernel<T, false>), grid, dim3(256), 0, s, A, ld, row0, ncols,
-                     skip_lo, skip_hi, list, count, static_cast<T*>(tmp));
+  constexpr int LW = 256 / sizeof(T);
+  hipLaunchKernelGGL((laswp_list_kernel<T>), dim3((unsigned)ceil_div(width, LW)), dim3(1024), 0,
+                     s, A, ld, row0, ncols, skip_lo, skip_hi, list, count);
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -696,9 +725,9 @@ template int launch_getf2_panel_ex<double>(int64_t, int, double*, int64_t, int32
 template int launch_getf2_panel_ex<float>(int64_t, int, float*, int64_t, int32_t*, int32_t*, int,
                                           int*, int*, hipStream_t);
 template int launch_laswp_list<double>(double*, int64_t, int64_t, int64_t, int64_t, int64_t,
-                                       const int*, const int*, int, hipStream_t, int);
+                                       const int*, const int*, int, hipStream_t);
 template int launch_laswp_list<float>(float*, int64_t, int64_t, int64_t, int64_t, int64_t,
-                                      const int*, const int*, int, hipStream_t, int);
+                                      const int*, const int*, int, hipStream_t);
 template int launch_gather_rows<double>(int64_t, int64_t, const double*, int64_t, const int32_t*,
                                         double*, int64_t, bool, hipStream_t);
 template int launch_gather_rows<float>(int64_t, int64_t, const float*, int64_t, const int32_t*,

@@ -61,55 +61,117 @@ __device__ __forceinline__ float block_scale(float amax, bool bf16) {
 // in blocks of 128 share one power-of-two scale; each scaled FP32 value x is split into
 // TERMS low-precision parts x = h + m + l with h = rn(x), m = rn(x - h), l = rn(x - h - m),
 // written K-contiguous with rows padded to Kp (a multiple of 32) with zeros:
-// out[t][r * Kp + k].  One workgroup per 128-row block; the block is
-// staged in LDS so that both the read (KC: x(r,k) = src[r ld + k]; else src[k ld + r]) and the
-// write are coalesced.  Done once per update, not once per output tile.
+// out[t][r * Kp + k].  One workgroup per 128-row block (KC: x(r,k) = src[r ld + k]; else
+// src[k ld + r]).  Every thread issues all of its sixteen 4-wide reads before it uses one
+// (the block is latency-bound, not bandwidth-bound: no read may wait for the previous one); the transposed form goes through LDS so
+// that both sides stay coalesced.  Done once per update, not once per output tile.
 template <bool BF16, int TERMS, bool KC>
 __global__ __launch_bounds__(256) void split_kernel(int rows, int K, int Kp,
                                                     const float* __restrict__ src, int64_t ld,
                                                     typename LowP<BF16>::e* __restrict__ out,
                                                     float* __restrict__ scales) {
   typedef typename LowP<BF16>::e E;
-  __shared__ float xs[HB][NBLK + 1];
+  typedef E e4 __attribute__((ext_vector_type(4)));
+  __shared__ float xs[KC ? 1 : HB][HB + 1];
   __shared__ float red[4];
   const int r0 = blockIdx.x * HB;
   const int nr = min(HB, rows - r0);
   const int tid = threadIdx.x;
-  float mx = 0.0f;
-  for (int t = tid; t < HB * K; t += 256) {
-    int r, k;
-    if (KC) {
-      r = t / K;
-      k = t % K;
-    } else {
-      k = t / HB;
-      r = t % HB;
+  const bool vec = (ld & 3) == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+  // item f = i 256 + tid covers (a, b4 .. b4 + 3), a = f / 32, b4 = 4 (f mod 32): for KC row a
+  // of the block and k = b4.., else k = a and rows b4.. of the block
+  f4 v[16];
+  if (vec) {
+    // whole 4-wide items: branch-free 16-byte reads (an item outside reads src itself and is
+    // zeroed), all in flight together; items cut by the edge are read element-wise after
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int f = i * 256 + tid, a = f >> 5, b4 = (f & 31) * 4;
+      const bool full = (KC ? a < nr : a < K) && b4 + 3 < (KC ? K : nr);
+      const float* p = src + (int64_t)(KC ? r0 + a : a) * ld + (KC ? b4 : r0 + b4);
+      v[i] = *reinterpret_cast<const f4*>(full ? p : src);
     }
-    const float x = (r < nr) ? (KC ? src[(int64_t)(r0 + r) * ld + k] : src[(int64_t)k * ld + r0 + r])
-                             : 0.0f;
-    xs[r][k] = x;
-    mx = fmaxf(mx, fabsf(x));
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int f = i * 256 + tid, a = f >> 5, b4 = (f & 31) * 4;
+      const bool aok = KC ? a < nr : a < K;
+      const int lim = KC ? K : nr;
+      if (!(aok && b4 + 3 < lim)) {
+        const float* p = src + (int64_t)(KC ? r0 + a : a) * ld + (KC ? b4 : r0 + b4);
+        f4 x = {0.f, 0.f, 0.f, 0.f};
+        if (aok) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (b4 + e < lim) x[e] = p[e];
+        }
+        v[i] = x;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int f = i * 256 + tid, a = f >> 5, b4 = (f & 31) * 4;
+      const bool aok = KC ? a < nr : a < K;
+      const int lim = KC ? K : nr;
+      const int64_t off = (int64_t)(KC ? r0 + a : a) * ld + (KC ? b4 : r0 + b4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[i][e] = src[(aok && b4 + e < lim) ? off + e : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int f = i * 256 + tid, a = f >> 5, b4 = (f & 31) * 4;
+      const bool aok = KC ? a < nr : a < K;
+      const int lim = KC ? K : nr;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[i][e] = (aok && b4 + e < lim) ? v[i][e] : 0.f;
+    }
   }
+  float mx = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[i][0]), fabsf(v[i][1])), fmaxf(fabsf(v[i][2]), fabsf(v[i][3]))));
   for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
   if ((tid & 63) == 0) red[tid >> 6] = mx;
+  if (!KC) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int f = i * 256 + tid, a = f >> 5, b4 = (f & 31) * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) xs[b4 + e][a] = v[i][e];
+    }
+  }
   __syncthreads();
   const float sc = block_scale(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])), BF16);
   if (tid == 0) scales[blockIdx.x] = sc;
   const size_t plane = (size_t)rows * Kp;
-  for (int t = tid; t < nr * Kp; t += 256) {
-    const int r = t / Kp, k = t % Kp;
-    float x = k < K ? xs[r][k] * sc : 0.0f;
 #pragma unroll
-    for (int q = 0; q < TERMS; ++q) {
-      const E h = static_cast<E>(x);
-      out[q * plane + (size_t)(r0 + r) * Kp + k] = h;
-      x = x - static_cast<float>(h);
+  for (int i = 0; i < 16; ++i) {
+    const int f = i * 256 + tid, a = f >> 5, b4 = (f & 31) * 4;  // row a, k = b4 ..
+    if (a >= nr || b4 >= Kp) continue;
+    f4 x;
+    if (KC) {
+      x = v[i];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = xs[a][b4 + e];
+    }
+    x *= sc;
+    E* o = out + (size_t)(r0 + a) * Kp + b4;
+#pragma unroll
+    for (int t = 0; t < TERMS; ++t) {
+      e4 h;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        h[e] = static_cast<E>(x[e]);
+        x[e] = x[e] - static_cast<float>(h[e]);
+      }
+      *reinterpret_cast<e4*>(o + t * plane) = h;
     }
   }
 }
 
-// C (FP32, m x n) += alpha * A B from the split operands Ap[t][m][K], Bp[t][n][K] (both
-// K-contiguous, K a multiple of 32 with zero padding) with FP32 accumulation on the matrix cores (v_mfma_f32_16x16x32_{f16,bf16}).
+// C (FP32, m x n) += alpha * A B from the split operands Ap[t pa + i K + k], Bp[t pb + j K + k]
+// (both K-contiguous, K a multiple of 32 with zero padding) with FP32 accumulation on the matrix cores (v_mfma_f32_16x16x32_{f16,bf16}).
 // The tile product sums the part products whose combined weight reaches the FP32 level
 // (TERMS=1: hh; 2: hh+hm+mh; 3: hh+hm+mh+hl+lh+mm), small ones first.  128 x 128 tile per
 // workgroup, K step 32 through LDS with register prefetch; the C tile is read at the start
@@ -121,7 +183,7 @@ __global__ __launch_bounds__(256) void hgemm_kernel(int M, int N, int K, float a
                                                     float* __restrict__ C, int64_t ldc,
                                                     const float* __restrict__ sa,
                                                     const float* __restrict__ sb, int tiles_m,
-                                                    int tiles_n) {
+                                                    int tiles_n, int64_t pa, int64_t pb) {
   typedef LowP<BF16> LP;
   typedef typename LP::v8 v8;
   typedef typename LP::e E;
@@ -137,7 +199,6 @@ __global__ __launch_bounds__(256) void hgemm_kernel(int M, int N, int K, float a
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int l16 = lane & 15, lk = lane >> 4;
-  const size_t pa = (size_t)M * K, pb = (size_t)N * K;
 
   f4 acc[4][4];
 #pragma unroll
@@ -537,41 +598,55 @@ struct Timer {
 
 }  // namespace
 
-// C += alpha * A * B with FP16/BF16 operands and FP32 accumulation (see hgemm_kernel): A m x k
-// (k contiguous); B k x n with n contiguous, or (b_kcontig) B(k, j) at B[j ldb + k]; k <= 128.
-int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k,
-                 float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                 int64_t ldc, float* scale_ws, hipStream_t s, int ws_slot) {
-  if (m <= 0 || n <= 0 || k <= 0) return 0;
-  if (k > NBLK) {
-    set_error("hgemm: k = %lld exceeds %d", (long long)k, NBLK);
+// Splits rows x k of src (kcontig: x(r, c) = src[r ld + c], else src[c ld + r]) into
+// split_kernel's layout at out (terms planes of rows x Kp) and one scale per 128 rows.
+int launch_split(bool bf16, int terms, bool kcontig, int64_t rows, int64_t k, const float* src,
+                 int64_t ld, void* out, float* scales, hipStream_t s) {
+  if (rows <= 0) return 0;
+  if (k <= 0 || k > NBLK || rows > INT32_MAX) {
+    set_error("split: %lld x %lld outside k <= %d", (long long)rows, (long long)k, NBLK);
     return DDLA_ERR_INVALID_ARG;
   }
-  terms = std::min(3, std::max(1, terms));
-  const int tm = ceil_div(m, 128), tn = ceil_div(n, 128);
-  float* sa = scale_ws;
-  float* sb = scale_ws + tm;
-  void* ws = nullptr;
   const int K = (int)k, Kp = (K + 31) / 32 * 32;
-  DDLA_RETURN_IF(workspace(ws_slot, (size_t)terms * (m + n) * Kp * 2, &ws));
-  const int M = (int)m, N = (int)n;
+  const dim3 grid((unsigned)ceil_div(rows, HB)), block(256);
+#define SPL(BF, TM)                                                                              \
+  do {                                                                                           \
+    typedef typename LowP<BF>::e E;                                                              \
+    if (kcontig)                                                                                 \
+      hipLaunchKernelGGL((split_kernel<BF, TM, true>), grid, block, 0, s, (int)rows, K, Kp, src, \
+                         ld, static_cast<E*>(out), scales);                                      \
+    else                                                                                         \
+      hipLaunchKernelGGL((split_kernel<BF, TM, false>), grid, block, 0, s, (int)rows, K, Kp,     \
+                         src, ld, static_cast<E*>(out), scales);                                 \
+  } while (0)
+  if (bf16) {
+    if (terms == 1) SPL(true, 1); else if (terms == 2) SPL(true, 2); else SPL(true, 3);
+  } else {
+    if (terms == 1) SPL(false, 1); else if (terms == 2) SPL(false, 2); else SPL(false, 3);
+  }
+#undef SPL
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// C (m x n) += alpha A B from split operands: row i of A is a.parts row i (scale block
+// i / 128), column j of B is b.parts row j; k <= 128 (padded to Kp in the parts).
+int launch_hgemm_parts(bool bf16, int terms, int tri, int64_t m, int64_t n, int64_t k,
+                       float alpha, SplitOp a, SplitOp b, float* C, int64_t ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0 || k <= 0) return 0;
+  const int tm = ceil_div(m, 128), tn = ceil_div(n, 128);
+  const int Kp = ((int)k + 31) / 32 * 32;
   const double mn = (tri ? 0.5 : 1.0) * (double)m * n;
   ProfScope prof(bf16 ? "hgemm_mfma_bf16" : "hgemm_mfma_f16",
                  2.0 * mn * k * std::max(1, terms * (terms + 1) / 2),
                  4.0 * 2.0 * mn + 2.0 * terms * ((double)m + n) * k, s);
-#define SPLIT(BF, TM, KC, rows, src, ld, out, sc)                                            \
-  hipLaunchKernelGGL((split_kernel<BF, TM, KC>), dim3(ceil_div(rows, HB)), dim3(256), 0, s, \
-                     (int)(rows), K, Kp, src, ld, out, sc)
-#define HG(BF, TM, T)                                                                        \
-  do {                                                                                       \
-    typedef typename LowP<BF>::e E;                                                          \
-    E* ap = static_cast<E*>(ws);                                                             \
-    E* bp = ap + (size_t)TM * m * Kp;                                                        \
-    SPLIT(BF, TM, true, m, A, lda, ap, sa);                                                  \
-    if (b_kcontig) SPLIT(BF, TM, true, n, B, ldb, bp, sb);                                   \
-    else SPLIT(BF, TM, false, n, B, ldb, bp, sb);                                            \
-    hipLaunchKernelGGL((hgemm_kernel<BF, TM, T>), dim3((unsigned)((int64_t)tm * tn)),        \
-                       dim3(256), 0, s, M, N, Kp, alpha, ap, bp, C, ldc, sa, sb, tm, tn);    \
+#define HG(BF, TM, T)                                                                             \
+  do {                                                                                            \
+    typedef typename LowP<BF>::e E;                                                               \
+    hipLaunchKernelGGL((hgemm_kernel<BF, TM, T>), dim3((unsigned)((int64_t)tm * tn)), dim3(256),  \
+                       0, s, (int)m, (int)n, Kp, alpha, static_cast<const E*>(a.parts),           \
+                       static_cast<const E*>(b.parts), C, ldc, a.scales, b.scales, tm, tn,        \
+                       a.plane, b.plane);                                                         \
   } while (0)
 #define HG_T(BF, TM)           \
   do {                         \
@@ -585,9 +660,32 @@ int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64
   }
 #undef HG_T
 #undef HG
-#undef SPLIT
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+// C += alpha * A * B with FP16/BF16 operands and FP32 accumulation (see hgemm_kernel): A m x k
+// (k contiguous); B k x n with n contiguous, or (b_kcontig) B(k, j) at B[j ldb + k]; k <= 128.
+// Both operands are split into the workspace slot first.
+int launch_hgemm(bool bf16, int terms, bool b_kcontig, int tri, int64_t m, int64_t n, int64_t k,
+                 float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                 int64_t ldc, float* scale_ws, hipStream_t s, int ws_slot) {
+  if (m <= 0 || n <= 0 || k <= 0) return 0;
+  if (k > NBLK) {
+    set_error("hgemm: k = %lld exceeds %d", (long long)k, NBLK);
+    return DDLA_ERR_INVALID_ARG;
+  }
+  terms = std::min(3, std::max(1, terms));
+  const int tm = ceil_div(m, 128);
+  const int64_t Kp = (k + 31) / 32 * 32;
+  void* ws = nullptr;
+  DDLA_RETURN_IF(workspace(ws_slot, (size_t)terms * (m + n) * Kp * 2, &ws));
+  char* ap = static_cast<char*>(ws);
+  char* bp = ap + (size_t)terms * m * Kp * 2;
+  DDLA_RETURN_IF(launch_split(bf16, terms, true, m, k, A, lda, ap, scale_ws, s));
+  DDLA_RETURN_IF(launch_split(bf16, terms, b_kcontig, n, k, B, ldb, bp, scale_ws + tm, s));
+  return launch_hgemm_parts(bf16, terms, tri, m, n, k, alpha, SplitOp{ap, m * Kp, scale_ws},
+                            SplitOp{bp, n * Kp, scale_ws + tm}, C, ldc, s);
 }
 
 namespace {
@@ -619,7 +717,8 @@ int mixed_ws(int64_t n, int64_t np, int64_t nrhs, MixedWs* w) {
   w->np = np;
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   const int64_t nb = (np + 127) / 128 + 1;
-  const size_t s_scales = al(4 * nb * sizeof(float));  // two streams' update scales
+  // update scales: main stream, auxiliary stream, two split panels in flight
+  const size_t s_scales = al(6 * nb * sizeof(float));
   const size_t s_perm = al(np * sizeof(int));
   const size_t s_piv = al(np * sizeof(int32_t));
   const size_t s_D = al((size_t)np * nrhs * sizeof(float));
@@ -651,8 +750,10 @@ int mixed_ws(int64_t n, int64_t np, int64_t nrhs, MixedWs* w) {
 // FP32 matrix cores for DDLA_PREC_FP32).  Fills w.perm.  Returns info via *info_host.
 // One trailing update of the mixed LU: U12 = L11^-1 A12 for columns [c0, c0 + nc) of the row
 // block, then A22 -= L21 U12 on those columns (FP32, or FP16 / BF16 matrix cores).
+// l21: the panel's L21 already split (FP16 / BF16), or null to split it here.
 int lu_update_cols(int prec, int terms, int64_t n, int64_t j, int b, int64_t c0, int64_t nc,
-                   float* a, int64_t lda, float* scales, hipStream_t s, int split_slot) {
+                   float* a, int64_t lda, float* scales, hipStream_t s, int split_slot,
+                   const SplitOp* l21) {
   if (nc <= 0) return 0;
   float* panel = a + j * lda + j;
   const int64_t rest = n - j - b;
@@ -660,6 +761,15 @@ int lu_update_cols(int prec, int terms, int64_t n, int64_t j, int b, int64_t c0,
   DDLA_RETURN_IF(launch_trsm_slab<float>(true, true, b, nc, panel, lda, 1, u12, lda, s));
   if (rest <= 0) return 0;
   float* c = a + (j + b) * lda + c0;
+  if (l21 && prec != DDLA_PREC_FP32) {
+    const bool bf16 = prec == DDLA_PREC_BF16;
+    const int64_t Kp = (b + 31) / 32 * 32;
+    void* ws = nullptr;
+    DDLA_RETURN_IF(workspace(split_slot, (size_t)terms * nc * Kp * 2, &ws));
+    DDLA_RETURN_IF(launch_split(bf16, terms, false, nc, b, u12, lda, ws, scales, s));
+    return launch_hgemm_parts(bf16, terms, 0, rest, nc, b, -1.f, *l21,
+                              SplitOp{ws, nc * Kp, scales}, c, lda, s);
+  }
   if (prec == DDLA_PREC_FP32)
     return launch_sgemm(rest, nc, b, -1.f, panel + b * lda, lda, true, u12, lda, true, 1.f, c, lda,
                         0, s);
@@ -676,8 +786,9 @@ int lu_update_cols(int prec, int terms, int64_t n, int64_t j, int b, int64_t c0,
 // latency-bound (one exchange between workgroups per column), so the bulk of the trailing
 // update runs in its shadow.  Ordering, with E0(k) = "panel k done" and E2(k) = "update of
 // the remaining columns after panel k done":
-//   main: panel(k); wait E2(k-1); swaps + update of [0, j) and the next block; panel(k+1) ...
-//   aux:  wait E0(k); swaps + update of the columns after the next block; record E2(k)
+//   main: panel(k); wait E2(k-1); swaps + update of the next block; panel(k+1) ...
+//   aux:  wait E0(k); swaps on [0, j) and the columns after the next block, their update;
+//         record E2(k)
 int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, MixedWs& w,
                  int32_t* info_host) {
   hipStream_t s = stream();
@@ -690,9 +801,17 @@ int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, Mi
   DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
   const int64_t nb = (n + NBLK - 1) / NBLK + 1;
   float* scales_aux = w.scales + 2 * nb;
-  {  // size the main stream's interchange staging once (its width grows with j)
+  // FP16 / BF16 updates: each panel's L21 is split once, on the main stream right after the
+  // panel, and both streams' updates multiply that copy.  Two copies alternate: the one of
+  // panel k is rewritten after panel k + 2, when the main stream has already waited for
+  // E2(k), the last update that reads it.
+  const bool share = prec != DDLA_PREC_FP32;
+  const size_t pslot = (size_t)w.terms * n * NBLK * 2;
+  char* pbuf = nullptr;
+  if (share) {
     void* t = nullptr;
-    DDLA_RETURN_IF(workspace(WS_TEMP, (size_t)2 * NBLK * n * sizeof(float), &t));
+    DDLA_RETURN_IF(workspace(WS_SPLITP, 2 * pslot, &t));
+    pbuf = static_cast<char*>(t);
   }
   // the auxiliary stream starts after everything queued so far on the main stream
   DDLA_HIP_CHECK(hipEventRecord(ev[0], s));
@@ -710,6 +829,15 @@ int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, Mi
                                                   list_k, counts + k, s));
     const int64_t nxt = std::min<int64_t>(NBLK, n - j - b);  // the next panel's width
     const int64_t far0 = j + b + nxt, far = n - far0;        // columns after the next panel
+    SplitOp l21{};
+    if (share && n - j - b > 0) {
+      const int64_t rest = n - j - b, Kp = (b + 31) / 32 * 32;
+      float* sc = w.scales + (4 + (k & 1)) * nb;
+      char* out = pbuf + (k & 1) * pslot;
+      DDLA_RETURN_IF(launch_split(prec == DDLA_PREC_BF16, w.terms, true, rest, b, panel + b * lda,
+                                  lda, out, sc, s));
+      l21 = SplitOp{out, rest * Kp, sc};
+    }
     if (far > 0) {
       DDLA_HIP_CHECK(hipEventRecord(ev[0], s));  // E0(k)
     }
@@ -717,18 +845,24 @@ int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, Mi
       DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));  // E2(k-1): next block fully updated
       pending = false;
     }
-    // main stream: swaps on [0, j) and the next block, its update
-    DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, j + b + nxt, j, j + b, list_k,
-                                            counts + k, 2 * NBLK, s));
+    // main stream: swaps on the next block (and on [0, j) when no far columns are left), its
+    // update
+    if (far > 0)
+      DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda + j + b, lda, 0, nxt, 0, 0, list_k,
+                                              counts + k, 2 * NBLK, s));
+    else
+      DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, j + b + nxt, j, j + b, list_k,
+                                              counts + k, 2 * NBLK, s));
     DDLA_RETURN_IF(lu_update_cols(prec, w.terms, n, j, b, j + b, nxt, a, lda, w.scales, s,
-                                  WS_SPLIT));
-    // auxiliary stream: swaps and update of the far columns
+                                  WS_SPLIT, share ? &l21 : nullptr));
+    // auxiliary stream: swaps on [0, j) and the far columns (off the critical path: nothing
+    // reads the finished columns until the end), update of the far columns
     if (far > 0) {
       DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
-      DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda + far0, lda, 0, far, 0, 0, list_k,
-                                              counts + k, 2 * NBLK, s2, WS_TEMP2));
+      DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, n, j, far0, list_k, counts + k,
+                                              2 * NBLK, s2));
       DDLA_RETURN_IF(lu_update_cols(prec, w.terms, n, j, b, far0, far, a, lda, scales_aux, s2,
-                                    WS_SPLIT2));
+                                    WS_SPLIT2, share ? &l21 : nullptr));
       DDLA_HIP_CHECK(hipEventRecord(ev[1], s2));  // E2(k)
       pending = true;
     }
@@ -797,10 +931,21 @@ int spotrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
     return launch_hgemm(prec == DDLA_PREC_BF16, w.terms, true, 1, m, nc, b, -1.f, A, lda, A, lda,
                         C, lda, sc, st, slot);
   };
+  // FP16 / BF16: L21 is split once per block column and serves as both operands of both
+  // streams' updates (two copies alternate, as in sgetrf_mixed)
+  const bool share = prec != DDLA_PREC_FP32;
+  const size_t pslot = (size_t)w.terms * n * NBLK * 2;
+  char* pbuf = nullptr;
+  if (share) {
+    void* t = nullptr;
+    DDLA_RETURN_IF(workspace(WS_SPLITP, 2 * pslot, &t));
+    pbuf = static_cast<char*>(t);
+  }
   DDLA_HIP_CHECK(hipEventRecord(ev[0], s));
   DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
   bool pending = false;
-  for (int64_t j = 0; j < n; j += NBLK) {
+  int k = 0;
+  for (int64_t j = 0; j < n; j += NBLK, ++k) {
     const int b = (int)std::min<int64_t>(NBLK, n - j);
     float* a11 = a + j * lda + j;
     DDLA_RETURN_IF(launch_potf2_ex<float>(b, a11, lda, dinfo, (int)j, s));
@@ -809,16 +954,37 @@ int spotrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
     float* a21 = a11 + b * lda;
     DDLA_RETURN_IF(launch_trsm_rlt_panel<float>(b, rest, a11, lda, a21, lda, s, false));
     const int64_t nxt = std::min<int64_t>(NBLK, rest), far = rest - nxt;
+    SplitOp l21{};
+    const int64_t Kp = (b + 31) / 32 * 32;
+    if (share) {
+      float* sc = w.scales + (4 + (k & 1)) * nb;
+      char* out = pbuf + (k & 1) * pslot;
+      DDLA_RETURN_IF(launch_split(prec == DDLA_PREC_BF16, w.terms, true, rest, b, a21, lda, out,
+                                  sc, s));
+      l21 = SplitOp{out, rest * Kp, sc};
+    }
     if (far > 0) DDLA_HIP_CHECK(hipEventRecord(ev[0], s));  // E0(k)
     if (pending) {
       DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));  // E2(k-1)
       pending = false;
     }
-    DDLA_RETURN_IF(syrk(rest, nxt, b, a21, a21 + b, w.scales, s, WS_SPLIT));
+    if (share)
+      DDLA_RETURN_IF(launch_hgemm_parts(prec == DDLA_PREC_BF16, w.terms, 1, rest, nxt, b, -1.f,
+                                        l21, l21, a21 + b, lda, s));
+    else
+      DDLA_RETURN_IF(syrk(rest, nxt, b, a21, a21 + b, w.scales, s, WS_SPLIT));
     if (far > 0) {
       DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
       float* af = a21 + nxt * lda;
-      DDLA_RETURN_IF(syrk(far, far, b, af, af + b + nxt, scales_aux, s2, WS_SPLIT2));
+      if (share) {
+        // rows nxt.. of the split L21: nxt = 128 here, one whole scale block
+        const SplitOp lf{static_cast<const char*>(l21.parts) + (size_t)nxt * Kp * 2, l21.plane,
+                         l21.scales + nxt / HB};
+        DDLA_RETURN_IF(launch_hgemm_parts(prec == DDLA_PREC_BF16, w.terms, 1, far, far, b, -1.f,
+                                          lf, lf, af + b + nxt, lda, s2));
+      } else {
+        DDLA_RETURN_IF(syrk(far, far, b, af, af + b + nxt, scales_aux, s2, WS_SPLIT2));
+      }
       DDLA_HIP_CHECK(hipEventRecord(ev[1], s2));  // E2(k)
       pending = true;
     }

@@ -5,10 +5,11 @@
 // (DDLA/Factorizations/Cholesky.cs:156) and CholeskySolveUncheck (Cholesky.cs:210-226).
 //
 // Structure: the triangle is cut into 128-row diagonal blocks.  A diagonal block is
-// solved by trsm_slab_kernel: the packed triangle sits in LDS and each lane substitutes
-// down one right-hand-side column on its own (no barriers).
+// solved by trsm_slab_kernel: the triangle sits in LDS, a wave keeps a few right-hand-side
+// columns in registers (two positions per lane) and substitutes down them together.
 // The off-diagonal coupling is a GEMM update on the matrix cores (launch_dgemm).
 #include "ddla_internal.hpp"
+#include "wave_ops.hpp"
 
 #include <algorithm>
 
@@ -17,63 +18,103 @@ namespace {
 
 constexpr int TB = 128;  // diagonal block size
 
-// Solves op(T) X = B in place for one bs x bs triangle T (strides rs, cs) and the bs x ncols
-// block B.  LOWER selects forward substitution; non-unit diagonals divide.
+// Solves op(T) X = B in place for one bs x bs triangle T (element (r, c) at A[r rs + c cs])
+// and the bs x ncols block B.  LOWER selects forward substitution; non-unit diagonals divide.
 //
-// Each lane owns one column of B and solves it alone, so there is no barrier after the
-// triangle is staged in LDS.  Everything runs in solve-order positions q (row q for LOWER,
-// row bs-1-q otherwise): M[qk][q] is the coupling of the already solved position qk into
-// position q > qk, so the couplings a step needs are contiguous (16-byte LDS reads, the
-// same address in every lane: broadcasts) and M[q][q] is the diagonal.  Positions go in
-// sub-blocks of SB: the sub-block's values sit in registers, take the contributions of the
-// positions already solved (read back from B, in order), then the substitution inside; each
-// x_q = (b_q - sum_k t_qk x_k) / t_qq is the FMA chain in k order of a row-by-row solve.
-template <typename T, bool LOWER, bool UNIT>
+// Everything runs in solve-order positions q (row q for LOWER, row bs-1-q otherwise).  A wave
+// solves CW columns of B: lane l keeps positions l and l + 64 of each of its columns in
+// registers.  Step k (ascending) finishes x_k in its owner lane k mod 64 (division by t_kk
+// unless UNIT), broadcasts it with v_readlane and every lane with an unsolved position q > k
+// subtracts t_qk x_k with one fma: per column the FMA chain in k order of a row-by-row
+// solve.  The strictly lower couplings sit in LDS as Mt[q][k] (stride TB + 1: the column
+// reads of one step hit distinct banks), zero where q <= k.
+template <typename T, bool LOWER, bool UNIT, int CW>
 __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
                                                         const T* __restrict__ A, int64_t rs,
                                                         int64_t cs, T* __restrict__ B,
                                                         int64_t ldb) {
-  constexpr int SB = 16;
-  __shared__ __attribute__((aligned(16))) T M[TB][TB];
-  const int tid = threadIdx.x;
+  __shared__ T Mt[TB][TB + 1];
+  __shared__ T dg[TB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   auto row_of = [&](int q) { return LOWER ? q : bs - 1 - q; };
-  for (int p = tid; p < bs * bs; p += 256) {
-    const int qk = p / bs, q = p % bs;  // q runs along memory for the lower case
-    const int r = row_of(q), c = row_of(qk);
-    // element (r, c) of the stored triangle: lower r >= c, upper r <= c
-    M[qk][q] = (q >= qk) ? A[(int64_t)r * rs + (int64_t)c * cs] : T(0);
+  const int64_t c0 = ((int64_t)blockIdx.x * 4 + wave) * CW;
+  T x0[CW], x1[CW];  // positions lane and lane + 64 (read first: in flight during the staging)
+  const int r0 = lane < bs ? row_of(lane) : 0, r1 = lane + 64 < bs ? row_of(lane + 64) : 0;
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    const bool in = c0 + c < ncols;
+    const bool i0 = in && lane < bs, i1 = in && lane + 64 < bs;
+    x0[c] = B[i0 ? (int64_t)r0 * ldb + c0 + c : 0];
+    x1[c] = B[i1 ? (int64_t)r1 * ldb + c0 + c : 0];
+  }
+  const bool kfast = cs == 1;  // consecutive threads read consecutive addresses
+  // Staging in batches of 32 (FP64: 16) reads per thread, issued back to back (branch-free: an
+  // out-of-range element reads A[0] and is replaced by zero), then written to LDS.
+  constexpr int PER = TB * TB / 256, BATCH = sizeof(T) == 4 ? 32 : 16;
+#pragma unroll
+  for (int b0 = 0; b0 < PER; b0 += BATCH) {
+    T v[BATCH];
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+      const int p = (b0 + i) * 256 + tid;
+      const int q = kfast ? p / TB : p % TB, k = kfast ? p % TB : p / TB;
+      const bool in = q < bs && k <= q;
+      v[i] = A[in ? (int64_t)row_of(q) * rs + (int64_t)row_of(k) * cs : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+      const int p = (b0 + i) * 256 + tid;
+      const int q = kfast ? p / TB : p % TB, k = kfast ? p % TB : p / TB;
+      const T x = (q < bs && k <= q) ? v[i] : T(0);
+      if (k == q) dg[q] = x;
+      Mt[q][k] = k == q ? T(0) : x;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    const bool in = c0 + c < ncols;
+    x0[c] = (in && lane < bs) ? x0[c] : T(0);
+    x1[c] = (in && lane + 64 < bs) ? x1[c] : T(0);
   }
   __syncthreads();
-  const int64_t col = (int64_t)blockIdx.x * 256 + tid;
-  if (col >= ncols) return;
-  T* b = B + col;
-  constexpr int VW = 16 / sizeof(T);  // elements per 16-byte read
-  typedef T vec __attribute__((ext_vector_type(VW)));
-  for (int s0 = 0; s0 < bs; s0 += SB) {
-    T x[SB];
-#pragma unroll
-    for (int j = 0; j < SB; ++j) x[j] = (s0 + j < bs) ? b[(int64_t)row_of(s0 + j) * ldb] : T(0);
-    // positions already solved, in order
+  if (c0 >= ncols) return;  // after the barrier: every wave helped stage the triangle
+  const int h = min(bs, 64);
 #pragma unroll 2
-    for (int qk = 0; qk < s0; ++qk) {
-      const T xk = b[(int64_t)row_of(qk) * ldb];
+  for (int k = 0; k < h; ++k) {
+    const T m0 = Mt[lane][k], m1 = Mt[lane + 64][k];
+    if (!UNIT && lane == k) {
+      const T d = dg[k];
 #pragma unroll
-      for (int v = 0; v < SB / VW; ++v) {
-        const vec m = *reinterpret_cast<const vec*>(&M[qk][s0 + v * VW]);
-#pragma unroll
-        for (int e = 0; e < VW; ++e) x[v * VW + e] = fma(-m[e], xk, x[v * VW + e]);
-      }
-    }
-    // inside the sub-block (entries past bs are zero couplings of zero values)
-#pragma unroll
-    for (int i = 0; i < SB; ++i) {
-      if (!UNIT && s0 + i < bs) x[i] = x[i] / M[s0 + i][s0 + i];
-#pragma unroll
-      for (int j = i + 1; j < SB; ++j) x[j] = fma(-M[s0 + i][s0 + j], x[i], x[j]);
+      for (int c = 0; c < CW; ++c) x0[c] = x0[c] / d;
     }
 #pragma unroll
-    for (int j = 0; j < SB; ++j)
-      if (s0 + j < bs) b[(int64_t)row_of(s0 + j) * ldb] = x[j];
+    for (int c = 0; c < CW; ++c) {
+      const T xk = read_lane(x0[c], k);
+      const T u = fma(-m0, xk, x0[c]);
+      x0[c] = lane > k ? u : x0[c];
+      x1[c] = fma(-m1, xk, x1[c]);
+    }
+  }
+#pragma unroll 2
+  for (int k = 64; k < bs; ++k) {
+    const T m1 = Mt[lane + 64][k];
+    if (!UNIT && lane == k - 64) {
+      const T d = dg[k];
+#pragma unroll
+      for (int c = 0; c < CW; ++c) x1[c] = x1[c] / d;
+    }
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+      const T xk = read_lane(x1[c], k - 64);
+      const T u = fma(-m1, xk, x1[c]);
+      x1[c] = lane + 64 > k ? u : x1[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    if (c0 + c >= ncols) break;
+    if (lane < bs) B[(int64_t)r0 * ldb + c0 + c] = x0[c];
+    if (lane + 64 < bs) B[(int64_t)r1 * ldb + c0 + c] = x1[c];
   }
 }
 
@@ -87,17 +128,25 @@ int launch_trsm_slab(bool lower, bool unit, int bs, int64_t ncols, const T* A, i
     set_error("trsm block %d > %d", bs, TB);
     return DDLA_ERR_INVALID_ARG;
   }
-  dim3 grid(ceil_div(ncols, 256)), block(256);
   ProfScope prof(sizeof(T) == 8 ? "trsm_slab_f64" : "trsm_slab_f32", (double)bs * bs * ncols,
                  sizeof(T) * (2.0 * bs * ncols + 0.5 * bs * bs), s);
-  if (lower && unit)
-    hipLaunchKernelGGL((trsm_slab_kernel<T, true, true>), grid, block, 0, s, bs, ncols, A, rs, cs, B, ldb);
-  else if (lower)
-    hipLaunchKernelGGL((trsm_slab_kernel<T, true, false>), grid, block, 0, s, bs, ncols, A, rs, cs, B, ldb);
-  else if (unit)
-    hipLaunchKernelGGL((trsm_slab_kernel<T, false, true>), grid, block, 0, s, bs, ncols, A, rs, cs, B, ldb);
-  else
-    hipLaunchKernelGGL((trsm_slab_kernel<T, false, false>), grid, block, 0, s, bs, ncols, A, rs, cs, B, ldb);
+  // 4 columns per wave; 8 for wide blocks (half the triangle stagings per column)
+  const bool wide = ncols >= 4096;
+  const dim3 grid((unsigned)ceil_div(ncols, 4 * (wide ? 8 : 4))), block(256);
+#define TRSM_SLAB(L, U)                                                                          \
+  do {                                                                                           \
+    if (wide)                                                                                    \
+      hipLaunchKernelGGL((trsm_slab_kernel<T, L, U, 8>), grid, block, 0, s, bs, ncols, A, rs, cs, \
+                         B, ldb);                                                                \
+    else                                                                                         \
+      hipLaunchKernelGGL((trsm_slab_kernel<T, L, U, 4>), grid, block, 0, s, bs, ncols, A, rs, cs, \
+                         B, ldb);                                                                \
+  } while (0)
+  if (lower && unit) TRSM_SLAB(true, true);
+  else if (lower) TRSM_SLAB(true, false);
+  else if (unit) TRSM_SLAB(false, true);
+  else TRSM_SLAB(false, false);
+#undef TRSM_SLAB
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
 }
