@@ -37,6 +37,14 @@ constexpr int NT = 256;    // threads per workgroup = maximum order
 constexpr int CH = 16;     // column chunk of the updates outside the panel
 constexpr int NB_MAX = 32;
 
+// M[idx] if ok, else 0, without a branch: the read always happens (M[0] when !ok), so the
+// reads of an unrolled loop are all in flight together instead of one round trip each
+template <typename T>
+__device__ __forceinline__ T load_or_zero(const T* __restrict__ M, int64_t idx, bool ok) {
+  const T x = M[ok ? idx : 0];
+  return ok ? x : T(0);
+}
+
 template <typename T>
 struct BatchArgs {
   const int32_t* n;
@@ -50,23 +58,28 @@ struct BatchArgs {
   const int32_t* start;  // [G + 1] ranges into sched
 };
 
-template <typename T>
+template <typename T, int NB>
 struct Smem {
   T sa[4];
   int sr[4];
-  T prow[NB_MAX];        // pivot row of the step (the new row i)
-  T srow[NB_MAX];        // old row i (moves to the pivot row's place)
+  T prow[NB];            // pivot row of the step (the new row i)
+  T srow[NB];            // old row i (moves to the pivot row's place)
   T dsh;                 // Cholesky: diagonal of the next step
   int stop;
   int orig[NT];          // panel-local source row of each panel row after the interchanges
   T lcol[NT];            // Cholesky: column i of the panel's rows
-  T U[NB_MAX][CH + 1];   // LU: one chunk of U12
-  T Pn[NT][NB_MAX + 1];  // the factored panel (row t of the panel in row t)
+  union {
+    T Pn[NT][NB + 1];    // Cholesky: the factored panel (row t of the panel in row t)
+    struct {
+      T L11[NB][NB + 1];  // LU: the panel's unit lower triangle
+      T U12[NB][NT];      // LU: U12 = L11^-1 A12 of all the columns right of the panel
+    } lu;
+  };
 };
 
 // block-wide arg-max of (a, r) over the 4 waves: largest a, smallest r among equal a
-template <typename T>
-__device__ __forceinline__ void block_argmax(T& a, int& r, Smem<T>& sm) {
+template <typename T, int NB>
+__device__ __forceinline__ void block_argmax(T& a, int& r, Smem<T, NB>& sm) {
   wave_argmax_dpp(a, r);
   const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
@@ -91,8 +104,8 @@ __device__ __forceinline__ void block_argmax(T& a, int& r, Smem<T>& sm) {
 // P (LU.cs:386-391); *info = first zero pivot column + 1, and the problem stops there (the
 // reference throws from InvScal, BLAS/1v.cs:99).
 template <typename T, int NB>
-__device__ void lu_problem(int n, T* __restrict__ M, int32_t* __restrict__ P,
-                           int32_t* __restrict__ info, Smem<T>& sm) {
+__device__ __forceinline__ void lu_problem(int n, T* __restrict__ M, int32_t* __restrict__ P,
+                                           int32_t* __restrict__ info, Smem<T, NB>& sm) {
   const int tid = threadIdx.x;
   if (tid == 0) sm.stop = 0;
   for (int j0 = 0; j0 < n; j0 += NB) {
@@ -101,7 +114,7 @@ __device__ void lu_problem(int n, T* __restrict__ M, int32_t* __restrict__ P,
     T v[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c)
-      v[c] = (own && c < jb) ? M[(int64_t)(j0 + tid) * n + j0 + c] : T(0);
+      v[c] = load_or_zero(M, (int64_t)(j0 + tid) * n + j0 + c, own && c < jb);
     if (own) sm.orig[tid] = tid;
     __syncthreads();
     bool stop = false;
@@ -160,9 +173,9 @@ __device__ void lu_problem(int n, T* __restrict__ M, int32_t* __restrict__ P,
       for (int c = 0; c < NB; ++c)
         if (c < jb) M[(int64_t)(j0 + tid) * n + j0 + c] = v[c];
     }
-    if (own) {
+    if (tid < jb) {
 #pragma unroll
-      for (int c = 0; c < NB; ++c) sm.Pn[tid][c] = v[c];
+      for (int c = 0; c < NB; ++c) sm.lu.L11[tid][c] = v[c];
     }
     const int src = own ? sm.orig[tid] : tid;
     const bool moved = own && src != tid;
@@ -171,7 +184,7 @@ __device__ void lu_problem(int n, T* __restrict__ M, int32_t* __restrict__ P,
       T x[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c)
-        x[c] = (moved && c0 + c < j0) ? M[(int64_t)(j0 + src) * n + c0 + c] : T(0);
+        x[c] = load_or_zero(M, (int64_t)(j0 + src) * n + c0 + c, moved && c0 + c < j0);
       __syncthreads();
       if (moved) {
 #pragma unroll
@@ -180,44 +193,76 @@ __device__ void lu_problem(int n, T* __restrict__ M, int32_t* __restrict__ P,
       }
       __syncthreads();
     }
-    // columns right of the panel: gather, U12 = L11^-1 A12, A22 -= L21 U12
-    for (int c0 = j0 + jb; c0 < n; c0 += CH) {
-      const int w = min(CH, n - c0);
-      T x[CH];
+    // Columns right of the panel.  The source row of a panel row t >= jb is t itself or one
+    // of rows 0..jb-1 (every interchange involves a row above jb), so:
+    //   1. U12 = L11^-1 A12 of every column at once, thread c down column c in registers,
+    //      from the source rows of rows 0..jb-1, into LDS;
+    //   2. rows t >= jb: A22 -= L21 U12, each thread along its own row (read from its source
+    //      row, written to row t) with no barrier: nobody writes rows 0..jb-1 meanwhile;
+    //   3. U12 into rows 0..jb-1.
+    // Per element the arithmetic is the substitution's and the update's fma chain in k
+    // order, as in the unblocked elimination.
+    const int w = n - j0 - jb;  // columns right of the panel
+    if (w > 0) {
+      __syncthreads();  // L11 and the row origins in LDS
+      if (tid < w) {
+        T u[NB];
 #pragma unroll
-      for (int c = 0; c < CH; ++c)
-        x[c] = (own && c < w) ? M[(int64_t)(j0 + src) * n + c0 + c] : T(0);
-      __syncthreads();  // every source row of the chunk is read before any is written
-      if (tid < jb) {
+        for (int r = 0; r < NB; ++r)
+          u[r] = load_or_zero(M, (int64_t)(j0 + sm.orig[min(r, jb - 1)]) * n + j0 + jb + tid,
+                              r < jb);
 #pragma unroll
-        for (int c = 0; c < CH; ++c) sm.U[tid][c] = x[c];
+        for (int r = 1; r < NB; ++r) {
+          if (r < jb) {
+#pragma unroll
+            for (int k = 0; k < r; ++k) u[r] = fma(-sm.lu.L11[r][k], u[k], u[r]);
+          }
+          __builtin_amdgcn_sched_barrier(0);  // one row's LDS reads at a time
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r)
+          if (r < jb) sm.lu.U12[r][tid] = u[r];
       }
       __syncthreads();
-      if (tid < w) {  // forward substitution down column tid of the chunk (unit L11)
-        for (int r = 1; r < jb; ++r) {
-          T s = sm.U[r][tid];
-          for (int k = 0; k < r; ++k) s = fma(-sm.Pn[r][k], sm.U[k][tid], s);
-          sm.U[r][tid] = s;
+      if (own && tid >= jb) {
+        const T* sr = M + (int64_t)(j0 + src) * n + j0 + jb;
+        T* dr = M + (int64_t)(j0 + tid) * n + j0 + jb;
+        // the row's multipliers, read back (L2-resident) rather than kept in registers
+        // through step 1
+        T l21[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) l21[k] = load_or_zero(dr - jb, k, k < jb);
+        T x[CH], y[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) x[c] = load_or_zero(sr, c, c < w);
+        for (int c0 = 0; c0 < w; c0 += CH) {
+          // the next chunk's reads go out before this chunk's arithmetic
+#pragma unroll
+          for (int c = 0; c < CH; ++c) y[c] = load_or_zero(sr, c0 + CH + c, c0 + CH + c < w);
+#pragma unroll
+          for (int k = 0; k < NB; ++k) {
+            if (k < jb) {
+              const T l = l21[k];
+#pragma unroll
+              for (int c = 0; c < CH; ++c) x[c] = fma(-l, sm.lu.U12[k][c0 + c], x[c]);
+            }
+            __builtin_amdgcn_sched_barrier(0);  // one k's LDS reads at a time
+          }
+#pragma unroll
+          for (int c = 0; c < CH; ++c)
+            if (c0 + c < w) dr[c0 + c] = x[c];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) x[c] = y[c];
         }
       }
       __syncthreads();
-      if (tid < jb) {
+      if (tid < w) {
 #pragma unroll
-        for (int c = 0; c < CH; ++c)
-          if (c < w) M[(int64_t)(j0 + tid) * n + c0 + c] = sm.U[tid][c];
-      } else if (own) {
-#pragma unroll 2
-        for (int k = 0; k < jb; ++k) {
-          const T l = sm.Pn[tid][k];
-#pragma unroll
-          for (int c = 0; c < CH; ++c) x[c] = fma(-l, sm.U[k][c], x[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < CH; ++c)
-          if (c < w) M[(int64_t)(j0 + tid) * n + c0 + c] = x[c];
+        for (int r = 0; r < NB; ++r)
+          if (r < jb) M[(int64_t)(j0 + r) * n + j0 + jb + tid] = sm.lu.U12[r][tid];
       }
-      __syncthreads();  // U is rewritten by the next chunk
     }
+    __syncthreads();  // L11 / U12 are rewritten by the next panel
   }
 }
 
@@ -225,8 +270,8 @@ __device__ void lu_problem(int n, T* __restrict__ M, int32_t* __restrict__ P,
 // A22 -= a21 a21^T on the lower triangle (Cholesky.cs:127-140); only the lower triangle is
 // written.  *info = first non-positive pivot + 1.
 template <typename T, int NB>
-__device__ void chol_problem(int n, T* __restrict__ M, int64_t ld, int32_t* __restrict__ info,
-                             Smem<T>& sm) {
+__device__ __forceinline__ void chol_problem(int n, T* __restrict__ M, int64_t ld,
+                                             int32_t* __restrict__ info, Smem<T, NB>& sm) {
   const int tid = threadIdx.x;
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int jb = min(NB, n - j0), m = n - j0;
@@ -234,7 +279,7 @@ __device__ void chol_problem(int n, T* __restrict__ M, int64_t ld, int32_t* __re
     T v[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c)
-      v[c] = (own && c < jb) ? M[(int64_t)(j0 + tid) * ld + j0 + c] : T(0);
+      v[c] = load_or_zero(M, (int64_t)(j0 + tid) * ld + j0 + c, own && c < jb);
     if (tid == 0) sm.dsh = v[0];
     __syncthreads();
     bool stop = false;
@@ -276,11 +321,15 @@ __device__ void chol_problem(int n, T* __restrict__ M, int64_t ld, int32_t* __re
     // A22 -= L21 L21^T, lower triangle; row tid of the panel is row j0 + tid
     if (own && tid >= jb) {
       const int row = j0 + tid;
+      T x[CH], y[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+        x[c] = load_or_zero(M, (int64_t)row * ld + j0 + jb + c, j0 + jb + c <= row);
       for (int c0 = j0 + jb; c0 <= row; c0 += CH) {
-        T x[CH];
+        // the next chunk's reads go out before this chunk's arithmetic
 #pragma unroll
         for (int c = 0; c < CH; ++c)
-          x[c] = (c0 + c <= row) ? M[(int64_t)row * ld + c0 + c] : T(0);
+          y[c] = load_or_zero(M, (int64_t)row * ld + c0 + CH + c, c0 + CH + c <= row);
         const int lc0 = c0 - j0;  // panel row of column c0
 #pragma unroll 2
         for (int k = 0; k < jb; ++k) {
@@ -292,6 +341,8 @@ __device__ void chol_problem(int n, T* __restrict__ M, int64_t ld, int32_t* __re
 #pragma unroll
         for (int c = 0; c < CH; ++c)
           if (c0 + c <= row) M[(int64_t)row * ld + c0 + c] = x[c];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) x[c] = y[c];
       }
     }
     __syncthreads();
@@ -300,7 +351,7 @@ __device__ void chol_problem(int n, T* __restrict__ M, int64_t ld, int32_t* __re
 
 template <typename T, int NB>
 __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs<T> args) {
-  __shared__ Smem<T> sm;
+  __shared__ Smem<T, NB> sm;
   const int b = args.start[blockIdx.x], e = args.start[blockIdx.x + 1];
   for (int t = b; t < e; ++t) {
     const int p = args.sched[t];
