@@ -65,16 +65,6 @@ __device__ __forceinline__ void put(u64* p, unsigned payload, unsigned tag) {
 __device__ __forceinline__ u64 get(const u64* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// waits for the word to carry `tag`; returns its payload
-__device__ __forceinline__ unsigned wait_word(const u64* p, unsigned tag) {
-  u64 v = get(p);
-  while (static_cast<unsigned>(v) != tag) {
-    __builtin_amdgcn_s_sleep(1);
-    v = get(p);
-  }
-  return static_cast<unsigned>(v >> 32);
-}
-
 template <typename T>
 __device__ __forceinline__ void put_elem(u64* p, T v, unsigned tag) {
   if constexpr (sizeof(T) == 8) {
@@ -85,47 +75,55 @@ __device__ __forceinline__ void put_elem(u64* p, T v, unsigned tag) {
     put(p, __builtin_bit_cast(unsigned, v), tag);
   }
 }
-// Waits for one element (both words of an FP64 value are requested together, so the
-// common case costs a single round trip).
-template <typename T>
-__device__ __forceinline__ T wait_elem(const u64* p, unsigned tag) {
-  if constexpr (sizeof(T) == 8) {
-    u64 a = get(p), b = get(p + 1);
-    while (static_cast<unsigned>(a) != tag || static_cast<unsigned>(b) != tag) {
-      __builtin_amdgcn_s_sleep(1);
-      a = get(p);
-      b = get(p + 1);
-    }
-    return __builtin_bit_cast(T, ((a >> 32) << 32) | (b >> 32));
-  } else {
-    return __builtin_bit_cast(T, wait_word(p, tag));
-  }
-}
-
-// Waits for two elements at once (pivot row and diagonal row of the same column).
-template <typename T>
-__device__ __forceinline__ void wait_elem2(const u64* p, const u64* q, unsigned tag, T& x, T& y) {
-  constexpr int NW = sizeof(T) == 8 ? 2 : 1;
-  u64 a[NW], b[NW];
+// Reads the published row at `src` (n elements, 2 words each) into LDS `dst`, and, when `dst2`
+// is set, the row at `src2` into `dst2`.  Lane l takes elements l, l + 64, ...; every load of
+// a lane is issued before any tag is checked, so a row that is already visible costs one round
+// trip, and a round with a stale word re-reads the whole set.
+template <typename T, int CPL>
+__device__ __forceinline__ void fetch_rows(const u64* src, const u64* src2, int n, int lane,
+                                           unsigned tag, T* dst, T* dst2) {
+  constexpr int EW = sizeof(T) == 8 ? 2 : 1;
+  u64 a[CPL][EW], b[CPL][EW];
+  const bool two = src2 != nullptr;
   for (;;) {
 #pragma unroll
-    for (int k = 0; k < NW; ++k) {
-      a[k] = get(p + k);
-      b[k] = get(q + k);
+    for (int j = 0; j < CPL; ++j) {
+      const int c = lane + 64 * j;
+      if (c < n) {
+#pragma unroll
+        for (int e = 0; e < EW; ++e) a[j][e] = get(src + 2 * c + e);
+        if (two) {
+#pragma unroll
+          for (int e = 0; e < EW; ++e) b[j][e] = get(src2 + 2 * c + e);
+        }
+      }
     }
     bool ok = true;
 #pragma unroll
-    for (int k = 0; k < NW; ++k)
-      ok = ok && static_cast<unsigned>(a[k]) == tag && static_cast<unsigned>(b[k]) == tag;
+    for (int j = 0; j < CPL; ++j) {
+      if (lane + 64 * j < n) {
+#pragma unroll
+        for (int e = 0; e < EW; ++e) {
+          ok = ok && static_cast<unsigned>(a[j][e]) == tag;
+          if (two) ok = ok && static_cast<unsigned>(b[j][e]) == tag;
+        }
+      }
+    }
     if (ok) break;
     __builtin_amdgcn_s_sleep(1);
   }
-  if constexpr (sizeof(T) == 8) {
-    x = __builtin_bit_cast(T, ((a[0] >> 32) << 32) | (a[1] >> 32));
-    y = __builtin_bit_cast(T, ((b[0] >> 32) << 32) | (b[1] >> 32));
-  } else {
-    x = __builtin_bit_cast(T, static_cast<unsigned>(a[0] >> 32));
-    y = __builtin_bit_cast(T, static_cast<unsigned>(b[0] >> 32));
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const int c = lane + 64 * j;
+    if (c < n) {
+      if constexpr (sizeof(T) == 8) {
+        dst[c] = __builtin_bit_cast(T, ((a[j][0] >> 32) << 32) | (a[j][1] >> 32));
+        if (two) dst2[c] = __builtin_bit_cast(T, ((b[j][0] >> 32) << 32) | (b[j][1] >> 32));
+      } else {
+        dst[c] = __builtin_bit_cast(T, static_cast<unsigned>(a[j][0] >> 32));
+        if (two) dst2[c] = __builtin_bit_cast(T, static_cast<unsigned>(b[j][0] >> 32));
+      }
+    }
   }
 }
 
@@ -370,13 +368,10 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
       const bool own_p = (row != i) && (row >= r0 && row < r0 + R);
       const u64* cw = sy.cand + ((size_t)par * WMAX + wwin) * NBX * 2;
       const u64* dg = sy.diag + (size_t)par * NBX * 2;
-      for (int c = lane; c < NBX; c += 64) {
-        if (own_p && !own_i) {
-          wait_elem2<T>(cw + 2 * c, dg + 2 * c, tag, prow[c], drow[c]);
-        } else {
-          prow[c] = wait_elem<T>(cw + 2 * c, tag);
-        }
-      }
+      // the origin column (index NB_MAX) sits right after the row, so NBX elements cover
+      // columns [0, nb) of any width and the origin
+      fetch_rows<T, (NBX + 63) / 64>(cw, (own_p && !own_i) ? dg : nullptr, NBX, lane, tag, prow,
+                                     drow);
       if (lane == 0) sr[1][0] = row;
     }
     __syncthreads();  // prow / drow / the winner in LDS
