@@ -151,22 +151,25 @@ __device__ __forceinline__ void reg_set(X (&v)[N], int k, X x) {
 
 // The workgroup's R panel rows live in registers: local row lr = wave + NW k (k < R/NW) is
 // held by the 64 lanes of wave lr % NW, lane l holding columns l and l + 64; the row's origin
-// sits in LDS.  Column i of a wave's rows therefore sits in one lane: the rank-1 update takes
-// each row's multiplier with v_readlane, and the next column goes to LDS for the arg-max.
+// sits in LDS.  Column i of a wave's rows therefore sits in one lane.
 //
-// Column c of a wave's rows (lane c & 63 of v0 or v1) goes to colb[0, RW) in LDS, so that the
-// arg-max over the rows is a lane-parallel reduction instead of a scan in one lane.
+// Column c of a wave's rows (lane c & 63 of v0 or v1) goes to colb[0, RW) in LDS: the arg-max
+// over the rows is then a lane-parallel reduction instead of a scan in one lane, and the rank-1
+// update reads the rows' multipliers from there as broadcasts.  Rows above c (k < kz) are
+// stored as zero, so that they take a zero multiplier.  (Rows below the panel's last row hold
+// zeros from the start; the arg-max and the write-back skip them.)
 template <typename T, int RW>
 __device__ __forceinline__ void column_to_lds(const T (&v0)[RW], const T (&v1)[RW], int c,
-                                              int lane, T* colb) {
+                                              int lane, int kz, T* colb) {
   if (lane == (c & 63)) {
-    if (c < 64) {
+    // the row test runs in a vector register (one compare per row); as a scalar it would be
+    // hoisted out of the column loop into one live mask per row
+    const int kzv = opaque(kz);
+    T x[RW];
 #pragma unroll
-      for (int k = 0; k < RW; ++k) colb[k] = v0[k];
-    } else {
+    for (int k = 0; k < RW; ++k) x[k] = c < 64 ? v0[k] : v1[k];
 #pragma unroll
-      for (int k = 0; k < RW; ++k) colb[k] = v1[k];
-    }
+    for (int k = 0; k < RW; ++k) colb[k] = k >= kzv ? x[k] : T(0);
   }
 }
 
@@ -189,45 +192,43 @@ __device__ __forceinline__ void wave_argmax(const T* colb, int kmin, int kmax, i
 }
 
 // Step i of the rank-1 update on one wave's rows gr = gr0 + NW k: the multiplier of row gr is
-// its column-i entry (lane i & 63) times 1/pivot, stored in place; columns > i take
-// fma(-l, u, a) (GeR), columns < i keep their values (u is zero there, so the fma returns
-// them unchanged).  Column i + 1 of the rows then goes to LDS for the next arg-max.  Branch-free: a row outside (i, m) takes multiplier 0 and keeps column i.
-// HI: column i is in the upper half (v1).  The work is laid out in passes over all rows
-// (multipliers, their broadcast through v_readlane, the update) so that the rows'
-// independent latency chains overlap instead of running one row after the other.
+// its column-i entry times 1/pivot, stored in place; columns > i take fma(-l, u, a) (GeR),
+// columns < i keep their values (u is zero there, so the fma returns them unchanged).  The
+// column-i entries come from colb, where the rows outside (i, m) hold zero (the row at i was
+// cleared by the interchange), so every lane forms the same multipliers from LDS broadcasts
+// and a row outside (i, m) takes multiplier 0 without a per-row test; only the in-place store
+// in the lane of column i is limited to the rows k >= kmin, by one compare per row against a
+// lane value.  Column i + 1 of the rows then goes to LDS for the next arg-max.
+// HI: column i is in the upper half (v1).
 template <typename T, int RW, int NW, bool HI>
 __device__ __forceinline__ void panel_update(T (&v0)[RW], T (&v1)[RW], T* colb, int i,
-                                             int gr0, int m, T rinv, T u0, T u1, int lane) {
-  // rows k in [kmin, kmax) lie in (i, m)
+                                             int gr0, T rinv, T u0, T u1, int lane) {
+  // rows k >= kmin lie below row i
   // (a negative numerator truncates to a value <= 0, which the clamp turns into 0)
   const int kmin = min(max((i + NW - gr0) / NW, 0), RW);
-  const int kmax = min(max((m + NW - 1 - gr0) / NW, 0), RW);
   const int cl = HI ? i - 64 : i;  // lane of column i
+  const int q = lane == cl ? kmin : RW;
   T l[RW];
 #pragma unroll
-  for (int k = 0; k < RW; ++k) l[k] = (HI ? v1[k] : v0[k]) * rinv;
-#pragma unroll
-  for (int k = 0; k < RW; ++k) l[k] = read_lane(l[k], cl);
-#pragma unroll
-  for (int k = 0; k < RW; ++k) l[k] = (k >= kmin && k < kmax) ? l[k] : T(0);
-  const bool isl = lane == cl;
+  for (int k = 0; k < RW; ++k) l[k] = colb[k] * rinv;
   if constexpr (!HI) {
     const T z0 = lane > i ? u0 : T(0);
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const T n0 = fma(-l[k], z0, v0[k]);
       v1[k] = fma(-l[k], u1, v1[k]);  // every column >= 64 > i
-      v0[k] = (isl && k >= kmin && k < kmax) ? l[k] : n0;
+      v0[k] = k >= q ? l[k] : n0;
     }
   } else {
     const T z1 = lane > cl ? u1 : T(0);
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const T n1 = fma(-l[k], z1, v1[k]);
-      v1[k] = (isl && k >= kmin && k < kmax) ? l[k] : n1;
+      v1[k] = k >= q ? l[k] : n1;
     }
   }
-  column_to_lds(v0, v1, i + 1, lane, colb);
+  // rows k >= kmin (gr > i) are the candidates of step i + 1
+  column_to_lds(v0, v1, i + 1, lane, kmin, colb);
 }
 
 template <typename T, int R, int NW>
@@ -271,9 +272,9 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     v0[k] = (ok && c0 < nb) ? P[(int64_t)gr * ld + c0] : T(0);
     v1[k] = (ok && c1 < nb) ? P[(int64_t)gr * ld + c1] : T(0);
   }
-  column_to_lds(v0, v1, 0, lane, colb[wave]);
   const int gr0 = r0 + wave;
   const int kmax = min(max((m + NW - 1 - gr0) / NW, 0), RW);  // rows k < kmax lie below m
+  column_to_lds(v0, v1, 0, lane, 0, colb[wave]);
   for (int t = tid; t < R; t += NT) orgs[t] = r0 + t;
 
   for (int i = 0; i < nb; ++i) {
@@ -382,19 +383,25 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     const bool stop = (pv == static_cast<T>(0));
     if (stop && w == 0 && tid == 0) atomicCAS(info, 0, info_base + i + 1);
     // 4. interchange rows i and `row` (register rows, selected with uniform indices)
+    //    colb follows: the pivot row at i takes no multiplier (zero), the row moved to
+    //    `row` brings its column-i entry
     const T u0 = prow[c0], u1 = prow[c1];
-    if (row != i) {
-      if (own_i && wave == (i - r0) % NW) {
-        const int kd = (i - r0) / NW;
+    if (own_i && wave == (i - r0) % NW) {
+      const int kd = (i - r0) / NW;
+      if (row != i) {
         reg_set(v0, kd, u0);
         reg_set(v1, kd, u1);
         if (lane == 0) orgs[i - r0] = static_cast<int>(prow[NB_MAX]);
       }
-      if (own_p && wave == (row - r0) % NW) {
-        const int kp = (row - r0) / NW;
-        reg_set(v0, kp, drow[c0]);
-        reg_set(v1, kp, drow[c1]);
-        if (lane == 0) orgs[row - r0] = static_cast<int>(drow[NB_MAX]);
+      if (lane == 0) colb[wave][kd] = T(0);
+    }
+    if (own_p && wave == (row - r0) % NW) {
+      const int kp = (row - r0) / NW;
+      reg_set(v0, kp, drow[c0]);
+      reg_set(v1, kp, drow[c1]);
+      if (lane == 0) {
+        orgs[row - r0] = static_cast<int>(drow[NB_MAX]);
+        colb[wave][kp] = drow[i];
       }
     }
     if (stop) break;
@@ -402,9 +409,9 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     //    arg-max of column i + 1 over the updated rows
     const T rinv = static_cast<T>(1) / pv;
     if (!hi) {
-      panel_update<T, RW, NW, false>(v0, v1, colb[wave], i, gr0, m, rinv, u0, u1, lane);
+      panel_update<T, RW, NW, false>(v0, v1, colb[wave], i, gr0, rinv, u0, u1, lane);
     } else {
-      panel_update<T, RW, NW, true>(v0, v1, colb[wave], i, gr0, m, rinv, u0, u1, lane);
+      panel_update<T, RW, NW, true>(v0, v1, colb[wave], i, gr0, rinv, u0, u1, lane);
     }
   }
 
