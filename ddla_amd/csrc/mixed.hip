@@ -801,10 +801,10 @@ int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, Mi
   DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
   const int64_t nb = (n + NBLK - 1) / NBLK + 1;
   float* scales_aux = w.scales + 2 * nb;
-  // FP16 / BF16 updates: each panel's L21 is split once, on the main stream right after the
-  // panel, and both streams' updates multiply that copy.  Two copies alternate: the one of
-  // panel k is rewritten after panel k + 2, when the main stream has already waited for
-  // E2(k), the last update that reads it.
+  // FP16 / BF16 updates: the far columns' update splits the panel's L21 once on the auxiliary
+  // stream (two copies alternate by panel parity).  The next panel's 128 columns, on the
+  // critical path between two panels, take their update in FP32 on the matrix cores instead:
+  // no split kernels there, and about 1/64 of the update flops at n = 16384.
   const bool share = prec != DDLA_PREC_FP32;
   const size_t pslot = (size_t)w.terms * n * NBLK * 2;
   char* pbuf = nullptr;
@@ -829,15 +829,6 @@ int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, Mi
                                                   list_k, counts + k, s));
     const int64_t nxt = std::min<int64_t>(NBLK, n - j - b);  // the next panel's width
     const int64_t far0 = j + b + nxt, far = n - far0;        // columns after the next panel
-    SplitOp l21{};
-    if (share && n - j - b > 0) {
-      const int64_t rest = n - j - b, Kp = (b + 31) / 32 * 32;
-      float* sc = w.scales + (4 + (k & 1)) * nb;
-      char* out = pbuf + (k & 1) * pslot;
-      DDLA_RETURN_IF(launch_split(prec == DDLA_PREC_BF16, w.terms, true, rest, b, panel + b * lda,
-                                  lda, out, sc, s));
-      l21 = SplitOp{out, rest * Kp, sc};
-    }
     if (far > 0) {
       DDLA_HIP_CHECK(hipEventRecord(ev[0], s));  // E0(k)
     }
@@ -846,21 +837,30 @@ int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, Mi
       pending = false;
     }
     // main stream: swaps on the next block (and on [0, j) when no far columns are left), its
-    // update
+    // update (FP32)
     if (far > 0)
       DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda + j + b, lda, 0, nxt, 0, 0, list_k,
                                               counts + k, 2 * NBLK, s));
     else
       DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, j + b + nxt, j, j + b, list_k,
                                               counts + k, 2 * NBLK, s));
-    DDLA_RETURN_IF(lu_update_cols(prec, w.terms, n, j, b, j + b, nxt, a, lda, w.scales, s,
-                                  WS_SPLIT, share ? &l21 : nullptr));
+    DDLA_RETURN_IF(lu_update_cols(DDLA_PREC_FP32, w.terms, n, j, b, j + b, nxt, a, lda,
+                                  w.scales, s, WS_SPLIT, nullptr));
     // auxiliary stream: swaps on [0, j) and the far columns (off the critical path: nothing
     // reads the finished columns until the end), update of the far columns
     if (far > 0) {
       DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
       DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, n, j, far0, list_k, counts + k,
                                               2 * NBLK, s2));
+      SplitOp l21{};
+      if (share) {
+        const int64_t rest = n - j - b, Kp = (b + 31) / 32 * 32;
+        float* sc = w.scales + (4 + (k & 1)) * nb;
+        char* out = pbuf + (k & 1) * pslot;
+        DDLA_RETURN_IF(launch_split(prec == DDLA_PREC_BF16, w.terms, true, rest, b,
+                                    panel + b * lda, lda, out, sc, s2));
+        l21 = SplitOp{out, rest * Kp, sc};
+      }
       DDLA_RETURN_IF(lu_update_cols(prec, w.terms, n, j, b, far0, far, a, lda, scales_aux, s2,
                                     WS_SPLIT2, share ? &l21 : nullptr));
       DDLA_HIP_CHECK(hipEventRecord(ev[1], s2));  // E2(k)
