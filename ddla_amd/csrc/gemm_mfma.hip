@@ -8,8 +8,8 @@
 //   * one workgroup (256 lanes = 4 wave64) computes a BM x BN tile of row-major C;
 //     the 4 waves form a 2 x 2 grid and each wave owns (BM/2) x (BN/2) = TM x TN MFMA
 //     16x16 tiles held in accumulator registers;
-//   * K advances in steps of 16: the next A/B slices are fetched from HBM into
-//     registers while the matrix cores work on the current slices in LDS (double
+//   * K advances in steps of 16 (32 for the 64 x 64 tiles): the next A/B slices are fetched
+//     from HBM into registers while the matrix cores work on the current slices in LDS (double
 //     buffered LDS, one barrier per step) -- this replaces the CPU path's explicit
 //     packing pass;
 //   * LDS rows are padded by two elements so the MFMA operand reads (16 lanes = 16 rows
@@ -49,12 +49,12 @@ struct Vec<float> {
   static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
-template <typename T, int BM_, int BN_>
+template <typename T, int BM_, int BN_, int BK_ = 16>
 struct Cfg {
   typedef T E;
   typedef typename Vec<T>::v2 v2;
   typedef typename Vec<T>::v4 v4;
-  static constexpr int BM = BM_, BN = BN_, BK = 16;
+  static constexpr int BM = BM_, BN = BN_, BK = BK_;
   static constexpr int WM = BM / 2, WN = BN / 2;
   static constexpr int TM = WM / 16, TN = WN / 16;
   static constexpr int PA = BK + 2;  // LDS row stride of A tile ([m][k])
@@ -359,8 +359,10 @@ int gemm_any(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, 
   if (big_tiles >= 2 * cus)
     return launch_layout<Cfg<T, 128, 128>>(M, N, K, alpha, A, lda, a_kcontig, B, ldb, b_ncontig,
                                            beta, C, ldc, tri, s);
-  return launch_layout<Cfg<T, 64, 64>>(M, N, K, alpha, A, lda, a_kcontig, B, ldb, b_ncontig, beta,
-                                       C, ldc, tri, s);
+  // small tiles: one wave per SIMD, so each K step is made long enough (32) for the MFMA work
+  // to cover the latency of the next step's global loads
+  return launch_layout<Cfg<T, 64, 64, 32>>(M, N, K, alpha, A, lda, a_kcontig, B, ldb, b_ncontig,
+                                           beta, C, ldc, tri, s);
 }
 
 }  // namespace
