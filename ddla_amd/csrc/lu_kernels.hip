@@ -7,24 +7,27 @@
 // Panel (getf2_panel_kernel): the m x nb panel is spread over W co-resident workgroups,
 // R rows each, held in registers for the whole panel.  Partial pivoting needs one global
 // arg-max per column; each column step is one exchange round between the workgroups:
-//   1. every workgroup reduces |a_ri| over its rows (first index wins ties, NaN is the
-//      smallest value, exactly AMax's rule) and publishes its candidate (key, row) and the
-//      candidate row's contents; the owner of the diagonal row also publishes that row;
-//   2. every workgroup reads the W candidates, reduces them to the same winner, reads the
-//      winner's row, and performs the interchange locally (the owner of the pivot row takes
-//      the published diagonal row);
+//   1. every workgroup reduces |a_ri| over its rows that are not yet pivotal (first index
+//      wins ties, NaN is the smallest value, exactly AMax's rule) and publishes its candidate
+//      (key, label) and the candidate row's contents;
+//   2. every workgroup reads the W candidates, reduces them to the same winner and reads the
+//      winner's row;
 //   3. every workgroup scales its part of the column by 1/pivot (InvScal multiplies by
 //      the reciprocal, DDLA/BLAS/1v.cs:102) and applies the rank-1 update to its rows.
+// Interchanges move labels, not data: every register row carries its logical row index
+// (its position in the reference's physically interchanged panel).  Swapping rows i and p
+// gives the pivot row label i and the row labelled i label p; arg-max ties go to the smallest
+// label, the rows labelled < i are pivotal and excluded.  The arithmetic on every row is the
+// same as with moved rows.  At the end each row is written to the position of its label,
+// and each workgroup emits (dst, src) pairs for the rows that moved, so the interchanges for
+// the rest of the matrix are one parallel gather and scatter (laswp_list_kernel) instead of
+// nb sequential swaps.
 // The exchange needs no fence or barrier flag: every published 64-bit word carries the
 // 32-bit step tag next to 32 bits of payload (an FP64 value takes two words), is written
 // and read with single-copy-atomic agent-scope accesses, and a reader simply re-reads a
 // word until its tag is the current step's.  Slots alternate between two buffers by step
 // parity; a workgroup can only reach step i+2 after every workgroup published step i+1,
 // i.e. after every reader of step i finished, so a slot is never overwritten while read.
-// Every row carries one extra value: the panel-relative index of the row whose
-// trailing data belongs there.  At the end each workgroup emits (dst, src) pairs for
-// moved rows, so the interchanges for the rest of the matrix are one parallel gather and
-// scatter (laswp_list_kernel) instead of nb sequential swaps.
 #include "ddla_internal.hpp"
 #include "wave_ops.hpp"
 
@@ -35,9 +38,8 @@ namespace ddla {
 
 struct PanelSync {
   // double-buffered exchange area (one workspace allocation), 64-bit tagged words
-  unsigned long long* keys;  // [2][WMAX][KSTRIDE]: key high, key low, row (+ padding)
-  unsigned long long* cand;  // [2][WMAX][NBX][2]: candidate row (2 words per element)
-  unsigned long long* diag;  // [2][NBX][2]: diagonal row of the step
+  unsigned long long* keys;  // [2][WMAX][KSTRIDE]: key high, key low, packed label (+ padding)
+  unsigned long long* cand;  // [2][WMAX][NB_MAX][2]: candidate row (2 words per element)
   int* list;                 // [2*nb][2] (dst, src) panel-relative
   int* list_count;           // [1]
 };
@@ -45,7 +47,6 @@ struct PanelSync {
 namespace {
 
 constexpr int NB_MAX = 128;
-constexpr int NBX = NB_MAX + 1;  // + origin column
 constexpr int WMAX = 256;        // at most one workgroup per CU
 constexpr int KSTRIDE = 16;      // one 128-byte line per key slot: readers spread over lines
 
@@ -155,80 +156,89 @@ __device__ __forceinline__ void reg_set(X (&v)[N], int k, X x) {
 //
 // Column c of a wave's rows (lane c & 63 of v0 or v1) goes to colb[0, RW) in LDS: the arg-max
 // over the rows is then a lane-parallel reduction instead of a scan in one lane, and the rank-1
-// update reads the rows' multipliers from there as broadcasts.  Rows above c (k < kz) are
-// stored as zero, so that they take a zero multiplier.  (Rows below the panel's last row hold
-// zeros from the start; the arg-max and the write-back skip them.)
+// update reads the rows' multipliers from there as broadcasts.  Rows whose label is <= thr
+// (already pivotal) are stored as zero, so that they take a zero multiplier.  (Rows below the
+// panel's last row hold zeros from the start; the arg-max and the write-back skip them.)
 template <typename T, int RW>
 __device__ __forceinline__ void column_to_lds(const T (&v0)[RW], const T (&v1)[RW], int c,
-                                              int lane, int kz, T* colb) {
+                                              int lane, const int (&lbl)[RW], int thr,
+                                              T* colb) {
   if (lane == (c & 63)) {
-    // the row test runs in a vector register (one compare per row); as a scalar it would be
-    // hoisted out of the column loop into one live mask per row
-    const int kzv = opaque(kz);
     T x[RW];
 #pragma unroll
     for (int k = 0; k < RW; ++k) x[k] = c < 64 ? v0[k] : v1[k];
 #pragma unroll
-    for (int k = 0; k < RW; ++k) colb[k] = k >= kzv ? x[k] : T(0);
+    for (int k = 0; k < RW; ++k) colb[k] = lbl[k] > thr ? x[k] : T(0);
   }
 }
 
-// Arg-max of |colb[k]| over k in [kmin, kmax) across the wave: first index wins ties, a NaN
-// entry counts as -1 (below every number), -2 / INT_MAX when the range is empty.  Row of k:
-// gr0 + NW k.  Every lane ends with the result.
-template <typename T, int RW, int NW>
-__device__ __forceinline__ void wave_argmax(const T* colb, int kmin, int kmax, int gr0, int lane,
-                                            double& key, int& row) {
+// Arg-max of |colb[k]| over the wave's candidate rows (k < kmax, label >= i): the largest
+// value wins and the smallest label breaks ties (the first index in the row order of the
+// reference, whose rows are physically interchanged), a NaN entry counts as -1 (below every
+// number); key -2 and INT_MAX when the wave has no candidate.  Labels are packed as
+// label * 256 + workgroup.  Every lane ends with the key, the packed label, the slot k of the
+// winner and the slot of the row labelled i (-1 when the wave does not hold it).
+template <typename T, int RW>
+__device__ __forceinline__ void wave_argmax(const T* colb, const int* lblw, int i, int kmax, int w,
+                                            int lane, double& key, int& pk, int& kbest,
+                                            int& kdiag) {
   // the column was written by one lane of this wave: wait for the LDS writes (in-order
   // LDS queue per wave), and keep the compiler from hoisting the reads above them
   __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
   __builtin_amdgcn_wave_barrier();
   const T x = lane < RW ? colb[lane] : T(0);
-  T a = (lane >= kmin && lane < kmax) ? fmax(fabs(x), T(-1)) : T(-2);
-  int k = lane;
+  const int L = lane < RW ? lblw[lane] : INT_MAX;
+  const bool cand = lane < kmax && L >= i;
+  T a = cand ? fmax(fabs(x), T(-1)) : T(-2);
+  const int mine = cand ? L * 256 + w : INT_MAX;
+  int k = mine;
   wave_argmax_dpp(a, k);
   key = static_cast<double>(a);
-  row = a > T(-2) ? gr0 + NW * k : INT_MAX;
+  pk = k;
+  const unsigned long long bw = __ballot(cand && mine == k);
+  kbest = bw ? __ffsll(bw) - 1 : -1;
+  const unsigned long long bd = __ballot(lane < kmax && L == i);
+  kdiag = bd ? __ffsll(bd) - 1 : -1;
 }
 
-// Step i of the rank-1 update on one wave's rows gr = gr0 + NW k: the multiplier of row gr is
-// its column-i entry times 1/pivot, stored in place; columns > i take fma(-l, u, a) (GeR),
-// columns < i keep their values (u is zero there, so the fma returns them unchanged).  The
-// column-i entries come from colb, where the rows outside (i, m) hold zero (the row at i was
-// cleared by the interchange), so every lane forms the same multipliers from LDS broadcasts
-// and a row outside (i, m) takes multiplier 0 without a per-row test; only the in-place store
-// in the lane of column i is limited to the rows k >= kmin, by one compare per row against a
-// lane value.  Column i + 1 of the rows then goes to LDS for the next arg-max.
-// HI: column i is in the upper half (v1).
-template <typename T, int RW, int NW, bool HI>
-__device__ __forceinline__ void panel_update(T (&v0)[RW], T (&v1)[RW], T* colb, int i,
-                                             int gr0, T rinv, T u0, T u1, int lane) {
-  // rows k >= kmin lie below row i
-  // (a negative numerator truncates to a value <= 0, which the clamp turns into 0)
-  const int kmin = min(max((i + NW - gr0) / NW, 0), RW);
+// Step i of the rank-1 update on one wave's rows: the multiplier of a row is its column-i
+// entry times 1/pivot, stored in place; columns > i take fma(-l, u, a) (GeR), columns < i keep
+// their values (u is zero there, so the fma returns them unchanged).  The column-i entries
+// come from colb, where the rows that are already pivotal and the pivot row of this step hold
+// zero, so every lane forms the same multipliers from LDS broadcasts and those rows take
+// multiplier 0 without a per-row test; only the in-place store in the lane of column i is
+// limited to the rows labelled > i, by one compare per row against a lane value.  Column i + 1
+// of the rows then goes to LDS for the next arg-max.  HI: column i is in the upper half (v1).
+template <typename T, int RW, bool HI>
+__device__ __forceinline__ void panel_update(T (&v0)[RW], T (&v1)[RW], T* colb, const int* lblw,
+                                             int i, T rinv, T u0, T u1, int lane) {
   const int cl = HI ? i - 64 : i;  // lane of column i
-  const int q = lane == cl ? kmin : RW;
+  const int q = lane == cl ? i : INT_MAX;
+  int lbl[RW];
   T l[RW];
 #pragma unroll
-  for (int k = 0; k < RW; ++k) l[k] = colb[k] * rinv;
+  for (int k = 0; k < RW; ++k) {
+    lbl[k] = lblw[k];
+    l[k] = colb[k] * rinv;
+  }
   if constexpr (!HI) {
     const T z0 = lane > i ? u0 : T(0);
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const T n0 = fma(-l[k], z0, v0[k]);
       v1[k] = fma(-l[k], u1, v1[k]);  // every column >= 64 > i
-      v0[k] = k >= q ? l[k] : n0;
+      v0[k] = lbl[k] > q ? l[k] : n0;
     }
   } else {
     const T z1 = lane > cl ? u1 : T(0);
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const T n1 = fma(-l[k], z1, v1[k]);
-      v1[k] = k >= q ? l[k] : n1;
+      v1[k] = lbl[k] > q ? l[k] : n1;
     }
   }
-  // rows k >= kmin (gr > i) are the candidates of step i + 1
-  column_to_lds(v0, v1, i + 1, lane, kmin, colb);
+  // rows labelled <= i are pivotal from now on
+  column_to_lds(v0, v1, i + 1, lane, lbl, i, colb);
 }
 
 template <typename T, int R, int NW>
@@ -238,15 +248,12 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
                                                           PanelSync sy, unsigned base,
                                                           int info_base) {
   constexpr int RW = R / NW;  // rows per wave
-  constexpr int NT = NW * 64;
-  __shared__ T prow[NBX];
-  __shared__ T drow[NBX];
-  __shared__ int orgs[R];
-  __shared__ T colb[NW][RW];  // per wave: the next column of its rows
-  // [0]: per-wave local arg-max, [1]: per-wave share of the global winner (separate slots:
-  // a fast wave may publish its global share while a slow one still reads the local ones)
-  __shared__ double sk[2][NW];
-  __shared__ int sr[2][NW];
+  __shared__ T prow[NB_MAX];
+  __shared__ __attribute__((aligned(16))) T colb[NW][RW];    // per wave: the next column
+  __shared__ __attribute__((aligned(16))) int lblw[NW][RW];  // per wave: labels of its rows
+  __shared__ double sk[NW];  // per wave: local arg-max key, packed label, slot
+  __shared__ int sr[NW], ss[NW];
+  __shared__ int swin;
 
   const int W = gridDim.x;
   const int w = blockIdx.x;
@@ -265,73 +272,74 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
 
   T v0[RW], v1[RW];
   const int c0 = lane, c1 = lane + 64;
+  const int gr0 = r0 + wave;
 #pragma unroll
   for (int k = 0; k < RW; ++k) {
-    const int gr = r0 + wave + NW * k;
+    const int gr = gr0 + NW * k;
     const bool ok = gr < m;
     v0[k] = (ok && c0 < nb) ? P[(int64_t)gr * ld + c0] : T(0);
     v1[k] = (ok && c1 < nb) ? P[(int64_t)gr * ld + c1] : T(0);
   }
-  const int gr0 = r0 + wave;
   const int kmax = min(max((m + NW - 1 - gr0) / NW, 0), RW);  // rows k < kmax lie below m
-  column_to_lds(v0, v1, 0, lane, 0, colb[wave]);
-  for (int t = tid; t < R; t += NT) orgs[t] = r0 + t;
+  {
+    int lbl[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) lbl[k] = gr0 + NW * k;
+    if (lane < RW) lblw[wave][lane] = gr0 + NW * lane;
+    column_to_lds(v0, v1, 0, lane, lbl, -1, colb[wave]);
+  }
 
   for (int i = 0; i < nb; ++i) {
     const int par = i & 1;
     const unsigned tag = base + static_cast<unsigned>(i) + 1u;
-    const int ci = i & 63;
-    const bool hi = i >= 64;
-    // 1. local arg-max of column i over the rows >= i: per wave, then across waves
+    // 1. local arg-max of column i over the rows not yet pivotal: per wave, then across waves
     double key;
-    int row;
+    int pk, kd;
     {
-      const int kmin = min(max((i + NW - 1 - gr0) / NW, 0), RW);  // rows k >= kmin: gr >= i
-      wave_argmax<T, RW, NW>(colb[wave], kmin, kmax, gr0, lane, key, row);
+      int kb;
+      wave_argmax<T, RW>(colb[wave], lblw[wave], i, kmax, w, lane, key, pk, kb, kd);
       if (lane == 0) {
-        sk[0][wave] = key;
-        sr[0][wave] = row;
+        sk[wave] = key;
+        sr[wave] = pk;
+        ss[wave] = kb;
       }
     }
+    kd = __builtin_amdgcn_readfirstlane(kd);
     __syncthreads();
-    key = sk[0][0];
-    row = sr[0][0];
+    key = sk[0];
+    pk = sr[0];
+    int qb = 0;
 #pragma unroll
-    for (int q = 1; q < NW; ++q) better(key, row, sk[0][q], sr[0][q]);
-    row = __builtin_amdgcn_readfirstlane(row);
-    // 2. publish the candidate row, the diagonal row and the key
-    if (key > -2.0 && wave == (row - r0) % NW) {
-      const int ks = (row - r0) / NW;
-      u64* my = sy.cand + ((size_t)par * WMAX + w) * NBX * 2;
-      put_elem<T>(my + 2 * c0, reg_get(v0, ks), tag);
-      put_elem<T>(my + 2 * c1, reg_get(v1, ks), tag);
-      if (lane == 0) put_elem<T>(my + 2 * NB_MAX, static_cast<T>(orgs[row - r0]), tag);
+    for (int q = 1; q < NW; ++q) {
+      const double k2 = sk[q];
+      const int p2 = sr[q];
+      if (k2 > key || (k2 == key && p2 < pk)) {
+        key = k2;
+        pk = p2;
+        qb = q;
+      }
     }
-    const bool own_i = (i >= r0 && i < r0 + R);
-    if (own_i && wave == (i - r0) % NW) {
-      const int kd = (i - r0) / NW;
-      u64* dg = sy.diag + (size_t)par * NBX * 2;
-      const T d0 = reg_get(v0, kd), d1 = reg_get(v1, kd);
-      const T dor = static_cast<T>(orgs[i - r0]);
-      put_elem<T>(dg + 2 * c0, d0, tag);
-      put_elem<T>(dg + 2 * c1, d1, tag);
-      if (lane == 0) put_elem<T>(dg + 2 * NB_MAX, dor, tag);
-      drow[c0] = d0;
-      drow[c1] = d1;
-      if (lane == 0) drow[NB_MAX] = dor;
+    pk = __builtin_amdgcn_readfirstlane(pk);
+    qb = __builtin_amdgcn_readfirstlane(qb);
+    const int kc = __builtin_amdgcn_readfirstlane(ss[qb]);  // slot of the candidate in wave qb
+    // 2. publish the candidate row and the key
+    if (key > -2.0 && wave == qb) {
+      u64* my = sy.cand + ((size_t)par * WMAX + w) * NB_MAX * 2;
+      put_elem<T>(my + 2 * c0, reg_get(v0, kc), tag);
+      put_elem<T>(my + 2 * c1, reg_get(v1, kc), tag);
     }
     if (tid == 0) {
       u64* kp = sy.keys + ((size_t)par * WMAX + w) * KSTRIDE;
       const u64 kb = __builtin_bit_cast(u64, key);
       put(kp, static_cast<unsigned>(kb >> 32), tag);
       put(kp + 1, static_cast<unsigned>(kb), tag);
-      put(kp + 2, static_cast<unsigned>(row), tag);
+      put(kp + 2, static_cast<unsigned>(pk), tag);
     }
     // 3. global winner (identical in every workgroup): wave 0 reads the W slots, lane q the
     //    slots q, q + 64, ...; all requests of a lane are in flight together
     if (wave == 0) {
-      key = -2.0;
-      row = INT_MAX;
+      double gk = -2.0;
+      int gp = INT_MAX;
       constexpr int SPL = WMAX / 64;
       u64 a[SPL], b[SPL], c[SPL];
       for (;;) {
@@ -358,78 +366,60 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
 #pragma unroll
       for (int j = 0; j < SPL; ++j) {
         if (lane + 64 * j < W)
-          better(key, row, __builtin_bit_cast(double, ((a[j] >> 32) << 32) | (b[j] >> 32)),
+          better(gk, gp, __builtin_bit_cast(double, ((a[j] >> 32) << 32) | (b[j] >> 32)),
                  static_cast<int>(c[j] >> 32));
       }
-      wave_argmax_dpp(key, row);  // DPP across the lanes (every lane ends with the winner)
-      // the winner's row (and the diagonal row for the workgroup that gives its row up),
-      // read by this wave right away, to LDS for the others
-      row = __builtin_amdgcn_readfirstlane(row);
-      const int wwin = row / R;
-      const bool own_p = (row != i) && (row >= r0 && row < r0 + R);
-      const u64* cw = sy.cand + ((size_t)par * WMAX + wwin) * NBX * 2;
-      const u64* dg = sy.diag + (size_t)par * NBX * 2;
-      // the origin column (index NB_MAX) sits right after the row, so NBX elements cover
-      // columns [0, nb) of any width and the origin
-      fetch_rows<T, (NBX + 63) / 64>(cw, (own_p && !own_i) ? dg : nullptr, NBX, lane, tag, prow,
-                                     drow);
-      if (lane == 0) sr[1][0] = row;
+      wave_argmax_dpp(gk, gp);  // DPP across the lanes (every lane ends with the winner)
+      gp = __builtin_amdgcn_readfirstlane(gp);
+      // the winner's row, read by this wave right away, to LDS for the others
+      const u64* cw = sy.cand + ((size_t)par * WMAX + (gp & 255)) * NB_MAX * 2;
+      fetch_rows<T, NB_MAX / 64>(cw, nullptr, NB_MAX, lane, tag, prow, prow);
+      if (lane == 0) swin = gp;
     }
-    __syncthreads();  // prow / drow / the winner in LDS
-    row = __builtin_amdgcn_readfirstlane(sr[1][0]);
-    const bool own_p = (row != i) && (row >= r0 && row < r0 + R);
+    __syncthreads();  // prow and the winner in LDS
+    const int gp = __builtin_amdgcn_readfirstlane(swin);
+    const int p = gp >> 8;  // label of the pivot row
     const T pv = prow[i];
-    if (w == 0 && tid == 0) piv[i] = row - i;
+    if (w == 0 && tid == 0) piv[i] = p - i;
     const bool stop = (pv == static_cast<T>(0));
     if (stop && w == 0 && tid == 0) atomicCAS(info, 0, info_base + i + 1);
-    // 4. interchange rows i and `row` (register rows, selected with uniform indices)
-    //    colb follows: the pivot row at i takes no multiplier (zero), the row moved to
-    //    `row` brings its column-i entry
-    const T u0 = prow[c0], u1 = prow[c1];
-    if (own_i && wave == (i - r0) % NW) {
-      const int kd = (i - r0) / NW;
-      if (row != i) {
-        reg_set(v0, kd, u0);
-        reg_set(v1, kd, u1);
-        if (lane == 0) orgs[i - r0] = static_cast<int>(prow[NB_MAX]);
-      }
-      if (lane == 0) colb[wave][kd] = T(0);
+    // 4. interchange rows i and p by their labels (no data moves): the pivot row takes label
+    //    i and, from now on, no multiplier; the row labelled i takes label p
+    if (w == (gp & 255) && wave == qb && lane == 0) {
+      lblw[wave][kc] = i;
+      colb[wave][kc] = T(0);
     }
-    if (own_p && wave == (row - r0) % NW) {
-      const int kp = (row - r0) / NW;
-      reg_set(v0, kp, drow[c0]);
-      reg_set(v1, kp, drow[c1]);
-      if (lane == 0) {
-        orgs[row - r0] = static_cast<int>(drow[NB_MAX]);
-        colb[wave][kp] = drow[i];
-      }
-    }
+    if (kd >= 0 && p != i && lane == 0) lblw[wave][kd] = p;
     if (stop) break;
     // 5. scale column i below the pivot, update the rest of every row and track the
     //    arg-max of column i + 1 over the updated rows
     const T rinv = static_cast<T>(1) / pv;
-    if (!hi) {
-      panel_update<T, RW, NW, false>(v0, v1, colb[wave], i, gr0, rinv, u0, u1, lane);
+    const T u0 = prow[c0], u1 = prow[c1];
+    if (i < 64) {
+      panel_update<T, RW, false>(v0, v1, colb[wave], lblw[wave], i, rinv, u0, u1, lane);
     } else {
-      panel_update<T, RW, NW, true>(v0, v1, colb[wave], i, gr0, rinv, u0, u1, lane);
+      panel_update<T, RW, true>(v0, v1, colb[wave], lblw[wave], i, rinv, u0, u1, lane);
     }
   }
 
-  // write back and emit the row movement list (dst <- src, panel-relative)
+  // write every row to the position of its label and emit the row movement list
+  // (dst <- src, panel-relative) for the rest of the matrix
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's label writes
+  __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int k = 0; k < RW; ++k) {
-    const int gr = r0 + wave + NW * k;
-    if (gr < m) {
-      if (c0 < nb) P[(int64_t)gr * ld + c0] = v0[k];
-      if (c1 < nb) P[(int64_t)gr * ld + c1] = v1[k];
+    if (k < kmax) {
+      const int dst = lblw[wave][k];
+      if (c0 < nb) P[(int64_t)dst * ld + c0] = v0[k];
+      if (c1 < nb) P[(int64_t)dst * ld + c1] = v1[k];
     }
   }
-  __syncthreads();  // orgs written by the interchanges of the last step
-  for (int t = tid; t < R; t += NT) {
-    if (r0 + t < m && orgs[t] != r0 + t) {
+  if (lane < kmax) {
+    const int src = gr0 + NW * lane, dst = lblw[wave][lane];
+    if (dst != src) {
       const int slot = atomicAdd(sy.list_count, 1);
-      sy.list[2 * slot] = r0 + t;
-      sy.list[2 * slot + 1] = orgs[t];
+      sy.list[2 * slot] = dst;
+      sy.list[2 * slot + 1] = src;
     }
   }
 }
@@ -604,10 +594,9 @@ bool fits(int64_t m, int limit) {
 // exchange area for panel kernels
 int panel_sync(PanelSync* sy, unsigned** base_counter) {
   const size_t kb = 2 * (size_t)WMAX * KSTRIDE * sizeof(u64);
-  const size_t cb = 2 * (size_t)WMAX * NBX * 2 * sizeof(u64);
-  const size_t db = 2 * (size_t)NBX * 2 * sizeof(u64);
+  const size_t cb = 2 * (size_t)WMAX * NB_MAX * 2 * sizeof(u64);
   const size_t lb = 4 * (size_t)NB_MAX * sizeof(int) + 256;
-  const size_t bytes = kb + cb + db + lb;
+  const size_t bytes = kb + cb + lb;
   void* ws = nullptr;
   static void* zeroed_ptr = nullptr;
   DDLA_RETURN_IF(workspace(WS_PANEL, bytes, &ws));
@@ -618,8 +607,7 @@ int panel_sync(PanelSync* sy, unsigned** base_counter) {
   char* p = static_cast<char*>(ws);
   sy->keys = reinterpret_cast<u64*>(p);
   sy->cand = reinterpret_cast<u64*>(p + kb);
-  sy->diag = reinterpret_cast<u64*>(p + kb + cb);
-  sy->list = reinterpret_cast<int*>(p + kb + cb + db);
+  sy->list = reinterpret_cast<int*>(p + kb + cb);
   sy->list_count = sy->list + 4 * NB_MAX;
   static unsigned counter = 0;
   *base_counter = &counter;
