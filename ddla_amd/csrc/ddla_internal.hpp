@@ -186,6 +186,9 @@ int launch_tslu_panel(int64_t m, int nb, float* P, int64_t ld, int32_t* info, in
 
 // triangular block solves (trsm.hip): op(T) X = B for one bs x bs triangle (bs <= 128)
 template <typename T>
+int launch_laswp_trsm_slab(int bs, int64_t ncols, const T* L, int64_t ldl, T* B, int64_t ldb,
+                           const int* list, const int* count, hipStream_t s);
+template <typename T>
 int launch_trsm_slab(bool lower, bool unit, int bs, int64_t ncols, const T* A, int64_t rs,
                      int64_t cs, T* B, int64_t ldb, hipStream_t s);
 // op(A) X = B, A n x n row-major; lower/upper name the stored triangle of A

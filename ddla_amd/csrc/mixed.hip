@@ -751,14 +751,16 @@ int mixed_ws(int64_t n, int64_t np, int64_t nrhs, MixedWs* w) {
 // One trailing update of the mixed LU: U12 = L11^-1 A12 for columns [c0, c0 + nc) of the row
 // block, then A22 -= L21 U12 on those columns (FP32, or FP16 / BF16 matrix cores).
 // l21: the panel's L21 already split (FP16 / BF16), or null to split it here.
+// u12_ready: U12 was already formed (launch_laswp_trsm_slab).
 int lu_update_cols(int prec, int terms, int64_t n, int64_t j, int b, int64_t c0, int64_t nc,
                    float* a, int64_t lda, float* scales, hipStream_t s, int split_slot,
-                   const SplitOp* l21) {
+                   const SplitOp* l21, bool u12_ready = false) {
   if (nc <= 0) return 0;
   float* panel = a + j * lda + j;
   const int64_t rest = n - j - b;
   float* u12 = a + j * lda + c0;
-  DDLA_RETURN_IF(launch_trsm_slab<float>(true, true, b, nc, panel, lda, 1, u12, lda, s));
+  if (!u12_ready)
+    DDLA_RETURN_IF(launch_trsm_slab<float>(true, true, b, nc, panel, lda, 1, u12, lda, s));
   if (rest <= 0) return 0;
   float* c = a + (j + b) * lda + c0;
   if (l21 && prec != DDLA_PREC_FP32) {
@@ -838,14 +840,15 @@ int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, Mi
     }
     // main stream: swaps on the next block (and on [0, j) when no far columns are left), its
     // update (FP32)
+    // (with far columns left, the next block's swaps and its U12 are one launch)
     if (far > 0)
-      DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda + j + b, lda, 0, nxt, 0, 0, list_k,
-                                              counts + k, 2 * NBLK, s));
+      DDLA_RETURN_IF(launch_laswp_trsm_slab<float>(b, nxt, panel, lda, a + j * lda + j + b, lda,
+                                                   list_k, counts + k, s));
     else
       DDLA_RETURN_IF(launch_laswp_list<float>(a + j * lda, lda, 0, j + b + nxt, j, j + b, list_k,
                                               counts + k, 2 * NBLK, s));
     DDLA_RETURN_IF(lu_update_cols(DDLA_PREC_FP32, w.terms, n, j, b, j + b, nxt, a, lda,
-                                  w.scales, s, WS_SPLIT, nullptr));
+                                  w.scales, s, WS_SPLIT, nullptr, far > 0));
     // auxiliary stream: swaps on [0, j) and the far columns (off the critical path: nothing
     // reads the finished columns until the end), update of the far columns
     if (far > 0) {

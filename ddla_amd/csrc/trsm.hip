@@ -28,24 +28,45 @@ constexpr int TB = 128;  // diagonal block size
 // subtracts t_qk x_k with one fma: per column the FMA chain in k order of a row-by-row
 // solve.  The strictly lower couplings sit in LDS as Mt[q][k] (stride TB + 1: the column
 // reads of one step hit distinct banks), zero where q <= k.
-template <typename T, bool LOWER, bool UNIT, int CW>
+//
+// PIV (unit lower only): B first takes a panel's row interchanges, given as the (dst <- src)
+// list of laswp_list_kernel with rows relative to B's first row: the rows of the block are
+// read from their sources, and the rows that leave the block (dst >= bs; their sources all
+// lie inside the block) are written to their destinations by the wave that owns the columns,
+// before the block itself is overwritten.
+template <typename T, bool LOWER, bool UNIT, int CW, bool PIV = false>
 __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
                                                         const T* __restrict__ A, int64_t rs,
                                                         int64_t cs, T* __restrict__ B,
-                                                        int64_t ldb) {
+                                                        int64_t ldb,
+                                                        const int* __restrict__ list = nullptr,
+                                                        const int* __restrict__ count = nullptr) {
   __shared__ T Mt[TB][TB + 1];
   __shared__ T dg[TB];
+  __shared__ int srcq[PIV ? TB : 1];
+  __shared__ int lst[PIV ? 4 * TB : 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   auto row_of = [&](int q) { return LOWER ? q : bs - 1 - q; };
   const int64_t c0 = ((int64_t)blockIdx.x * 4 + wave) * CW;
+  int cnt = 0;
+  if constexpr (PIV) {
+    cnt = min(*count, 2 * TB);
+    for (int t = tid; t < TB; t += 256) srcq[t] = t;
+    for (int e = tid; e < 2 * cnt; e += 256) lst[e] = list[e];
+    __syncthreads();
+    for (int e = tid; e < cnt; e += 256)
+      if (lst[2 * e] < bs) srcq[lst[2 * e]] = lst[2 * e + 1];
+    __syncthreads();
+  }
   T x0[CW], x1[CW];  // positions lane and lane + 64 (read first: in flight during the staging)
   const int r0 = lane < bs ? row_of(lane) : 0, r1 = lane + 64 < bs ? row_of(lane + 64) : 0;
+  const int s0 = PIV ? srcq[r0] : r0, s1 = PIV ? srcq[r1] : r1;
 #pragma unroll
   for (int c = 0; c < CW; ++c) {
     const bool in = c0 + c < ncols;
     const bool i0 = in && lane < bs, i1 = in && lane + 64 < bs;
-    x0[c] = B[i0 ? (int64_t)r0 * ldb + c0 + c : 0];
-    x1[c] = B[i1 ? (int64_t)r1 * ldb + c0 + c : 0];
+    x0[c] = B[i0 ? (int64_t)s0 * ldb + c0 + c : 0];
+    x1[c] = B[i1 ? (int64_t)s1 * ldb + c0 + c : 0];
   }
   const bool kfast = cs == 1;  // consecutive threads read consecutive addresses
   // Staging in batches of 32 (FP64: 16) reads per thread, issued back to back (branch-free: an
@@ -78,6 +99,17 @@ __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
   }
   __syncthreads();
   if (c0 >= ncols) return;  // after the barrier: every wave helped stage the triangle
+  if constexpr (PIV) {
+    // every read of the block's sources is complete (their values were consumed above)
+    for (int e = lane; e < cnt; e += 64) {
+      const int d = lst[2 * e], sr = lst[2 * e + 1];
+      if (d >= bs) {
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+          if (c0 + c < ncols) B[(int64_t)d * ldb + c0 + c] = B[(int64_t)sr * ldb + c0 + c];
+      }
+    }
+  }
   const int h = min(bs, 64);
 #pragma unroll 2
   for (int k = 0; k < h; ++k) {
@@ -208,6 +240,35 @@ template int trsm_left<double>(bool, bool, bool, int64_t, int64_t, const double*
                                double*, int64_t, hipStream_t);
 template int trsm_left<float>(bool, bool, bool, int64_t, int64_t, const float*, int64_t, float*,
                               int64_t, hipStream_t);
+// The row interchanges of a panel (list, *count pairs) followed by the unit-lower solve
+// L11^-1 on the bs x ncols block B (rows relative to the panel's first row); replaces
+// laswp_list + trsm_slab on the same columns with one launch.
+template <typename T>
+int launch_laswp_trsm_slab(int bs, int64_t ncols, const T* L, int64_t ldl, T* B, int64_t ldb,
+                           const int* list, const int* count, hipStream_t s) {
+  if (ncols <= 0 || bs <= 0) return 0;
+  if (bs > TB) {
+    set_error("trsm block %d > %d", bs, TB);
+    return DDLA_ERR_INVALID_ARG;
+  }
+  ProfScope prof(sizeof(T) == 8 ? "laswp_trsm_slab_f64" : "laswp_trsm_slab_f32",
+                 (double)bs * bs * ncols, sizeof(T) * (2.0 * bs * ncols + 0.5 * bs * bs), s);
+  const bool wide = ncols >= 4096;
+  const dim3 grid((unsigned)ceil_div(ncols, 4 * (wide ? 8 : 4))), block(256);
+  if (wide)
+    hipLaunchKernelGGL((trsm_slab_kernel<T, true, true, 8, true>), grid, block, 0, s, bs, ncols,
+                       L, ldl, (int64_t)1, B, ldb, list, count);
+  else
+    hipLaunchKernelGGL((trsm_slab_kernel<T, true, true, 4, true>), grid, block, 0, s, bs, ncols,
+                       L, ldl, (int64_t)1, B, ldb, list, count);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template int launch_laswp_trsm_slab<double>(int, int64_t, const double*, int64_t, double*,
+                                            int64_t, const int*, const int*, hipStream_t);
+template int launch_laswp_trsm_slab<float>(int, int64_t, const float*, int64_t, float*, int64_t,
+                                           const int*, const int*, hipStream_t);
 template int launch_trsm_slab<double>(bool, bool, int, int64_t, const double*, int64_t, int64_t,
                                       double*, int64_t, hipStream_t);
 template int launch_trsm_slab<float>(bool, bool, int, int64_t, const float*, int64_t, int64_t,
