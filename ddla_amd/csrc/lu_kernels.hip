@@ -128,6 +128,41 @@ __device__ __forceinline__ void fetch_rows(const u64* src, const u64* src2, int 
   }
 }
 
+// Polls the W key slots (3 tagged words each, slot q at keys + q KSTRIDE) until all carry
+// `tag`, and reduces them per lane into (k, r).  Lane l reads slots l, l + 64, ... (S per lane);
+// all reads of a round are issued before any tag is checked, and they are unconditional (a
+// lane's unused slots read slot W - 1 and are not checked): reads under a per-slot branch
+// would each be merged into fresh registers and waited for one group at a time.
+template <int S>
+__device__ __forceinline__ void poll_keys(const u64* keys, int W, int lane, unsigned tag,
+                                          double& k, int& r) {
+  u64 a[S], b[S], c[S];
+  for (;;) {
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const u64* kp = keys + (size_t)min(lane + 64 * j, W - 1) * KSTRIDE;
+      a[j] = get(kp);
+      b[j] = get(kp + 1);
+      c[j] = get(kp + 2);
+    }
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+      ok = ok && (lane + 64 * j >= W || (static_cast<unsigned>(a[j]) == tag &&
+                                         static_cast<unsigned>(b[j]) == tag &&
+                                         static_cast<unsigned>(c[j]) == tag));
+    if (ok) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  k = -2.0;
+  r = INT_MAX;
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+    if (lane + 64 * j < W)
+      better(k, r, __builtin_bit_cast(double, ((a[j] >> 32) << 32) | (b[j] >> 32)),
+             static_cast<int>(c[j] >> 32));
+}
+
 // Selects element k of a register array with a uniform run-time index (unrolled compares,
 // no dynamic register indexing).  The empty asm hides each element's origin: otherwise
 // the select chain over array loads folds into one load at a variable address, and the
@@ -338,37 +373,18 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     // 3. global winner (identical in every workgroup): wave 0 reads the W slots, lane q the
     //    slots q, q + 64, ...; all requests of a lane are in flight together
     if (wave == 0) {
-      double gk = -2.0;
-      int gp = INT_MAX;
-      constexpr int SPL = WMAX / 64;
-      u64 a[SPL], b[SPL], c[SPL];
-      for (;;) {
-        bool ok = true;
-#pragma unroll
-        for (int j = 0; j < SPL; ++j) {
-          const int q = lane + 64 * j;
-          if (q < W) {
-            const u64* kp = sy.keys + ((size_t)par * WMAX + q) * KSTRIDE;
-            a[j] = get(kp);
-            b[j] = get(kp + 1);
-            c[j] = get(kp + 2);
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < SPL; ++j) {
-          if (lane + 64 * j < W)
-            ok = ok && static_cast<unsigned>(a[j]) == tag && static_cast<unsigned>(b[j]) == tag &&
-                 static_cast<unsigned>(c[j]) == tag;
-        }
-        if (ok) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-#pragma unroll
-      for (int j = 0; j < SPL; ++j) {
-        if (lane + 64 * j < W)
-          better(gk, gp, __builtin_bit_cast(double, ((a[j] >> 32) << 32) | (b[j] >> 32)),
-                 static_cast<int>(c[j] >> 32));
-      }
+      double gk;
+      int gp;
+      const u64* ks = sy.keys + (size_t)par * WMAX * KSTRIDE;
+      const int nj = (W + 63) >> 6;
+      if (nj <= 1)
+        poll_keys<1>(ks, W, lane, tag, gk, gp);
+      else if (nj == 2)
+        poll_keys<2>(ks, W, lane, tag, gk, gp);
+      else if (nj == 3)
+        poll_keys<3>(ks, W, lane, tag, gk, gp);
+      else
+        poll_keys<4>(ks, W, lane, tag, gk, gp);
       wave_argmax_dpp(gk, gp);  // DPP across the lanes (every lane ends with the winner)
       gp = __builtin_amdgcn_readfirstlane(gp);
       // the winner's row, read by this wave right away, to LDS for the others
