@@ -227,7 +227,7 @@ __device__ __forceinline__ void wave_argmax(const T* colb, const int* lblw, int 
   T a = cand ? fmax(fabs(x), T(-1)) : T(-2);
   const int mine = cand ? L * 256 + w : INT_MAX;
   int k = mine;
-  wave_argmax_dpp(a, k);
+  lanes_argmax_dpp<RW>(a, k);  // the rows sit in lanes [0, RW)
   key = static_cast<double>(a);
   pk = k;
   const unsigned long long bw = __ballot(cand && mine == k);
@@ -341,21 +341,18 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     }
     kd = __builtin_amdgcn_readfirstlane(kd);
     __syncthreads();
-    key = sk[0];
-    pk = sr[0];
-    int qb = 0;
-#pragma unroll
-    for (int q = 1; q < NW; ++q) {
-      const double k2 = sk[q];
-      const int p2 = sr[q];
-      if (k2 > key || (k2 == key && p2 < pk)) {
-        key = k2;
-        pk = p2;
-        qb = q;
-      }
+    // the waves' candidates in lanes [0, NW), reduced by DPP; qb = the wave that holds the
+    // winner (its packed label is unique), kc = its slot there
+    int qb;
+    {
+      const int pq = lane < NW ? sr[lane] : INT_MAX;
+      key = lane < NW ? sk[lane] : -2.0;
+      pk = pq;
+      lanes_argmax_dpp<NW>(key, pk);
+      pk = __builtin_amdgcn_readfirstlane(pk);
+      const unsigned long long bq = __ballot(lane < NW && pq == pk);
+      qb = __builtin_amdgcn_readfirstlane(bq ? __ffsll(bq) - 1 : 0);
     }
-    pk = __builtin_amdgcn_readfirstlane(pk);
-    qb = __builtin_amdgcn_readfirstlane(qb);
     const int kc = __builtin_amdgcn_readfirstlane(ss[qb]);  // slot of the candidate in wave qb
     // 2. publish the candidate row and the key
     if (key > -2.0 && wave == qb) {

@@ -46,19 +46,27 @@ __device__ __forceinline__ void dpp_better(T& a, int& k) {
   k = b ? k2 : k;
 }
 
-// Arg-max of (a, k) over the 64 lanes (largest a, smallest k among equal a): Hillis-Steele
-// over DPP row shifts, then the gfx9 row broadcasts; every lane ends with lane 63's result.
-// Lanes with nothing to offer pass (-2, any index).
+// Arg-max of (a, k) over lanes [0, NL) (NL a power of two <= 64; largest a, smallest k among
+// equal a): Hillis-Steele over DPP row shifts, then the gfx9 row broadcasts, only as many
+// steps as NL needs; every lane ends with lane NL - 1's result.  Lanes with nothing to offer
+// pass (-2, any index).
+template <int NL, typename T>
+__device__ __forceinline__ void lanes_argmax_dpp(T& a, int& k) {
+  static_assert(NL >= 1 && NL <= 64 && (NL & (NL - 1)) == 0, "NL: a power of two <= 64");
+  if constexpr (NL > 1) dpp_better<0x111, 0xf>(a, k);   // row_shr:1
+  if constexpr (NL > 2) dpp_better<0x112, 0xf>(a, k);   // row_shr:2
+  if constexpr (NL > 4) dpp_better<0x114, 0xf>(a, k);   // row_shr:4
+  if constexpr (NL > 8) dpp_better<0x118, 0xf>(a, k);   // row_shr:8
+  if constexpr (NL > 16) dpp_better<0x142, 0xa>(a, k);  // row_bcast:15
+  if constexpr (NL > 32) dpp_better<0x143, 0xc>(a, k);  // row_bcast:31
+  a = read_lane(a, NL - 1);
+  k = __builtin_amdgcn_readlane(k, NL - 1);
+}
+
+// The same over all 64 lanes.
 template <typename T>
 __device__ __forceinline__ void wave_argmax_dpp(T& a, int& k) {
-  dpp_better<0x111, 0xf>(a, k);  // row_shr:1
-  dpp_better<0x112, 0xf>(a, k);  // row_shr:2
-  dpp_better<0x114, 0xf>(a, k);  // row_shr:4
-  dpp_better<0x118, 0xf>(a, k);  // row_shr:8
-  dpp_better<0x142, 0xa>(a, k);  // row_bcast:15
-  dpp_better<0x143, 0xc>(a, k);  // row_bcast:31
-  a = read_lane(a, 63);
-  k = __builtin_amdgcn_readlane(k, 63);
+  lanes_argmax_dpp<64>(a, k);
 }
 
 }  // namespace ddla
