@@ -38,7 +38,7 @@ namespace ddla {
 
 struct PanelSync {
   // double-buffered exchange area (one workspace allocation), 64-bit tagged words
-  unsigned long long* keys;  // [2][WMAX][KSTRIDE]: key high, key low, packed label (+ padding)
+  unsigned long long* keys;  // [2][WMAX][KSTRIDE]: key (FP64: high, low), packed label (+ pad)
   unsigned long long* cand;  // [2][WMAX][NB_MAX][2]: candidate row (2 words per element)
   int* list;                 // [2*nb][2] (dst, src) panel-relative
   int* list_count;           // [1]
@@ -52,7 +52,8 @@ constexpr int KSTRIDE = 16;      // one 128-byte line per key slot: readers spre
 
 typedef unsigned long long u64;
 
-__device__ __forceinline__ void better(double& k, int& r, double k2, int r2) {
+template <typename K>
+__device__ __forceinline__ void better(K& k, int& r, K k2, int r2) {
   if (k2 > k || (k2 == k && r2 < r)) {
     k = k2;
     r = r2;
@@ -128,39 +129,47 @@ __device__ __forceinline__ void fetch_rows(const u64* src, const u64* src2, int 
   }
 }
 
-// Polls the W key slots (3 tagged words each, slot q at keys + q KSTRIDE) until all carry
-// `tag`, and reduces them per lane into (k, r).  Lane l reads slots l, l + 64, ... (S per lane);
-// all reads of a round are issued before any tag is checked, and they are unconditional (a
-// lane's unused slots read slot W - 1 and are not checked): reads under a per-slot branch
-// would each be merged into fresh registers and waited for one group at a time.
-template <int S>
-__device__ __forceinline__ void poll_keys(const u64* keys, int W, int lane, unsigned tag,
-                                          double& k, int& r) {
+// Polls the W key slots (slot q at keys + q KSTRIDE: the key in one tagged word for FP32, two
+// for FP64, then the packed label) until all carry `tag`, and reduces them per lane into
+// (k, r).  Lane l reads slots l, l + 64, ... (S per lane); all reads of a round are issued
+// before any tag is checked, and they are unconditional (a lane's unused slots read slot
+// W - 1 and are not checked): reads under a per-slot branch would each be merged into fresh
+// registers and waited for one group at a time.
+template <typename K, int S>
+__device__ __forceinline__ void poll_keys(const u64* keys, int W, int lane, unsigned tag, K& k,
+                                          int& r) {
+  constexpr int EW = sizeof(K) == 8 ? 2 : 1;  // words of the key
   u64 a[S], b[S], c[S];
   for (;;) {
 #pragma unroll
     for (int j = 0; j < S; ++j) {
       const u64* kp = keys + (size_t)min(lane + 64 * j, W - 1) * KSTRIDE;
       a[j] = get(kp);
-      b[j] = get(kp + 1);
-      c[j] = get(kp + 2);
+      if constexpr (EW == 2) b[j] = get(kp + 1);
+      c[j] = get(kp + EW);
     }
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < S; ++j)
-      ok = ok && (lane + 64 * j >= W || (static_cast<unsigned>(a[j]) == tag &&
-                                         static_cast<unsigned>(b[j]) == tag &&
-                                         static_cast<unsigned>(c[j]) == tag));
+      ok = ok && (lane + 64 * j >= W ||
+                  (static_cast<unsigned>(a[j]) == tag && static_cast<unsigned>(c[j]) == tag &&
+                   (EW == 1 || static_cast<unsigned>(b[j]) == tag)));
     if (ok) break;
     __builtin_amdgcn_s_sleep(1);
   }
-  k = -2.0;
+  k = K(-2);
   r = INT_MAX;
 #pragma unroll
-  for (int j = 0; j < S; ++j)
-    if (lane + 64 * j < W)
-      better(k, r, __builtin_bit_cast(double, ((a[j] >> 32) << 32) | (b[j] >> 32)),
-             static_cast<int>(c[j] >> 32));
+  for (int j = 0; j < S; ++j) {
+    if (lane + 64 * j < W) {
+      K kj;
+      if constexpr (EW == 2)
+        kj = __builtin_bit_cast(K, ((a[j] >> 32) << 32) | (b[j] >> 32));
+      else
+        kj = __builtin_bit_cast(K, static_cast<unsigned>(a[j] >> 32));
+      better(k, r, kj, static_cast<int>(c[j] >> 32));
+    }
+  }
 }
 
 // Selects element k of a register array with a uniform run-time index (unrolled compares,
@@ -215,7 +224,7 @@ __device__ __forceinline__ void column_to_lds(const T (&v0)[RW], const T (&v1)[R
 // winner and the slot of the row labelled i (-1 when the wave does not hold it).
 template <typename T, int RW>
 __device__ __forceinline__ void wave_argmax(const T* colb, const int* lblw, int i, int kmax, int w,
-                                            int lane, double& key, int& pk, int& kbest,
+                                            int lane, T& key, int& pk, int& kbest,
                                             int& kdiag) {
   // the column was written by one lane of this wave: wait for the LDS writes (in-order
   // LDS queue per wave), and keep the compiler from hoisting the reads above them
@@ -228,7 +237,7 @@ __device__ __forceinline__ void wave_argmax(const T* colb, const int* lblw, int 
   const int mine = cand ? L * 256 + w : INT_MAX;
   int k = mine;
   lanes_argmax_dpp<RW>(a, k);  // the rows sit in lanes [0, RW)
-  key = static_cast<double>(a);
+  key = a;
   pk = k;
   const unsigned long long bw = __ballot(cand && mine == k);
   kbest = bw ? __ffsll(bw) - 1 : -1;
@@ -286,7 +295,7 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
   __shared__ T prow[NB_MAX];
   __shared__ __attribute__((aligned(16))) T colb[NW][RW];    // per wave: the next column
   __shared__ __attribute__((aligned(16))) int lblw[NW][RW];  // per wave: labels of its rows
-  __shared__ double sk[NW];  // per wave: local arg-max key, packed label, slot
+  __shared__ T sk[NW];  // per wave: local arg-max key, packed label, slot
   __shared__ int sr[NW], ss[NW];
   __shared__ int swin;
 
@@ -328,7 +337,7 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     const int par = i & 1;
     const unsigned tag = base + static_cast<unsigned>(i) + 1u;
     // 1. local arg-max of column i over the rows not yet pivotal: per wave, then across waves
-    double key;
+    T key;
     int pk, kd;
     {
       int kb;
@@ -346,7 +355,7 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     int qb;
     {
       const int pq = lane < NW ? sr[lane] : INT_MAX;
-      key = lane < NW ? sk[lane] : -2.0;
+      key = lane < NW ? sk[lane] : T(-2);
       pk = pq;
       lanes_argmax_dpp<NW>(key, pk);
       pk = __builtin_amdgcn_readfirstlane(pk);
@@ -355,33 +364,38 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     }
     const int kc = __builtin_amdgcn_readfirstlane(ss[qb]);  // slot of the candidate in wave qb
     // 2. publish the candidate row and the key
-    if (key > -2.0 && wave == qb) {
+    if (key > T(-2) && wave == qb) {
       u64* my = sy.cand + ((size_t)par * WMAX + w) * NB_MAX * 2;
       put_elem<T>(my + 2 * c0, reg_get(v0, kc), tag);
       put_elem<T>(my + 2 * c1, reg_get(v1, kc), tag);
     }
     if (tid == 0) {
       u64* kp = sy.keys + ((size_t)par * WMAX + w) * KSTRIDE;
-      const u64 kb = __builtin_bit_cast(u64, key);
-      put(kp, static_cast<unsigned>(kb >> 32), tag);
-      put(kp + 1, static_cast<unsigned>(kb), tag);
-      put(kp + 2, static_cast<unsigned>(pk), tag);
+      if constexpr (sizeof(T) == 8) {
+        const u64 kb = __builtin_bit_cast(u64, key);
+        put(kp, static_cast<unsigned>(kb >> 32), tag);
+        put(kp + 1, static_cast<unsigned>(kb), tag);
+        put(kp + 2, static_cast<unsigned>(pk), tag);
+      } else {
+        put(kp, __builtin_bit_cast(unsigned, key), tag);
+        put(kp + 1, static_cast<unsigned>(pk), tag);
+      }
     }
     // 3. global winner (identical in every workgroup): wave 0 reads the W slots, lane q the
     //    slots q, q + 64, ...; all requests of a lane are in flight together
     if (wave == 0) {
-      double gk;
+      T gk;
       int gp;
       const u64* ks = sy.keys + (size_t)par * WMAX * KSTRIDE;
       const int nj = (W + 63) >> 6;
       if (nj <= 1)
-        poll_keys<1>(ks, W, lane, tag, gk, gp);
+        poll_keys<T, 1>(ks, W, lane, tag, gk, gp);
       else if (nj == 2)
-        poll_keys<2>(ks, W, lane, tag, gk, gp);
+        poll_keys<T, 2>(ks, W, lane, tag, gk, gp);
       else if (nj == 3)
-        poll_keys<3>(ks, W, lane, tag, gk, gp);
+        poll_keys<T, 3>(ks, W, lane, tag, gk, gp);
       else
-        poll_keys<4>(ks, W, lane, tag, gk, gp);
+        poll_keys<T, 4>(ks, W, lane, tag, gk, gp);
       wave_argmax_dpp(gk, gp);  // DPP across the lanes (every lane ends with the winner)
       gp = __builtin_amdgcn_readfirstlane(gp);
       // the winner's row, read by this wave right away, to LDS for the others
