@@ -208,11 +208,19 @@ __device__ __forceinline__ void column_to_lds(const T (&v0)[RW], const T (&v1)[R
                                               int lane, const int (&lbl)[RW], int thr,
                                               T* colb) {
   if (lane == (c & 63)) {
-    T x[RW];
+    // 16-byte stores (colb is 16-byte aligned, RW a multiple of the vector width)
+    constexpr int VN = 16 / sizeof(T);
+    typedef T V __attribute__((ext_vector_type(VN)));
 #pragma unroll
-    for (int k = 0; k < RW; ++k) x[k] = c < 64 ? v0[k] : v1[k];
+    for (int k0 = 0; k0 < RW; k0 += VN) {
+      V y;
 #pragma unroll
-    for (int k = 0; k < RW; ++k) colb[k] = lbl[k] > thr ? x[k] : T(0);
+      for (int e = 0; e < VN; ++e) {
+        const T x = c < 64 ? v0[k0 + e] : v1[k0 + e];
+        y[e] = lbl[k0 + e] > thr ? x : T(0);
+      }
+      *reinterpret_cast<V*>(colb + k0) = y;
+    }
   }
 }
 
@@ -254,17 +262,14 @@ __device__ __forceinline__ void wave_argmax(const T* colb, const int* lblw, int 
 // limited to the rows labelled > i, by one compare per row against a lane value.  Column i + 1
 // of the rows then goes to LDS for the next arg-max.  HI: column i is in the upper half (v1).
 template <typename T, int RW, bool HI>
-__device__ __forceinline__ void panel_update(T (&v0)[RW], T (&v1)[RW], T* colb, const int* lblw,
-                                             int i, T rinv, T u0, T u1, int lane) {
+__device__ __forceinline__ void panel_update(T (&v0)[RW], T (&v1)[RW], T* colb,
+                                             const int (&lbl)[RW], int i, T rinv, T u0, T u1,
+                                             int lane) {
   const int cl = HI ? i - 64 : i;  // lane of column i
   const int q = lane == cl ? i : INT_MAX;
-  int lbl[RW];
   T l[RW];
 #pragma unroll
-  for (int k = 0; k < RW; ++k) {
-    lbl[k] = lblw[k];
-    l[k] = colb[k] * rinv;
-  }
+  for (int k = 0; k < RW; ++k) l[k] = colb[k] * rinv;
   if constexpr (!HI) {
     const T z0 = lane > i ? u0 : T(0);
 #pragma unroll
@@ -325,13 +330,13 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     v1[k] = (ok && c1 < nb) ? P[(int64_t)gr * ld + c1] : T(0);
   }
   const int kmax = min(max((m + NW - 1 - gr0) / NW, 0), RW);  // rows k < kmax lie below m
-  {
-    int lbl[RW];
+  // the wave's row labels: in LDS for the arg-max (lane k reads row k's) and, the same in every
+  // lane, in registers for the update
+  int lbl[RW];
 #pragma unroll
-    for (int k = 0; k < RW; ++k) lbl[k] = gr0 + NW * k;
-    if (lane < RW) lblw[wave][lane] = gr0 + NW * lane;
-    column_to_lds(v0, v1, 0, lane, lbl, -1, colb[wave]);
-  }
+  for (int k = 0; k < RW; ++k) lbl[k] = gr0 + NW * k;
+  if (lane < RW) lblw[wave][lane] = gr0 + NW * lane;
+  column_to_lds(v0, v1, 0, lane, lbl, -1, colb[wave]);
 
   for (int i = 0; i < nb; ++i) {
     const int par = i & 1;
@@ -412,20 +417,26 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
     if (stop && w == 0 && tid == 0) atomicCAS(info, 0, info_base + i + 1);
     // 4. interchange rows i and p by their labels (no data moves): the pivot row takes label
     //    i and, from now on, no multiplier; the row labelled i takes label p
-    if (w == (gp & 255) && wave == qb && lane == 0) {
-      lblw[wave][kc] = i;
-      colb[wave][kc] = T(0);
+    if (w == (gp & 255) && wave == qb) {
+      reg_set(lbl, kc, i);
+      if (lane == 0) {
+        lblw[wave][kc] = i;
+        colb[wave][kc] = T(0);
+      }
     }
-    if (kd >= 0 && p != i && lane == 0) lblw[wave][kd] = p;
+    if (kd >= 0 && p != i) {
+      reg_set(lbl, kd, p);
+      if (lane == 0) lblw[wave][kd] = p;
+    }
     if (stop) break;
     // 5. scale column i below the pivot, update the rest of every row and track the
     //    arg-max of column i + 1 over the updated rows
     const T rinv = static_cast<T>(1) / pv;
     const T u0 = prow[c0], u1 = prow[c1];
     if (i < 64) {
-      panel_update<T, RW, false>(v0, v1, colb[wave], lblw[wave], i, rinv, u0, u1, lane);
+      panel_update<T, RW, false>(v0, v1, colb[wave], lbl, i, rinv, u0, u1, lane);
     } else {
-      panel_update<T, RW, true>(v0, v1, colb[wave], lblw[wave], i, rinv, u0, u1, lane);
+      panel_update<T, RW, true>(v0, v1, colb[wave], lbl, i, rinv, u0, u1, lane);
     }
   }
 
