@@ -128,3 +128,21 @@ def test_mpir_many_rhs(ddla, oracle, fn):
         r = B[:, j] - A @ X[:, j]
         be = np.linalg.norm(r, np.inf) / (np.linalg.norm(A, np.inf) * np.linalg.norm(X[:, j], np.inf))
         assert be <= 2e-14, j
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3])
+@pytest.mark.parametrize("n", [129, 257, 300, 1021, 1300])
+def test_partial_factor_reconstructs(ddla, prec, n):
+    """The mixed LU's FP32 factor with partial pivoting (label-tracking panel, the next block's
+    interchanges fused into its unit-lower solve, FP32 look-ahead update, FP16 / BF16 / FP32 far
+    updates) reproduces P A, and its multipliers are bounded by 1 (partial pivoting; l is the
+    entry times the rounded reciprocal of the pivot, so up to one rounding above 1)."""
+    A = np.random.default_rng(7 * n + prec).uniform(-1, 1, (n, n)).astype(np.float32)
+    LU, perm, info = ddla.sgetrf_mp(A, prec, ddla.PIVOT_PARTIAL)
+    assert info == 0
+    assert sorted(perm.tolist()) == list(range(n))
+    L = np.tril(LU.astype(np.float64), -1) + np.eye(n)
+    U = np.triu(LU.astype(np.float64))
+    PA = A.astype(np.float64)[perm]
+    assert np.linalg.norm(PA - L @ U) / np.linalg.norm(PA) <= 5e-6 * np.sqrt(n)
+    assert np.abs(np.tril(LU, -1)).max() <= 1.0 + 2.0 ** -22
