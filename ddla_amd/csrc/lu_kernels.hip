@@ -195,8 +195,9 @@ __device__ __forceinline__ void reg_set(X (&v)[N], int k, X x) {
 }
 
 // The workgroup's R panel rows live in registers: local row lr = wave + NW k (k < R/NW) is
-// held by the 64 lanes of wave lr % NW, lane l holding columns l and l + 64; the row's origin
-// sits in LDS.  Column i of a wave's rows therefore sits in one lane.
+// held by the 64 lanes of wave lr % NW, lane l holding columns l and l + 64; the row's label
+// sits in LDS (lane k of the wave reads row k's for the arg-max) and, the same in every lane, in
+// registers.  Column i of a wave's rows therefore sits in one lane.
 //
 // Column c of a wave's rows (lane c & 63 of v0 or v1) goes to colb[0, RW) in LDS: the arg-max
 // over the rows is then a lane-parallel reduction instead of a scan in one lane, and the rank-1
