@@ -542,6 +542,57 @@ int max_trsv_groups() {
   return cached;
 }
 
+// B[0:rows, c0:c0+CW] <- D B[0:rows, c0:c0+CW] in place (D: rows x rows of a BS x BS
+// row-major block), one workgroup per CW columns: the workgroup stages its slice of B and D in
+// LDS before it writes, so the product reads no output.  Thread t computes rows t % BS of
+// columns (t / BS) * CW / 2 ... + CW / 2; D's rows are padded in LDS (the lanes of a read
+// take one column of D each, in distinct banks).
+template <int BS, int CW>
+__global__ __launch_bounds__(256) void block_apply_kernel(int rows, int nrhs,
+                                                          const float* __restrict__ D,
+                                                          float* __restrict__ B, int64_t ldb) {
+  static_assert(BS == 128 && CW % 2 == 0, "256 threads: two column halves of BS rows");
+  __shared__ float ds[BS][BS + 1];
+  __shared__ float xs[BS][CW];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * CW;
+  for (int idx = tid; idx < BS * CW; idx += 256) {
+    const int r = idx / CW, c = idx % CW;
+    xs[r][c] = (r < rows && c0 + c < nrhs) ? B[(int64_t)r * ldb + c0 + c] : 0.f;
+  }
+  // D is a whole BS x BS block in memory: its reads are unconditional, issued 16 at a time
+  constexpr int PER = BS * BS / 256, BATCH = 16;
+#pragma unroll
+  for (int b0 = 0; b0 < PER; b0 += BATCH) {
+    float v[BATCH];
+#pragma unroll
+    for (int q = 0; q < BATCH; ++q) v[q] = D[(b0 + q) * 256 + tid];
+#pragma unroll
+    for (int q = 0; q < BATCH; ++q) {
+      const int idx = (b0 + q) * 256 + tid, r = idx / BS, k = idx % BS;
+      ds[r][k] = (r < rows && k < rows) ? v[q] : 0.f;
+    }
+  }
+  __syncthreads();
+  const int r = tid % BS, h = tid / BS;
+  float acc[CW / 2];
+#pragma unroll
+  for (int c = 0; c < CW / 2; ++c) acc[c] = 0.f;
+#pragma unroll 8
+  for (int k = 0; k < BS; ++k) {
+    const float d = ds[r][k];
+#pragma unroll
+    for (int c = 0; c < CW / 2; ++c) acc[c] = fmaf(d, xs[k][h * (CW / 2) + c], acc[c]);
+  }
+  if (r < rows) {
+#pragma unroll
+    for (int c = 0; c < CW / 2; ++c) {
+      const int cc = c0 + h * (CW / 2) + c;
+      if (cc < nrhs) B[(int64_t)r * ldb + cc] = acc[c];
+    }
+  }
+}
+
 }  // namespace
 
 // Dinv (nblk x 64 x 64) for op(T); nblk = ceil(n/64)
@@ -577,7 +628,7 @@ int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64
 }
 
 // op(T) X = B for many right-hand sides (FP32), level-3 form on the prepared blocks: per
-// block row in dependency order, X_kb = Dfull_kb B_kb (a 128 x 128 x nrhs product), then the
+// block row in dependency order, X_kb = Dfull_kb B_kb in place (block_apply_kernel), then the
 // block rows still to come take B -= op(T)[later, kb] X_kb (one GEMM).  Same (lower, trans)
 // convention and prepared data as trsv_solve.
 int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float* A, int64_t lda,
@@ -586,23 +637,22 @@ int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float*
   constexpr int BS = solve_bs<float>();
   const int64_t nblk = blocks_of(n, BS);
   const float* Dfull = Dinv + inv64_elems(n);
-  void* t = nullptr;
-  DDLA_RETURN_IF(workspace(WS_TEMP2, (size_t)BS * nrhs * sizeof(float), &t));
-  float* X = static_cast<float*>(t);
+  constexpr int CW = 8;  // right-hand-side columns per workgroup of the diagonal-block product
+  const dim3 grid((unsigned)ceil_div(nrhs, CW));
   for (int64_t jb = 0; jb < nblk; ++jb) {
     const int64_t kb = lower ? jb : nblk - 1 - jb;
     const int64_t r0 = kb * BS;
     const int rows = (int)std::min<int64_t>(BS, n - r0);
-    DDLA_RETURN_IF(launch_sgemm(rows, nrhs, rows, 1.f, Dfull + kb * BS * BS, BS, true,
-                                B + r0 * ldb, ldb, true, 0.f, X, nrhs, 0, s));
-    DDLA_HIP_CHECK(hipMemcpy2DAsync(B + r0 * ldb, ldb * sizeof(float), X, nrhs * sizeof(float),
-                                    nrhs * sizeof(float), rows, hipMemcpyDeviceToDevice, s));
+    // X_kb = Dfull_kb B_kb, in place
+    hipLaunchKernelGGL((block_apply_kernel<BS, CW>), grid, dim3(256), 0, s, rows, nrhs,
+                       Dfull + kb * BS * BS, B + r0 * ldb, ldb);
+    DDLA_HIP_CHECK(hipGetLastError());
     // op(T)[r][c] is A[r lda + c], or A[c lda + r] when trans
     const int64_t lo = lower ? r0 + rows : 0, m = lower ? n - r0 - rows : r0;
     if (m <= 0) continue;
     const float* M = trans ? A + r0 * lda + lo : A + lo * lda + r0;
-    DDLA_RETURN_IF(launch_sgemm(m, nrhs, rows, -1.f, M, lda, !trans, X, nrhs, true, 1.f,
-                                B + lo * ldb, ldb, 0, s));
+    DDLA_RETURN_IF(launch_sgemm(m, nrhs, rows, -1.f, M, lda, !trans, B + r0 * ldb, ldb, true,
+                                1.f, B + lo * ldb, ldb, 0, s));
   }
   return 0;
 }
