@@ -253,10 +253,15 @@ int launch_laswp_trsm_slab(int bs, int64_t ncols, const T* L, int64_t ldl, T* B,
   }
   ProfScope prof(sizeof(T) == 8 ? "laswp_trsm_slab_f64" : "laswp_trsm_slab_f32",
                  (double)bs * bs * ncols, sizeof(T) * (2.0 * bs * ncols + 0.5 * bs * bs), s);
-  const bool wide = ncols >= 4096;
-  const dim3 grid((unsigned)ceil_div(ncols, 4 * (wide ? 8 : 4))), block(256);
-  if (wide)
+  // few columns (the mixed LU's next block: 128): one column per wave, four times the
+  // workgroups (measured 22 us against 28 us with four columns per wave at n = 16384)
+  const int cw = ncols >= 4096 ? 8 : ncols <= 512 ? 1 : 4;
+  const dim3 grid((unsigned)ceil_div(ncols, 4 * cw)), block(256);
+  if (cw == 8)
     hipLaunchKernelGGL((trsm_slab_kernel<T, true, true, 8, true>), grid, block, 0, s, bs, ncols,
+                       L, ldl, (int64_t)1, B, ldb, list, count);
+  else if (cw == 1)
+    hipLaunchKernelGGL((trsm_slab_kernel<T, true, true, 1, true>), grid, block, 0, s, bs, ncols,
                        L, ldl, (int64_t)1, B, ldb, list, count);
   else
     hipLaunchKernelGGL((trsm_slab_kernel<T, true, true, 4, true>), grid, block, 0, s, bs, ncols,
