@@ -58,12 +58,29 @@ int* lu_perm_area(int64_t n) {
   return reinterpret_cast<int*>(p + 64 + np * sizeof(int) + np * 4 * NBLK * sizeof(int));
 }
 
+// The steps of PLUDecBlock per panel, with a look-ahead of one panel: after panel k the main
+// stream interchanges, solves (U12) and updates only the next panel's columns and factors that
+// panel, while the auxiliary stream does the same for the columns after it and applies the
+// interchanges left of the panel.  The panel is latency-bound (one exchange between workgroups
+// per column), so the trailing DGEMM runs in its shadow.  Each element takes the same
+// products as without look-ahead (the next panel's columns may go through another GEMM tile
+// shape).  E0(k) = panel k done; E2(k) = the columns after the next panel updated, awaited
+// before the main stream touches them:
+//   main: panel(k); wait E2(k-1); swaps + U12 + update of the next panel's columns; panel(k+1)
+//   aux:  wait E0(k); swaps on [0, j) and the far columns, their U12 and update; record E2(k)
 int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* info_host) {
   hipStream_t s = stream();
+  hipStream_t s2;
+  hipEvent_t* ev;
+  DDLA_RETURN_IF(aux_stream(&s2, &ev));
   int32_t* dinfo;
   int *counts, *lists;
   DDLA_RETURN_IF(lu_bookkeeping(n, &dinfo, &counts, &lists));
   DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
+  // the auxiliary stream starts after everything queued so far on the main stream
+  DDLA_HIP_CHECK(hipEventRecord(ev[0], s));
+  DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
+  bool pending = false;
   int k = 0;
   for (int64_t j = 0; j < n; j += NBLK, ++k) {
     const int b = (int)std::min<int64_t>(NBLK, n - j);
@@ -72,18 +89,43 @@ int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* i
     int* list_k = lists + (size_t)k * 4 * NBLK;
     DDLA_RETURN_IF(launch_getf2_panel_ex<double>(m, b, panel, lda, dpiv + j, dinfo, (int)j, list_k,
                                                  counts + k, s));
-    // interchanges on columns [0, j) and [j+b, n) of rows j..n-1 (LU.cs:316-329)
-    DDLA_RETURN_IF(launch_laswp_list<double>(a + j * lda, lda, 0, n, j, j + b, list_k, counts + k,
-                                             2 * NBLK, s));
     const int64_t rest = n - j - b;
+    const int64_t nxt = std::min<int64_t>(NBLK, rest);  // the next panel's width
+    const int64_t far0 = j + b + nxt, far = n - far0;   // columns after the next panel
+    if (far > 0) DDLA_HIP_CHECK(hipEventRecord(ev[0], s));  // E0(k)
+    if (pending) {
+      DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));  // E2(k-1): next panel's columns updated
+      pending = false;
+    }
+    // interchanges on columns [0, j) and [j+b, n) of rows j..n-1 (LU.cs:316-329), here on the
+    // next panel's columns (fused with its U12) or, for the last panels, on all of them
+    if (far > 0)
+      DDLA_RETURN_IF(launch_laswp_trsm_slab<double>(b, nxt, panel, lda, panel + b, lda, list_k,
+                                                    counts + k, s));
+    else
+      DDLA_RETURN_IF(launch_laswp_list<double>(a + j * lda, lda, 0, n, j, j + b, list_k,
+                                               counts + k, 2 * NBLK, s));
     if (rest > 0) {
       // A12 = trilu(A11) \ A12 (LU.cs:332)
-      DDLA_RETURN_IF(launch_trsm_slab<double>(true, true, b, rest, panel, lda, 1, panel + b, lda, s));
-      // A22 -= A21 * A12 (LU.cs:334)
-      DDLA_RETURN_IF(launch_dgemm(rest, rest, b, -1.0, panel + b * lda, lda, true, panel + b, lda,
+      if (far <= 0)
+        DDLA_RETURN_IF(launch_trsm_slab<double>(true, true, b, nxt, panel, lda, 1, panel + b, lda, s));
+      // A22 -= A21 * A12 (LU.cs:334), the next panel's columns
+      DDLA_RETURN_IF(launch_dgemm(rest, nxt, b, -1.0, panel + b * lda, lda, true, panel + b, lda,
                                   true, 1.0, panel + b * lda + b, lda, 0, s));
     }
+    if (far > 0) {
+      DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
+      DDLA_RETURN_IF(launch_laswp_list<double>(a + j * lda, lda, 0, n, j, far0, list_k, counts + k,
+                                               2 * NBLK, s2));
+      double* a12 = a + j * lda + far0;
+      DDLA_RETURN_IF(launch_trsm_slab<double>(true, true, b, far, panel, lda, 1, a12, lda, s2));
+      DDLA_RETURN_IF(launch_dgemm(rest, far, b, -1.0, panel + b * lda, lda, true, a12, lda, true,
+                                  1.0, a12 + b * lda, lda, 0, s2));
+      DDLA_HIP_CHECK(hipEventRecord(ev[1], s2));  // E2(k)
+      pending = true;
+    }
   }
+  if (pending) DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));
   return read_info(dinfo, info_host);
 }
 
