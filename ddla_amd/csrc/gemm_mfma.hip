@@ -206,12 +206,8 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, typen
   store_b<C, BNC>(rb, Bs[0]);
   __syncthreads();
 
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) {
-      load_a<C, AK>(ra, A, lda, M, K, m0, (kt + 1) * C::BK, veca);
-      load_b<C, BNC>(rb, B, ldb, N, K, n0, (kt + 1) * C::BK, vecb);
-    }
+  // one K step of the matrix cores on the LDS slices `cur`
+  auto kstep = [&](int cur) {
     const E* as = As[cur] + (wm * C::WM + l16) * C::PA + lk;
     const E* bs = Bs[cur] + lk * C::PB + wn * C::WN + l16;
 #pragma unroll
@@ -226,11 +222,54 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, typen
 #pragma unroll
         for (int j = 0; j < C::TN; ++j) acc[i][j] = Vec<E>::mfma(a[i], b[j], acc[i][j]);
     }
-    if (kt + 1 < nk) {
-      store_a<C, AK>(ra, As[cur ^ 1]);
-      store_b<C, BNC>(rb, Bs[cur ^ 1]);
-    }
+  };
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int cur = kt & 1;
+    load_a<C, AK>(ra, A, lda, M, K, m0, (kt + 1) * C::BK, veca);
+    load_b<C, BNC>(rb, B, ldb, N, K, n0, (kt + 1) * C::BK, vecb);
+    kstep(cur);
+    store_a<C, AK>(ra, As[cur ^ 1]);
+    store_b<C, BNC>(rb, Bs[cur ^ 1]);
     __syncthreads();
+  }
+  // Tiles wholly inside C with beta != 0 (the trailing updates of the factorizations, where K
+  // is one block and the read of C is as long as the K loop): the C values of the first PF
+  // 16-row groups are fetched while the matrix cores run the last K step, the others in one
+  // batch per group below.
+  const bool full = TRI == 0 && m0 + C::BM <= M && n0 + C::BN <= N;
+  constexpr int PF = C::TM < 2 ? C::TM : 2;
+  E cp[PF][C::TN][4];
+  const int64_t crow0 = (int64_t)(m0 + wm * C::WM);
+  const int ccol0 = n0 + wn * C::WN + l16;
+  if (!BETA0 && full) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i)
+#pragma unroll
+      for (int j = 0; j < C::TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          cp[i][j][r] = Cm[(crow0 + i * 16 + Vec<E>::row(lk, r)) * ldc + ccol0 + j * 16];
+  }
+  if (nk > 0) kstep((nk - 1) & 1);
+
+  if (!BETA0 && full) {
+#pragma unroll
+    for (int i = 0; i < C::TM; ++i) {
+      E cv[C::TN][4];
+#pragma unroll
+      for (int j = 0; j < C::TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          cv[j][r] = i < PF ? cp[i < PF ? i : 0][j][r]
+                            : Cm[(crow0 + i * 16 + Vec<E>::row(lk, r)) * ldc + ccol0 + j * 16];
+#pragma unroll
+      for (int j = 0; j < C::TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Cm[(crow0 + i * 16 + Vec<E>::row(lk, r)) * ldc + ccol0 + j * 16] =
+              fma(alpha, acc[i][j][r], beta * cv[j][r]);
+    }
+    return;
   }
 
   // epilogue: register r of lane l holds row Vec<E>::row(l/16, r), column l%16 of each tile
