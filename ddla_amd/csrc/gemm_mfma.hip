@@ -232,11 +232,13 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, typen
     store_b<C, BNC>(rb, Bs[cur ^ 1]);
     __syncthreads();
   }
-  // Tiles wholly inside C with beta != 0 (the trailing updates of the factorizations, where K
-  // is one block and the read of C is as long as the K loop): the C values of the first PF
-  // 16-row groups are fetched while the matrix cores run the last K step, the others in one
-  // batch per group below.
-  const bool full = TRI == 0 && m0 + C::BM <= M && n0 + C::BN <= N;
+  // Tiles wholly inside C (and inside the stored triangle) with beta != 0 (the trailing updates
+  // of the factorizations, where K is one block and the read of C is as long as the K loop):
+  // the C values of the first PF 16-row groups are fetched while the matrix cores run the last
+  // K step, the others in one batch per group below.
+  const bool full = m0 + C::BM <= M && n0 + C::BN <= N &&
+                    (TRI == 0 || (TRI == 1 && n0 + C::BN - 1 <= m0) ||
+                     (TRI == 2 && m0 + C::BM - 1 <= n0));
   constexpr int PF = C::TM < 2 ? C::TM : 2;
   E cp[PF][C::TN][4];
   const int64_t crow0 = (int64_t)(m0 + wm * C::WM);
