@@ -45,6 +45,37 @@ def test_gemm_shapes(ddla, oracle, m, n, k):
     assert rel_fro(C, ref) <= 1e-13 * k
 
 
+@pytest.mark.parametrize("k", [1, 17, 128])
+def test_rank_k_update_large_tiles(ddla, oracle, k):
+    # C -= A B with beta = 1 at the factorizations' update shape (K one block or less), large
+    # enough for the 128 x 128 tiles: interior tiles fetch C during the last K step, the
+    # ragged right and bottom edges take the bounds-checked store
+    m, n = 3000, 2950  # 24 x 24 tiles of 128: at least two per CU, so the 128 x 128 kernel runs
+    A = oracle.uniform_matrix(m, k, 41 + k)
+    B = oracle.uniform_matrix(k, n, 42 + k)
+    C0 = oracle.uniform_matrix(m, n, 43 + k)
+    ref = oracle.gemm(-1.0, A, B, 1.0, C0.copy())
+    dA, dB, dC = (ddla.DeviceMatrix.from_host(X) for X in (A, B, C0))
+    ddla.BlasProvider.GeMM(-1.0, dA, dB, 1.0, dC)
+    assert rel_fro(dC.to_host(), ref) <= 1e-13 * max(k, 8)
+
+
+@pytest.mark.parametrize("uplo", [0xC0, 0x60])
+def test_syrk_large_tiles(ddla, oracle, uplo):
+    # SYRK update at 128 x 128 tiles: tiles wholly inside the stored triangle take the early C
+    # fetch, diagonal tiles the masked store; the other triangle stays untouched
+    n, k = 3000, 128
+    A = oracle.uniform_matrix(n, k, 51)
+    C0 = oracle.uniform_matrix(n, n, 52)
+    ref = oracle.syrk(uplo, 0x00, -1.0, A, 1.0, C0.copy())
+    dA, dC = ddla.DeviceMatrix.from_host(A), ddla.DeviceMatrix.from_host(C0)
+    ddla.BlasProvider.SyRk(uplo, 0x00, -1.0, dA, 1.0, dC)
+    C = dC.to_host()
+    mask = np.tril(np.ones((n, n), bool)) if uplo == 0xC0 else np.triu(np.ones((n, n), bool))
+    assert rel_fro(C[mask], ref[mask]) <= 1e-13 * k
+    np.testing.assert_array_equal(C[~mask], C0[~mask])
+
+
 @pytest.mark.parametrize("ta", [ddla_t for ddla_t in (0x00, 0x08)])
 @pytest.mark.parametrize("tb", [0x00, 0x08])
 def test_gemm_transposes(ddla, oracle, ta, tb):
