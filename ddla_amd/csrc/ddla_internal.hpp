@@ -53,6 +53,11 @@ struct Context {
   // growable scratch buffers (device memory kept for the life of the context)
   void* ws[16] = {};
   size_t ws_bytes[16] = {};
+  // bytes of each slot cleared since it was allocated, and the tag sequence of the exchange
+  // words kept there (kernels that wait on each other's words compare 32-bit tags; a cleared
+  // slot holds no live tag)
+  size_t ws_cleared[16] = {};
+  unsigned ws_tag[16] = {};
 };
 
 Context& ctx();
@@ -62,6 +67,12 @@ hipStream_t stream();
 int aux_stream(hipStream_t* s, hipEvent_t** ev);
 // Returns a device scratch buffer `slot` of at least `bytes` (contents undefined).
 int workspace(int slot, size_t bytes, void** out);
+// The same, with the first `bytes` bytes cleared at least once since the buffer was
+// allocated (on stream s, only when needed); *fresh tells the caller whether this call
+// cleared it, in which case ctx().ws_tag[slot] has been reset to 0.
+int workspace_cleared(int slot, size_t bytes, void** out, hipStream_t s, bool* fresh = nullptr);
+// Next tag of slot's sequence (never 0); a wrapped sequence clears the slot again first.
+int next_tag(int slot, size_t bytes, unsigned step, hipStream_t s, unsigned* tag);
 enum WorkspaceSlot {
   WS_PANEL = 0,   // panel-kernel exchange slots
   WS_PIVOT = 1,   // pivots / permutations / info words
@@ -154,6 +165,34 @@ int launch_split(bool bf16, int terms, bool kcontig, int64_t rows, int64_t k, co
                  int64_t ld, void* out, float* scales, hipStream_t s);
 int launch_hgemm_parts(bool bf16, int terms, int tri, int64_t m, int64_t n, int64_t k,
                        float alpha, SplitOp a, SplitOp b, float* C, int64_t ldc, hipStream_t s);
+
+// ------------------------------------------------------------ host-side helpers (api.hip) --
+bool valid_trans(int t);
+// real data: conjugation is the identity (BLIS ignores conj for real types)
+bool is_trans(int t);
+void pack_host(int64_t m, int64_t n, const double* src, int64_t rs, int64_t cs, double* dst);
+void unpack_host(int64_t m, int64_t n, const double* src, double* dst, int64_t rs, int64_t cs);
+// device buffer owned by a host-pointer entry point for the duration of the call
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  int alloc(size_t bytes);
+  double* d() const { return static_cast<double*>(p); }
+};
+// strided host view (m x n, rs, cs) -> new contiguous row-major device buffer, and back
+int upload_view(int64_t m, int64_t n, const double* host, int64_t rs, int64_t cs, DevBuf& buf);
+int download_view(int64_t m, int64_t n, const DevBuf& buf, double* host, int64_t rs, int64_t cs);
+// C = alpha op(A) op(B) + beta C on device pointers with any strides (BLIS conventions);
+// tri (m == n): 0 all of C, 1 / 2 only its lower / upper triangle
+int gemm_strided(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double* a, int64_t rsa, int64_t csa, const double* b, int64_t rsb,
+                 int64_t csb, double beta, double* c, int64_t rsc, int64_t csc, int tri);
+// B = alpha op(A)^-1 B (left) or alpha B op(A)^-1 (right), row-major device matrices
+int trsm_rowmajor(int side, int uplo, int transa, int diag, int64_t m, int64_t n, double alpha,
+                  const double* a, int64_t lda, double* b, int64_t ldb);
 
 // general strided copy / transpose helpers
 int launch_dcopy2d(int64_t m, int64_t n, const double* src, int64_t rs, int64_t cs, double* dst,

@@ -630,26 +630,23 @@ bool fits(int64_t m, int limit) {
 
 }  // namespace
 
-// exchange area for panel kernels
-int panel_sync(PanelSync* sy, unsigned** base_counter) {
+// exchange area for panel kernels; *base: the launch's tag range is base + 1 .. base + nb + 2
+// (a fresh range per launch, so words of earlier launches never match)
+int panel_sync(PanelSync* sy, int nb, unsigned* base, hipStream_t s) {
   const size_t kb = 2 * (size_t)WMAX * KSTRIDE * sizeof(u64);
   const size_t cb = 2 * (size_t)WMAX * NB_MAX * 2 * sizeof(u64);
   const size_t lb = 4 * (size_t)NB_MAX * sizeof(int) + 256;
   const size_t bytes = kb + cb + lb;
   void* ws = nullptr;
-  static void* zeroed_ptr = nullptr;
+  unsigned first = 0;
+  DDLA_RETURN_IF(next_tag(WS_PANEL, bytes, static_cast<unsigned>(nb) + 2u, s, &first));
   DDLA_RETURN_IF(workspace(WS_PANEL, bytes, &ws));
-  if (zeroed_ptr != ws) {  // tags start from a clean slate once per allocation
-    DDLA_HIP_CHECK(hipMemsetAsync(ws, 0, bytes, stream()));
-    zeroed_ptr = ws;
-  }
+  *base = first - 1u;
   char* p = static_cast<char*>(ws);
   sy->keys = reinterpret_cast<u64*>(p);
   sy->cand = reinterpret_cast<u64*>(p + kb);
   sy->list = reinterpret_cast<int*>(p + kb + cb);
   sy->list_count = sy->list + 4 * NB_MAX;
-  static unsigned counter = 0;
-  *base_counter = &counter;
   return 0;
 }
 
@@ -661,16 +658,12 @@ int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int
     return DDLA_ERR_INVALID_ARG;
   }
   PanelSync sy;
-  unsigned* counter = nullptr;
-  DDLA_RETURN_IF(panel_sync(&sy, &counter));
+  unsigned base = 0;
+  DDLA_RETURN_IF(panel_sync(&sy, nb, &base, s));
   if (list_out) {
     sy.list = list_out;
     sy.list_count = count_out;
   }
-  // step tags: every launch takes a fresh range, so stale words never match
-  const unsigned base = *counter;
-  *counter += static_cast<unsigned>(nb) + 2u;
-  if (*counter < base) *counter = 1;  // wrapped: restart (all words hold older tags)
   ProfScope prof(sizeof(T) == 8 ? "getf2_panel_f64" : "getf2_panel_f32",
                  (double)m * nb * nb - (double)nb * nb * nb / 3.0, 2.0 * sizeof(T) * m * nb, s);
   // fewest workgroups that hold the panel (fewer slots to read per step)

@@ -93,8 +93,36 @@ int workspace(int slot, size_t bytes, void** out) {
     }
     g_ctx.ws[slot] = p;
     g_ctx.ws_bytes[slot] = want;
+    g_ctx.ws_cleared[slot] = 0;
   }
   *out = g_ctx.ws[slot];
+  return 0;
+}
+
+int workspace_cleared(int slot, size_t bytes, void** out, hipStream_t s, bool* fresh) {
+  DDLA_RETURN_IF(workspace(slot, bytes, out));
+  if (bytes == 0) bytes = 256;
+  const bool clear = g_ctx.ws_cleared[slot] < bytes;
+  if (clear) {
+    DDLA_HIP_CHECK(hipMemsetAsync(*out, 0, bytes, s));
+    g_ctx.ws_cleared[slot] = bytes;
+    g_ctx.ws_tag[slot] = 0;
+  }
+  if (fresh) *fresh = clear;
+  return 0;
+}
+
+int next_tag(int slot, size_t bytes, unsigned step, hipStream_t s, unsigned* tag) {
+  void* p = nullptr;
+  DDLA_RETURN_IF(workspace_cleared(slot, bytes, &p, s));
+  unsigned& t = g_ctx.ws_tag[slot];
+  if (t > 0xFFFFFFFFu - step - 1u) {
+    // the sequence would wrap: words with small tags may still be in the slot
+    DDLA_HIP_CHECK(hipMemsetAsync(p, 0, g_ctx.ws_cleared[slot], s));
+    t = 0;
+  }
+  *tag = t + 1u;
+  t += step;
   return 0;
 }
 

@@ -671,21 +671,14 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
     set_error("trsv: nrhs %d > %d", nrhs, RMAX);
     return DDLA_ERR_INVALID_ARG;
   }
-  static unsigned tag = 0;
   void* ws = nullptr;
   // X words, then the helpers' partial-sum words (2 words per FP64 value)
   const size_t wbytes = (size_t)n * nrhs * 4 * sizeof(u64);
+  // one tag sequence for the FP32 and FP64 solves: both use this workspace, so a word left
+  // by a solve of either type never carries the tag of a later solve
+  unsigned tag = 0;
+  DDLA_RETURN_IF(next_tag(WS_GMRES, wbytes, 1, s, &tag));
   DDLA_RETURN_IF(workspace(WS_GMRES, wbytes, &ws));
-  static void* zeroed = nullptr;
-  static size_t zeroed_bytes = 0;
-  if (zeroed != ws || zeroed_bytes < wbytes) {
-    DDLA_HIP_CHECK(hipMemsetAsync(ws, 0, wbytes, s));
-    zeroed_bytes = wbytes;
-    zeroed = ws;
-    tag = 0;
-  }
-  tag += 1;
-  if (tag == 0) tag = 1;
   const int nblk = ceil_div(n, solve_bs<T>());
   ProfScope prof(sizeof(T) == 8 ? "trsv_f64" : "trsv_f32", (double)n * n * nrhs,
                  (0.5 * n * n + 2.0 * n * nrhs) * sizeof(T), s);

@@ -400,10 +400,6 @@ __global__ __launch_bounds__(TS_ROWS) void tslu_tree_kernel(TsluArgs a) {
   }
 }
 
-unsigned g_tag = 0;
-void* g_zeroed = nullptr;
-size_t g_zeroed_bytes = 0;
-
 }  // namespace
 
 bool tslu_supported(int64_t m, int nb) { return nb == TS_NB && m >= TS_NB && m <= (1 << 24); }
@@ -419,13 +415,9 @@ int launch_tslu_panel(int64_t m, int nb, float* P, int64_t ld, int32_t* info, in
   const size_t nodes = 2 * (size_t)leaves + 32;
   const size_t bytes = nodes * (TS_NB + 2) * sizeof(int);
   void* ws = nullptr;
+  unsigned tag = 0;  // arrival words of earlier launches never hold this launch's tag
+  DDLA_RETURN_IF(next_tag(WS_TSLU, bytes, 1, s, &tag));
   DDLA_RETURN_IF(workspace(WS_TSLU, bytes, &ws));
-  if (ws != g_zeroed || bytes > g_zeroed_bytes) {  // arrival words must not hold a live tag
-    DDLA_HIP_CHECK(hipMemsetAsync(ws, 0, bytes, s));
-    g_zeroed = ws;
-    g_zeroed_bytes = bytes;
-  }
-  if (++g_tag == 0) ++g_tag;
   TsluArgs a;
   a.m = (int)m;
   a.leaves = leaves;
@@ -434,7 +426,7 @@ int launch_tslu_panel(int64_t m, int nb, float* P, int64_t ld, int32_t* info, in
   a.sets = static_cast<int*>(ws);
   a.set_count = a.sets + nodes * TS_NB;
   a.flags = reinterpret_cast<unsigned*>(a.set_count + nodes);
-  a.tag = g_tag;
+  a.tag = tag;
   a.info = info;
   a.info_base = info_base;
   a.list = list_out;
