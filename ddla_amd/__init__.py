@@ -45,6 +45,11 @@ def synchronize():
     check(_lib().ddla_sync())
 
 
+def finalize():
+    """Releases the library context (also done at interpreter exit)."""
+    native.finalize()
+
+
 def stream_handle() -> int:
     """The library's hipStream_t as an integer (for torch.cuda.ExternalStream)."""
     p = ctypes.c_void_p()
@@ -158,7 +163,8 @@ class DeviceMatrix:
 
     def free(self):
         if getattr(self, "_owner", False) and self._ptr:
-            _lib().ddla_free(self._ptr)
+            if not native.finalized():
+                _lib().ddla_free(self._ptr)
             self._ptr = ctypes.c_void_p()
             self._owner = False
 
@@ -194,20 +200,93 @@ class BlasProvider:
                                        _ptr(C), *_strides(C)))
 
     @staticmethod
-    def TrSM(side, uplo, trans, diag, alpha, A: DeviceMatrix, B: DeviceMatrix):
-        """B = alpha*op(A)^-1*B or alpha*B*op(A)^-1 (DDLA/BLAS/3m.cs:287-314)."""
+    def TrSM(side, uplo, trans, diag, alpha, A, B):
+        """B = alpha*op(A)^-1*B or alpha*B*op(A)^-1 (DDLA/BLAS/3m.cs:287-305).
+        numpy operands (any strides) go through bli_dtrsm's argument list (ddla_dtrsm)."""
         al = ctypes.c_double(alpha)
-        m, n = B.shape
-        return check(_lib().ddla_dtrsm_dev(side, uplo, trans, diag, m, n, ctypes.byref(al), A.ptr,
-                                           A.ld, B.ptr, B.ld))
+        m, n = _shape(B)
+        _check_square(A, m if side == LEFT else n)
+        if isinstance(B, DeviceMatrix):
+            return check(_lib().ddla_dtrsm_dev(side, uplo, trans, diag, m, n, ctypes.byref(al),
+                                               A.ptr, A.ld, B.ptr, B.ld))
+        return check(_lib().ddla_dtrsm(side, uplo, trans, diag, m, n, ctypes.byref(al),
+                                       *_view(A), *_view(B)))
 
     @staticmethod
-    def SyRk(uplo, trans, alpha, A: DeviceMatrix, beta, C: DeviceMatrix):
-        """uplo(C) = beta*C + alpha*op(A)*op(A)^T (DDLA/BLAS/3m.cs:118-147)."""
+    def SyRk(uplo, trans, alpha, A, beta, C):
+        """uplo(C) = beta*C + alpha*op(A)*op(A)^T (DDLA/BLAS/3m.cs:118-138)."""
         al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
-        k = A.rows if trans == TRANSPOSE else A.cols
-        return check(_lib().ddla_dsyrk_dev(uplo, trans, C.rows, k, ctypes.byref(al), A.ptr, A.ld,
-                                           ctypes.byref(be), C.ptr, C.ld))
+        m = _check_square(C)
+        ma, k = _op_shape(A, trans)
+        if ma != m:
+            raise ValueError("Dimensions of matrixs A must be match!")
+        if isinstance(C, DeviceMatrix):
+            return check(_lib().ddla_dsyrk_dev(uplo, trans, m, k, ctypes.byref(al), A.ptr, A.ld,
+                                               ctypes.byref(be), C.ptr, C.ld))
+        return check(_lib().ddla_dsyrk(uplo, trans, m, k, ctypes.byref(al), *_view(A),
+                                       ctypes.byref(be), *_view(C)))
+
+    @staticmethod
+    def GeMMT(uplo, transA, transB, alpha, A, B, beta, C):
+        """uplo(C) = beta*C + alpha*op(A)*op(B) (DDLA/BLAS/3m.cs:44-70 -> bli_dgemmt)."""
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        m = _check_square(C)
+        ma, k = _op_shape(A, transA)
+        kb, mb = _op_shape(B, transB)
+        if ma != m or kb != k or mb != m:
+            raise ValueError("Dimensions of matrix A must be match!")
+        fn = _lib().ddla_dgemmt_dev if isinstance(C, DeviceMatrix) else _lib().ddla_dgemmt
+        return check(fn(uplo, transA, transB, m, k, ctypes.byref(al), *_view(A), *_view(B),
+                        ctypes.byref(be), *_view(C)))
+
+    @staticmethod
+    def SyR2k(uplo, transA, transB, alpha, A, B, beta, C):
+        """uplo(C) = beta*C + alpha*(op(A) op(B)^T + op(B) op(A)^T) (DDLA/BLAS/3m.cs:149-172)."""
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        m = _check_square(C)
+        ma, k = _op_shape(A, transA)
+        mb, kb = _op_shape(B, transB)
+        if ma != m or mb != m or kb != k:
+            raise ValueError("Dimensions of matrixs A and B must be match!")
+        fn = _lib().ddla_dsyr2k_dev if isinstance(C, DeviceMatrix) else _lib().ddla_dsyr2k
+        return check(fn(uplo, transA, transB, m, k, ctypes.byref(al), *_view(A), *_view(B),
+                        ctypes.byref(be), *_view(C)))
+
+    @staticmethod
+    def SyMM(side, uplo, transB, alpha, A, B, beta, C):
+        """C = beta*C + alpha*A*op(B) (LEFT) or alpha*op(B)*A (RIGHT), A symmetric from its
+        uplo triangle (DDLA/BLAS/3m.cs:83-106 -> bli_dsymm)."""
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        m, n = _shape(C)
+        if _op_shape(B, transB) != (m, n):
+            raise ValueError("Dimensions of matrixs must be match!")
+        _check_square(A, m if side == LEFT else n)
+        fn = _lib().ddla_dsymm_dev if isinstance(C, DeviceMatrix) else _lib().ddla_dsymm
+        return check(fn(side, uplo, 0, transB, m, n, ctypes.byref(al), *_view(A), *_view(B),
+                        ctypes.byref(be), *_view(C)))
+
+    @staticmethod
+    def TrMM(side, uplo, trans, diag, alpha, A, B):
+        """B = alpha*op(A)*B (LEFT) or alpha*B*op(A) (RIGHT), A triangular
+        (DDLA/BLAS/3m.cs:185-204 -> bli_dtrmm)."""
+        al = ctypes.c_double(alpha)
+        m, n = _shape(B)
+        _check_square(A, m if side == LEFT else n)
+        fn = _lib().ddla_dtrmm_dev if isinstance(B, DeviceMatrix) else _lib().ddla_dtrmm
+        return check(fn(side, uplo, trans, diag, m, n, ctypes.byref(al), *_view(A), *_view(B)))
+
+    @staticmethod
+    def TrMM3(side, uplo, transA, diag, transB, alpha, A, B, beta, C):
+        """C = beta*C + alpha*op(A)*op(B) (LEFT) or alpha*op(B)*op(A) (RIGHT), A triangular
+        (DDLA/BLAS/3m.cs:225-250 -> bli_dtrmm3)."""
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        m, n = _shape(C)
+        if _op_shape(B, transB) != (m, n):
+            raise ValueError("Dimensions of matrixs must be match!")
+        _check_square(A, m if side == LEFT else n)
+        fn = _lib().ddla_dtrmm3_dev if isinstance(C, DeviceMatrix) else _lib().ddla_dtrmm3
+        return check(fn(side, uplo, transA, diag, transB, m, n, ctypes.byref(al), *_view(A),
+                        *_view(B), ctypes.byref(be), *_view(C)))
 
     @staticmethod
     def AMax(x: DeviceMatrix, inc: int = 1) -> int:
@@ -216,6 +295,30 @@ class BlasProvider:
         out = ctypes.c_int64()
         check(_lib().ddla_damaxv_dev(n, x.ptr, inc if inc != 1 else 1, ctypes.byref(out)))
         return int(out.value)
+
+
+def _shape(X):
+    return X.shape if isinstance(X, DeviceMatrix) else _as_f64(X).shape
+
+
+def _op_shape(X, trans):
+    r, c = _shape(X)
+    return (c, r) if trans == TRANSPOSE else (r, c)
+
+
+def _check_square(X, n=None):
+    r, c = _shape(X)
+    if r != c or (n is not None and r != n):
+        raise ValueError("Dimensions of matrixs A must be match!")
+    return r
+
+
+def _view(X):
+    """(pointer, row stride, column stride) of a DeviceMatrix or a numpy array (elements)."""
+    if isinstance(X, DeviceMatrix):
+        return X.ptr, X.ld, 1
+    X = _as_f64(X)
+    return (_ptr(X), *_strides(X))
 
 
 def _zero_pivot(info: int):
@@ -325,7 +428,7 @@ class _DevInts:
 
     def __del__(self):
         try:
-            if self.ptr:
+            if self.ptr and not native.finalized():
                 _lib().ddla_free(self.ptr)
         except Exception:
             pass
@@ -425,8 +528,10 @@ PIVOT_PARTIAL, PIVOT_BUTTERFLY, PIVOT_TOURNAMENT = 0, 1, 2
 
 
 def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_iters, gmres,
-          terms, pivoting=PIVOT_PARTIAL):
-    opts = MpirOpts(prec, max_iters, tol, gmres, terms, pivoting, 0)
+          terms, pivoting=PIVOT_PARTIAL, x_ref=None):
+    opts = MpirOpts(prec, max_iters, tol, gmres, terms, pivoting, 0,
+                    x_ref.ptr if x_ref is not None else None,
+                    x_ref.ld if x_ref is not None else 0)
     info = MpirInfo()
     st = fn(A.rows, B.cols, A.ptr, A.ld, B.ptr, B.ld, X.ptr, X.ld, ctypes.byref(opts),
             ctypes.byref(info))
@@ -436,21 +541,24 @@ def _mpir(fn, A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec, tol, max_
 
 
 def gesv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16, tol=1e-14,
-              max_iters=50, gmres=0, terms=0, pivoting=PIVOT_PARTIAL) -> MpirInfo:
+              max_iters=50, gmres=0, terms=0, pivoting=PIVOT_PARTIAL, x_ref=None) -> MpirInfo:
     """Mixed-precision LU + iterative refinement to FP64 accuracy (no reference counterpart).
 
     terms: low-precision parts per operand of the trailing updates (0 = library default).
     pivoting: PIVOT_PARTIAL; PIVOT_BUTTERFLY (random butterfly transform + LU without
     pivoting); or PIVOT_TOURNAMENT (pivot rows of each 128-column panel chosen by a reduction
-    tree, as in CALU).  The last two fall back to partial pivoting if they do not refine."""
+    tree, as in CALU).  The last two fall back to partial pivoting if they do not refine.
+    x_ref: optional reference solution (DeviceMatrix): info.forward_error is measured
+    against it (else -1)."""
     return _mpir(_lib().ddla_dgesv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms,
-                 pivoting)
+                 pivoting, x_ref)
 
 
 def posv_mpir(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, prec=PREC_FP16, tol=1e-14,
-              max_iters=50, gmres=0, terms=0) -> MpirInfo:
+              max_iters=50, gmres=0, terms=0, x_ref=None) -> MpirInfo:
     """Mixed-precision Cholesky + iterative refinement (no reference counterpart)."""
-    return _mpir(_lib().ddla_dposv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms)
+    return _mpir(_lib().ddla_dposv_mpir_dev, A, B, X, prec, tol, max_iters, gmres, terms,
+                 PIVOT_PARTIAL, x_ref)
 
 
 def sgetrf_mp(A, prec=PREC_FP32, pivoting=PIVOT_TOURNAMENT):
@@ -495,7 +603,7 @@ class _DevBuf:
 
     def __del__(self):
         try:
-            if self.ptr:
+            if self.ptr and not native.finalized():
                 _lib().ddla_free(self.ptr)
         except Exception:
             pass
@@ -533,7 +641,7 @@ class BatchPlan:
 
     def __del__(self):
         try:
-            if getattr(self, "handle", None):
+            if getattr(self, "handle", None) and not native.finalized():
                 _lib().ddla_batch_destroy(self.handle)
         except Exception:
             pass

@@ -12,7 +12,7 @@ int finalize();
 
 using namespace ddla;
 
-namespace {
+namespace ddla {
 
 bool valid_trans(int t) {
   return t == DDLA_NO_TRANSPOSE || t == DDLA_TRANSPOSE || t == DDLA_CONJ_NO_TRANSPOSE ||
@@ -31,22 +31,20 @@ void unpack_host(int64_t m, int64_t n, const double* src, double* dst, int64_t r
     for (int64_t j = 0; j < n; ++j) dst[i * rs + j * cs] = src[i * n + j];
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) hipFree(p);
+DevBuf::~DevBuf() {
+  if (p) hipFree(p);
+}
+
+int DevBuf::alloc(size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    p = nullptr;
+    set_error("hipMalloc of %zu bytes failed", bytes);
+    return DDLA_ERR_ALLOC;
   }
-  int alloc(size_t bytes) {
-    if (bytes == 0) bytes = 8;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-      p = nullptr;
-      set_error("hipMalloc of %zu bytes failed", bytes);
-      return DDLA_ERR_ALLOC;
-    }
-    return 0;
-  }
-  double* d() const { return static_cast<double*>(p); }
-};
+  return 0;
+}
+
 
 int upload_view(int64_t m, int64_t n, const double* host, int64_t rs, int64_t cs, DevBuf& buf) {
   DDLA_RETURN_IF(buf.alloc((size_t)m * n * sizeof(double)));
@@ -74,10 +72,26 @@ int download_view(int64_t m, int64_t n, const DevBuf& buf, double* host, int64_t
   return 0;
 }
 
-// device GEMM with arbitrary strides, reduced to the row-major-C kernel
-int gemm_dev(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
-             int64_t rsa, int64_t csa, const double* b, int64_t rsb, int64_t csb, double beta,
-             double* c, int64_t rsc, int64_t csc) {
+int trsm_rowmajor(int side, int uplo, int transa, int diag, int64_t m, int64_t n, double alpha,
+                  const double* a, int64_t lda, double* b, int64_t ldb) {
+  hipStream_t s = stream();
+  DDLA_RETURN_IF(launch_dscale2d(m, n, alpha, b, ldb, 1, 0, s));
+  const bool lower = uplo == DDLA_LOWER, unit = diag == DDLA_UNIT_DIAG, tr = is_trans(transa);
+  if (side == DDLA_LEFT) return trsm_left<double>(lower, unit, tr, m, n, a, lda, b, ldb, s);
+  // X op(A) = B  <=>  op(A)^T X^T = B^T
+  void* ws = nullptr;
+  DDLA_RETURN_IF(workspace(WS_MIXED2, (size_t)m * n * sizeof(double), &ws));
+  double* bt = static_cast<double*>(ws);
+  DDLA_RETURN_IF(launch_dcopy2d(n, m, b, 1, ldb, bt, m, s));
+  DDLA_RETURN_IF(trsm_left<double>(lower, unit, !tr, n, m, a, lda, bt, m, s));
+  return launch_dcopy2d(m, n, bt, 1, m, b, ldb, s);
+}
+
+// device GEMM with arbitrary strides, reduced to the row-major-C kernel; tri (m == n): 0 the
+// whole of C, 1 / 2 only its lower / upper triangle (GEMMT), the other left untouched
+int gemm_strided(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double* a, int64_t rsa, int64_t csa, const double* b, int64_t rsb,
+                 int64_t csb, double beta, double* c, int64_t rsc, int64_t csc, int tri) {
   hipStream_t s = stream();
   if (m == 0 || n == 0) return 0;
   // op(A): m x k with strides (ra, ca); op(B): k x n with strides (rb, cb)
@@ -97,6 +111,7 @@ int gemm_dev(int transa, int transb, int64_t m, int64_t n, int64_t k, double alp
     int64_t nra = cb, nca = rb, nrb = ca, ncb = ra;
     ra = nra; ca = nca; rb = nrb; cb = ncb;
     ldc = csc;
+    tri = tri == 1 ? 2 : (tri == 2 ? 1 : 0);  // the lower triangle of C is the upper of C^T
   } else {
     c_tmp = true;
     ldc = n;
@@ -135,13 +150,18 @@ int gemm_dev(int transa, int transb, int64_t m, int64_t n, int64_t k, double alp
   if (ldb <= 0) ldb = 1;
   if (c_tmp) {
     double* pc = reinterpret_cast<double*>(tp);
-    if (beta != 0.0) DDLA_RETURN_IF(launch_dcopy2d(m, n, c, rsc, csc, pc, n, s));
-    DDLA_RETURN_IF(launch_dgemm(m, n, k, alpha, A, lda, ak, B, ldb, bn, beta, pc, n, 0, s));
+    // the copy is written back whole, so a triangle-only product needs the other triangle too
+    if (beta != 0.0 || tri != 0) DDLA_RETURN_IF(launch_dcopy2d(m, n, c, rsc, csc, pc, n, s));
+    DDLA_RETURN_IF(launch_dgemm(m, n, k, alpha, A, lda, ak, B, ldb, bn, beta, pc, n, tri, s));
     DDLA_RETURN_IF(launch_dscatter2d(m, n, pc, n, c, rsc, csc, s));
     return 0;
   }
-  return launch_dgemm(m, n, k, alpha, A, lda, ak, B, ldb, bn, beta, C, ldc, 0, s);
+  return launch_dgemm(m, n, k, alpha, A, lda, ak, B, ldb, bn, beta, C, ldc, tri, s);
 }
+
+}  // namespace ddla
+
+namespace {
 
 __global__ void amax_kernel(int64_t n, const double* __restrict__ x, int64_t inc,
                             int64_t* __restrict__ out) {
@@ -340,7 +360,8 @@ int ddla_dgemm_dev(int transa, int transb, ddla_dim_t m, ddla_dim_t n, ddla_dim_
   DDLA_ARG_CHECK(valid_trans(transa) && valid_trans(transb), "bad trans flag");
   DDLA_ARG_CHECK(m >= 0 && n >= 0 && k >= 0, "negative dimension");
   DDLA_ARG_CHECK(alpha && beta, "null alpha/beta");
-  return gemm_dev(transa, transb, m, n, k, *alpha, a, rsa, csa, b, rsb, csb, *beta, c, rsc, csc);
+  return gemm_strided(transa, transb, m, n, k, *alpha, a, rsa, csa, b, rsb, csb, *beta, c, rsc,
+                      csc, 0);
 }
 
 int ddla_dgemm(int transa, int transb, ddla_dim_t m, ddla_dim_t n, ddla_dim_t k,
@@ -362,8 +383,8 @@ int ddla_dgemm(int transa, int transb, ddla_dim_t m, ddla_dim_t n, ddla_dim_t k,
   } else {
     DDLA_RETURN_IF(dC.alloc((size_t)m * n * sizeof(double)));
   }
-  DDLA_RETURN_IF(gemm_dev(transa, transb, m, n, k, *alpha, dA.d(), an, 1, dB.d(), bn, 1, *beta,
-                          dC.d(), n, 1));
+  DDLA_RETURN_IF(gemm_strided(transa, transb, m, n, k, *alpha, dA.d(), an, 1, dB.d(), bn, 1,
+                              *beta, dC.d(), n, 1, 0));
   return download_view(m, n, dC, c, rsc, csc);
 }
 
@@ -383,6 +404,8 @@ int ddla_dgetrs_dev(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a, d
   DDLA_ARG_CHECK(valid_trans(trans), "bad trans flag");
   DDLA_ARG_CHECK(n >= 0 && nrhs >= 0 && lda >= n && ldb >= nrhs, "bad getrs arguments");
   if (n == 0 || nrhs == 0) return 0;
+  DDLA_ARG_CHECK(a && dpiv && b, "null pointer");
+  DDLA_RETURN_IF(check_pivots(n, dpiv, stream()));
   return dgetrs_internal(is_trans(trans), n, nrhs, a, lda, dpiv, b, ldb);
 }
 
@@ -405,6 +428,11 @@ int ddla_dgetrs(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_
   API_INIT();
   DDLA_ARG_CHECK(valid_trans(trans) && n >= 0 && nrhs >= 0, "bad getrs arguments");
   if (n == 0 || nrhs == 0) return 0;
+  DDLA_ARG_CHECK(a && piv && b, "null pointer");
+  for (int64_t i = 0; i < n; ++i)
+    DDLA_ARG_CHECK(piv[i] <= 0 || i + piv[i] < n,
+                   "the pivot at index %lld swaps outside the %lld rows (SwapRow throws there)",
+                   (long long)i, (long long)n);
   DevBuf dA, dB, dP;
   DDLA_RETURN_IF(upload_view(n, n, a, rsa, csa, dA));
   DDLA_RETURN_IF(upload_view(n, nrhs, b, rsb, csb, dB));
@@ -423,7 +451,9 @@ int ddla_dlaswp_dev(int side, int forward, ddla_dim_t m, ddla_dim_t n, double* x
   DDLA_ARG_CHECK(npiv == (side == DDLA_LEFT ? m : n),
                  "the length of pivs must match the permuted dimension (LU.cs:240-247)");
   if (m == 0 || n == 0) return 0;
+  DDLA_ARG_CHECK(x && dpiv, "null pointer");
   hipStream_t s = stream();
+  DDLA_RETURN_IF(check_pivots(npiv, dpiv, s));
   void* ws = nullptr;
   const size_t perm_bytes = ((size_t)npiv * sizeof(int) + 255) / 256 * 256;
   DDLA_RETURN_IF(workspace(WS_REFINE, perm_bytes + (size_t)m * n * sizeof(double), &ws));
@@ -524,17 +554,8 @@ int ddla_dtrsm_dev(int side, int uplo, int transa, int diag, ddla_dim_t m, ddla_
   const int64_t ka = (side == DDLA_LEFT) ? m : n;
   DDLA_ARG_CHECK(m >= 0 && n >= 0 && lda >= ka && ldb >= n, "bad trsm arguments");
   if (m == 0 || n == 0) return 0;
-  hipStream_t s = stream();
-  DDLA_RETURN_IF(launch_dscale2d(m, n, *alpha, b, ldb, 1, 0, s));
-  const bool lower = uplo == DDLA_LOWER, unit = diag == DDLA_UNIT_DIAG, tr = is_trans(transa);
-  if (side == DDLA_LEFT) return trsm_left<double>(lower, unit, tr, m, n, a, lda, b, ldb, s);
-  // X op(A) = B  <=>  op(A)^T X^T = B^T
-  void* ws = nullptr;
-  DDLA_RETURN_IF(workspace(WS_MIXED2, (size_t)m * n * sizeof(double), &ws));
-  double* bt = static_cast<double*>(ws);
-  DDLA_RETURN_IF(launch_dcopy2d(n, m, b, 1, ldb, bt, m, s));
-  DDLA_RETURN_IF(trsm_left<double>(lower, unit, !tr, n, m, a, lda, bt, m, s));
-  return launch_dcopy2d(m, n, bt, 1, m, b, ldb, s);
+  DDLA_ARG_CHECK(a && b, "null matrix pointer");
+  return trsm_rowmajor(side, uplo, transa, diag, m, n, *alpha, a, lda, b, ldb);
 }
 
 int ddla_dsyrk_dev(int uplo, int transa, ddla_dim_t m, ddla_dim_t k, const double* alpha,
@@ -544,7 +565,10 @@ int ddla_dsyrk_dev(int uplo, int transa, ddla_dim_t m, ddla_dim_t k, const doubl
   DDLA_ARG_CHECK(valid_trans(transa), "bad trans");
   DDLA_ARG_CHECK(alpha && beta, "null alpha/beta");
   DDLA_ARG_CHECK(m >= 0 && k >= 0 && ldc >= m, "bad syrk arguments");
+  DDLA_ARG_CHECK(lda >= (is_trans(transa) ? m : k), "lda %lld too small for op(A) of %lld x %lld",
+                 (long long)lda, (long long)m, (long long)k);
   if (m == 0) return 0;
+  DDLA_ARG_CHECK(c && (k == 0 || a), "null matrix pointer");
   const int tri = (uplo == DDLA_LOWER) ? 1 : 2;
   if (!is_trans(transa))  // C += A A^T, A m x k
     return launch_dgemm(m, m, k, *alpha, a, lda, true, a, lda, false, *beta, c, ldc, tri, stream());

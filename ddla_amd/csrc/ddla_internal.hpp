@@ -88,7 +88,9 @@ enum WorkspaceSlot {
   WS_TEMP2 = 11,  // row-interchange staging on the auxiliary stream
   WS_TSLU = 12,   // tournament-pivoting tree (candidate sets, arrival words)
   WS_SPLITP = 13, // split panel operand shared by the updates of one panel (two in flight)
-  WS_COUNT = 14,
+  WS_BLAS3 = 14,  // level-3 BLAS: the expanded triangular / symmetric operand
+  WS_BLAS3B = 15, // level-3 BLAS: the product before it is written back over B
+  WS_COUNT = 16,
 };
 
 // Optional per-kernel timing (ddla_profile_*): while enabled, a ProfScope around a launch
@@ -241,9 +243,10 @@ size_t trsv_dinv_elems(int64_t n);
 template <typename T>
 int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64_t lda, T* Dinv,
                  hipStream_t s);
+// skip (device word, may be null): when nonzero at the start of the launch the solve does nothing
 template <typename T>
 int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t lda, const T* Dinv,
-               T* B, int64_t ldb, hipStream_t s);
+               T* B, int64_t ldb, hipStream_t s, const int* skip = nullptr);
 // the same solve for many right-hand sides (FP32): GEMMs on the prepared diagonal inverses
 int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float* A, int64_t lda,
                        const float* Dinv, float* B, int64_t ldb, hipStream_t s);
@@ -260,6 +263,9 @@ template <typename T>
 int launch_getf2_nopiv(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s);
 int launch_dtranspose_inplace(int64_t n, double* a, int64_t lda, hipStream_t s);
 
+// relative pivots that stay inside n rows (0 <= i + piv[i] < n wherever piv[i] > 0), checked on
+// the device; synchronises s and returns DDLA_ERR_INVALID_ARG otherwise
+int check_pivots(int64_t n, const int32_t* piv, hipStream_t s);
 // pivots -> permutation (perm[i] = source row of row i after all interchanges)
 int launch_piv_to_perm(int64_t n, const int32_t* piv, int32_t* perm, hipStream_t s);
 template <typename T>

@@ -558,6 +558,25 @@ __global__ void piv_to_perm_kernel(int n, const int32_t* __restrict__ piv, int* 
   for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = perm[i];
 }
 
+// *bad = 1 + the first i with piv[i] > 0 and i + piv[i] >= n (a swap outside the matrix, where
+// the reference's SwapRow throws), or 0.  One workgroup; every entry is checked.
+__global__ void piv_check_kernel(int n, const int32_t* __restrict__ piv, int* __restrict__ bad) {
+  __shared__ int first[16];
+  int mine = INT_MAX;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int p = piv[i];
+    if (p > 0 && (int64_t)i + p >= n) mine = min(mine, i);
+  }
+  for (int off = 32; off > 0; off >>= 1) mine = min(mine, __shfl_xor(mine, off));
+  if ((threadIdx.x & 63) == 0) first[threadIdx.x >> 6] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int f = INT_MAX;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) f = min(f, first[w]);
+    *bad = f == INT_MAX ? 0 : f + 1;
+  }
+}
+
 // dst[i] = src[perm[i]] (forward) or dst[perm[i]] = src[i] (inverse), rows of ncols.
 template <typename T>
 __global__ void gather_rows_kernel(int64_t n, int64_t ncols, const T* __restrict__ src,
@@ -729,6 +748,24 @@ int launch_piv_to_perm(int64_t n, const int32_t* piv, int32_t* perm, hipStream_t
   }
   hipLaunchKernelGGL(piv_to_perm_kernel, dim3(1), dim3(256), sh, s, (int)n, piv, perm);
   DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int check_pivots(int64_t n, const int32_t* piv, hipStream_t s) {
+  if (n <= 0) return 0;
+  void* ws = nullptr;
+  DDLA_RETURN_IF(workspace(WS_SMALL, 64, &ws));
+  int* bad = static_cast<int*>(ws);
+  hipLaunchKernelGGL(piv_check_kernel, dim3(1), dim3(1024), 0, s, (int)n, piv, bad);
+  DDLA_HIP_CHECK(hipGetLastError());
+  int h = 0;
+  DDLA_HIP_CHECK(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  DDLA_HIP_CHECK(hipStreamSynchronize(s));
+  if (h != 0) {
+    set_error("the pivot at index %d swaps outside the %lld rows (SwapRow throws there)", h - 1,
+              (long long)n);
+    return DDLA_ERR_INVALID_ARG;
+  }
   return 0;
 }
 

@@ -318,10 +318,15 @@ __global__ void d2s_kernel(int64_t m, int64_t n, const double* __restrict__ src,
   }
 }
 
+// Kernels of a refinement step take `skip`, the done word of the device-side convergence
+// test (IrState::done, or null): once it is set, the steps queued after convergence do nothing.
+__device__ __forceinline__ bool skipped(const int* skip) { return skip && *skip; }
+
 // dst (float, n x nrhs contiguous) = (float) src[perm[i]]  (perm == nullptr: identity)
 __global__ void gather_d2s_kernel(int64_t n, int64_t nrhs, const double* __restrict__ src,
                                   int64_t lds, const int* __restrict__ perm,
-                                  float* __restrict__ dst) {
+                                  float* __restrict__ dst, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
   const int64_t total = n * nrhs;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -334,7 +339,8 @@ __global__ void gather_d2s_kernel(int64_t n, int64_t nrhs, const double* __restr
 // X += (double) D ; R = B
 __global__ void update_x_kernel(int64_t n, int64_t nrhs, const float* __restrict__ D,
                                 double* __restrict__ X, int64_t ldx, const double* __restrict__ B,
-                                int64_t ldb, double* __restrict__ R) {
+                                int64_t ldb, double* __restrict__ R, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
   const int64_t total = n * nrhs;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -380,7 +386,9 @@ __global__ __launch_bounds__(256) void col_inf_norms_kernel(int64_t n, int64_t n
                                                             const double* __restrict__ R,
                                                             const double* __restrict__ X,
                                                             int64_t ldx,
-                                                            unsigned long long* __restrict__ out) {
+                                                            unsigned long long* __restrict__ out,
+                                                            const int* __restrict__ skip) {
+  if (skipped(skip)) return;
   const int64_t j = blockIdx.y;
   double rm = 0.0, xm = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -409,7 +417,9 @@ template <int NR>
 __global__ __launch_bounds__(256) void dgemv_resid_kernel(int64_t n, const double* __restrict__ A,
                                                           int64_t lda, const double* __restrict__ X,
                                                           int64_t ldx, double* __restrict__ R,
-                                                          int64_t ldr, int nrhs) {
+                                                          int64_t ldr, int nrhs,
+                                                          const int* __restrict__ skip) {
+  if (skipped(skip)) return;
   constexpr int RPW = 4;
   const int64_t row0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW;
   if (row0 >= n) return;
@@ -448,10 +458,87 @@ __global__ __launch_bounds__(256) void dgemv_resid_kernel(int64_t n, const doubl
   }
 }
 
-__global__ void zero_u64_kernel(int64_t n, unsigned long long* p) {
+__global__ void zero_u64_kernel(int64_t n, unsigned long long* p, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = 0ull;
+}
+
+// Device-side state of a refinement: the convergence test runs on the GPU after every step,
+// so the host queues steps without waiting on each one.
+struct IrState {
+  int done;     // 1: converged, stagnated, not finite or out of steps (later steps do nothing)
+  int iters;    // steps taken
+  int slow;     // consecutive steps that lowered the backward error by less than 10%
+  int pad;
+  double berr;  // backward error after the last step
+  double prev;  // backward error before it
+};
+
+__global__ void ir_state_init_kernel(IrState* st) {
+  st->done = 0;
+  st->iters = 0;
+  st->slow = 0;
+  st->pad = 0;
+  st->berr = INFINITY;
+  st->prev = INFINITY;
+}
+
+// The convergence test of one refinement step, in FP64 on the GPU: berr = max_j ||r_j||_inf /
+// (||A||_inf ||x_j||_inf) from the norm words of col_inf_norms_kernel; done when berr <= tol,
+// after max_iters steps, when the error is not finite, or when three steps in a row lower it
+// by less than 10% (stagnation).
+__global__ void ir_check_kernel(int nrhs, const unsigned long long* __restrict__ norms,
+                                const double* __restrict__ normA, double tol, int max_iters,
+                                IrState* __restrict__ st) {
+  if (threadIdx.x != 0 || st->done) return;
+  const double na = *normA;
+  double worst = 0.0;
+  for (int j = 0; j < nrhs; ++j) {
+    const double r = __longlong_as_double((long long)norms[2 * j]);
+    const double x = __longlong_as_double((long long)norms[2 * j + 1]);
+    double e = (x > 0.0 && na > 0.0) ? r / (na * x) : (r == 0.0 ? 0.0 : INFINITY);
+    if (!isfinite(r) || !isfinite(x) || isnan(e)) e = INFINITY;  // NaN / Inf: failed
+    worst = fmax(worst, e);
+  }
+  st->iters += 1;
+  st->berr = worst;
+  if (!(worst > tol) || !isfinite(worst) || st->iters >= max_iters) {
+    st->done = 1;
+    return;
+  }
+  st->slow = (worst > 0.9 * st->prev) ? st->slow + 1 : 0;
+  if (st->slow >= 3) st->done = 1;
+  st->prev = worst;
+}
+
+// out[2j] = max_i |X[i][j] - Xr[i][j]|, out[2j+1] = max_i |Xr[i][j]| (bit patterns of
+// non-negative doubles, NaN as infinity; out cleared first), column j = blockIdx.y
+__global__ __launch_bounds__(256) void fwd_err_norms_kernel(int64_t n, const double* __restrict__ X,
+                                                            int64_t ldx,
+                                                            const double* __restrict__ Xr,
+                                                            int64_t ldr,
+                                                            unsigned long long* __restrict__ out) {
+  const int64_t j = blockIdx.y;
+  double dm = 0.0, rm = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double xr = Xr[i * ldr + j];
+    double d = fabs(X[i * ldx + j] - xr), r = fabs(xr);
+    if (d != d) d = INFINITY;
+    if (r != r) r = INFINITY;
+    dm = fmax(dm, d);
+    rm = fmax(rm, r);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    dm = fmax(dm, __shfl_xor(dm, off));
+    rm = fmax(rm, __shfl_xor(rm, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out + 2 * j, __double_as_longlong(dm));
+    atomicMax(out + 2 * j + 1, __double_as_longlong(rm));
+  }
 }
 
 int grid_for(int64_t total) {
@@ -533,7 +620,9 @@ __global__ __launch_bounds__(256) void rbt_matrix_kernel(int64_t n, int64_t np,
 
 // D (np x nrhs FP32) = U^T [R; 0], R n x nrhs FP64 (contiguous)
 __global__ void rbt_vec_kernel(int64_t n, int64_t np, int64_t nrhs, const double* __restrict__ Rv,
-                               const float* __restrict__ RU, float* __restrict__ D) {
+                               const float* __restrict__ RU, float* __restrict__ D,
+                               const int* __restrict__ skip) {
+  if (skipped(skip)) return;
   const int64_t n4 = np / 4;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n4 * nrhs;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -556,7 +645,8 @@ __global__ void rbt_vec_kernel(int64_t n, int64_t np, int64_t nrhs, const double
 __global__ void rbt_update_kernel(int64_t n, int64_t np, int64_t nrhs, const float* __restrict__ D,
                                   const float* __restrict__ RV, double* __restrict__ X,
                                   int64_t ldx, const double* __restrict__ B, int64_t ldb,
-                                  double* __restrict__ Rv) {
+                                  double* __restrict__ Rv, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
   const int64_t n4 = np / 4;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n4 * nrhs;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -704,6 +794,7 @@ struct MixedWs {
   double* rowsum;
   double* normA;
   unsigned long long* norms;
+  IrState* state;
   float* dinv1;  // inverted 64 x 64 diagonal blocks of the first / second triangular factor
   float* dinv2;
   float* rbt;    // butterfly diagonals: U then V (2 np each)
@@ -726,11 +817,13 @@ int mixed_ws(int64_t n, int64_t np, int64_t nrhs, MixedWs* w) {
   const size_t s_rs = al(n * sizeof(double));
   const size_t s_na = al(sizeof(double));
   const size_t s_no = al(2 * nrhs * sizeof(unsigned long long));
+  const size_t s_st = al(sizeof(IrState));
   const size_t s_di = al(trsv_dinv_elems(np) * sizeof(float));
   const size_t s_rb = al(4 * np * sizeof(float));
   void* p2 = nullptr;
   DDLA_RETURN_IF(workspace(
-      WS_MIXED2, s_scales + s_perm + s_piv + s_D + s_R + s_rs + s_na + s_no + 2 * s_di + s_rb, &p2));
+      WS_MIXED2,
+      s_scales + s_perm + s_piv + s_D + s_R + s_rs + s_na + s_no + s_st + 2 * s_di + s_rb, &p2));
   char* c = static_cast<char*>(p2);
   w->scales = reinterpret_cast<float*>(c); c += s_scales;
   w->perm = reinterpret_cast<int*>(c); c += s_perm;
@@ -740,6 +833,7 @@ int mixed_ws(int64_t n, int64_t np, int64_t nrhs, MixedWs* w) {
   w->rowsum = reinterpret_cast<double*>(c); c += s_rs;
   w->normA = reinterpret_cast<double*>(c); c += s_na;
   w->norms = reinterpret_cast<unsigned long long*>(c); c += s_no;
+  w->state = reinterpret_cast<IrState*>(c); c += s_st;
   w->dinv1 = reinterpret_cast<float*>(c); c += s_di;
   w->dinv2 = reinterpret_cast<float*>(c); c += s_di;
   w->rbt = reinterpret_cast<float*>(c);
@@ -998,14 +1092,12 @@ int spotrf_mixed(int64_t n, int prec, MixedWs& w, int32_t* info_host) {
   return 0;
 }
 
-// ||A||_inf into w.normA (device) and returned on the host
-int norm_inf(int64_t n, const double* A, int64_t lda, MixedWs& w, double* host) {
+// ||A||_inf into w.normA (device; read there by the convergence test)
+int norm_inf(int64_t n, const double* A, int64_t lda, MixedWs& w) {
   hipStream_t s = stream();
   hipLaunchKernelGGL(row_abs_sum_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, s, n, A, lda, w.rowsum);
   hipLaunchKernelGGL(max_kernel, dim3(1), dim3(1024), 0, s, n, w.rowsum, w.normA);
   DDLA_HIP_CHECK(hipGetLastError());
-  DDLA_HIP_CHECK(hipMemcpyAsync(host, w.normA, sizeof(double), hipMemcpyDeviceToHost, s));
-  DDLA_HIP_CHECK(hipStreamSynchronize(s));
   return 0;
 }
 
@@ -1014,93 +1106,113 @@ enum FactorKind { LU_PIVOT = 0, LU_BUTTERFLY = 1, CHOLESKY = 2, LU_TOURNAMENT = 
 
 // Classic refinement: X += M^-1 R, R = B - A X, with M the FP32 factor of kind `kind`.
 // Stops at backward error <= tol, after max_iters steps, or when three steps in a row fail
-// to lower the backward error by 10% (stagnation).
+// to lower the backward error by 10% (stagnation).  The test runs on the GPU after every step
+// (ir_check_kernel); the host queues STEPS steps at a time and reads the state once per group
+// (steps queued after convergence do nothing).  With more than 8 right-hand sides the
+// triangular solves and the residual are library GEMMs without the skip word, so the host
+// reads the state after every step there.
 int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
-           int64_t ldb, double* X, int64_t ldx, MixedWs& w, double normA, double tol, int max_iters,
-           int* iters, double* berr_out) {
+           int64_t ldb, double* X, int64_t ldx, MixedWs& w, double tol, int max_iters, int* iters,
+           double* berr_out) {
   hipStream_t s = stream();
   const int64_t tot = n * nrhs;
   const int64_t np = w.np;
   const int64_t totp = (np / 4) * nrhs;
   const float* RU = w.rbt;
   const float* RV = w.rbt + 2 * np;
+  const bool few = nrhs <= 8;
+  const int* skip = few ? &w.state->done : nullptr;
+  const int STEPS = few ? 4 : 1;
   // X = 0, R = B
   DDLA_RETURN_IF(launch_dscale2d(n, nrhs, 0.0, X, ldx, 1, 0, s));
   DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, B, ldb, 1, w.R, nrhs, s));
-  std::vector<unsigned long long> nh(2 * nrhs);
-  double berr = INFINITY, prev = INFINITY;
-  int slow = 0;
-  int it = 0;
-  for (; it < max_iters; ++it) {
-    // D = M^-1 R in FP32
-    if (kind == LU_BUTTERFLY)
-      hipLaunchKernelGGL(rbt_vec_kernel, dim3(grid_for(totp)), dim3(256), 0, s, n, np, nrhs, w.R,
-                         RU, w.D);
-    else
-      hipLaunchKernelGGL(gather_d2s_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.R,
-                         nrhs, kind == LU_PIVOT || kind == LU_TOURNAMENT ? w.perm : nullptr,
-                         w.D);
-    DDLA_HIP_CHECK(hipGetLastError());
-    const bool lu = kind != CHOLESKY;
-    // triangular solves with the inverted diagonal blocks (trsv.hip): the persistent
-    // latency-bound solve for up to 8 right-hand sides, GEMMs on the same blocks beyond
-    if (nrhs <= 8) {
-      DDLA_RETURN_IF(trsv_solve<float>(true, false, np, (int)nrhs, w.W, np, w.dinv1, w.D, nrhs, s));
-      DDLA_RETURN_IF(trsv_solve<float>(false, !lu, np, (int)nrhs, w.W, np, w.dinv2, w.D, nrhs, s));
-    } else {
-      DDLA_RETURN_IF(trsv_solve_blocked(true, false, np, (int)nrhs, w.W, np, w.dinv1, w.D, nrhs, s));
-      DDLA_RETURN_IF(trsv_solve_blocked(false, !lu, np, (int)nrhs, w.W, np, w.dinv2, w.D, nrhs, s));
-    }
-    // X += D (FP64, through V for the butterfly form); R = B - A X (FP64)
-    if (kind == LU_BUTTERFLY)
-      hipLaunchKernelGGL(rbt_update_kernel, dim3(grid_for(totp)), dim3(256), 0, s, n, np, nrhs, w.D,
-                         RV, X, ldx, B, ldb, w.R);
-    else
-      hipLaunchKernelGGL(update_x_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.D, X,
-                         ldx, B, ldb, w.R);
-    DDLA_HIP_CHECK(hipGetLastError());
-    if (nrhs <= 8) {
-      ProfScope prof("dgemv_resid_f64", 2.0 * n * n * nrhs, 8.0 * n * n, s);
-      const dim3 g((unsigned)ceil_div(n, 16));  // 4 waves x 4 rows per workgroup
-      if (nrhs == 1)
-        hipLaunchKernelGGL(dgemv_resid_kernel<1>, g, dim3(256), 0, s, n, A, lda, X, ldx, w.R,
-                           nrhs, (int)nrhs);
+  hipLaunchKernelGGL(ir_state_init_kernel, dim3(1), dim3(1), 0, s, w.state);
+  DDLA_HIP_CHECK(hipGetLastError());
+  IrState st{};
+  for (int queued = 0; queued < max_iters;) {
+    const int group = std::min(STEPS, max_iters - queued);
+    for (int q = 0; q < group; ++q) {
+      // D = M^-1 R in FP32
+      if (kind == LU_BUTTERFLY)
+        hipLaunchKernelGGL(rbt_vec_kernel, dim3(grid_for(totp)), dim3(256), 0, s, n, np, nrhs,
+                           w.R, RU, w.D, skip);
       else
-        hipLaunchKernelGGL(dgemv_resid_kernel<8>, g, dim3(256), 0, s, n, A, lda, X, ldx, w.R,
-                           nrhs, (int)nrhs);
-    } else {
-      DDLA_RETURN_IF(launch_dgemm(n, nrhs, n, -1.0, A, lda, true, X, ldx, true, 1.0, w.R, nrhs, 0, s));
+        hipLaunchKernelGGL(gather_d2s_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.R,
+                           nrhs, kind == LU_PIVOT || kind == LU_TOURNAMENT ? w.perm : nullptr,
+                           w.D, skip);
+      DDLA_HIP_CHECK(hipGetLastError());
+      const bool lu = kind != CHOLESKY;
+      // triangular solves with the inverted diagonal blocks (trsv.hip): the persistent
+      // latency-bound solve for up to 8 right-hand sides, GEMMs on the same blocks beyond
+      if (few) {
+        DDLA_RETURN_IF(trsv_solve<float>(true, false, np, (int)nrhs, w.W, np, w.dinv1, w.D, nrhs,
+                                         s, skip));
+        DDLA_RETURN_IF(trsv_solve<float>(false, !lu, np, (int)nrhs, w.W, np, w.dinv2, w.D, nrhs,
+                                         s, skip));
+      } else {
+        DDLA_RETURN_IF(trsv_solve_blocked(true, false, np, (int)nrhs, w.W, np, w.dinv1, w.D, nrhs, s));
+        DDLA_RETURN_IF(trsv_solve_blocked(false, !lu, np, (int)nrhs, w.W, np, w.dinv2, w.D, nrhs, s));
+      }
+      // X += D (FP64, through V for the butterfly form); R = B - A X (FP64)
+      if (kind == LU_BUTTERFLY)
+        hipLaunchKernelGGL(rbt_update_kernel, dim3(grid_for(totp)), dim3(256), 0, s, n, np, nrhs,
+                           w.D, RV, X, ldx, B, ldb, w.R, skip);
+      else
+        hipLaunchKernelGGL(update_x_kernel, dim3(grid_for(tot)), dim3(256), 0, s, n, nrhs, w.D, X,
+                           ldx, B, ldb, w.R, skip);
+      DDLA_HIP_CHECK(hipGetLastError());
+      if (few) {
+        ProfScope prof("dgemv_resid_f64", 2.0 * n * n * nrhs, 8.0 * n * n, s);
+        const dim3 g((unsigned)ceil_div(n, 16));  // 4 waves x 4 rows per workgroup
+        if (nrhs == 1)
+          hipLaunchKernelGGL(dgemv_resid_kernel<1>, g, dim3(256), 0, s, n, A, lda, X, ldx, w.R,
+                             nrhs, (int)nrhs, skip);
+        else
+          hipLaunchKernelGGL(dgemv_resid_kernel<8>, g, dim3(256), 0, s, n, A, lda, X, ldx, w.R,
+                             nrhs, (int)nrhs, skip);
+      } else {
+        DDLA_RETURN_IF(launch_dgemm(n, nrhs, n, -1.0, A, lda, true, X, ldx, true, 1.0, w.R, nrhs,
+                                    0, s));
+      }
+      hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, s, 2 * nrhs, w.norms, skip);
+      hipLaunchKernelGGL(col_inf_norms_kernel,
+                         dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 64), (unsigned)nrhs),
+                         dim3(256), 0, s, n, nrhs, w.R, X, ldx, w.norms, skip);
+      hipLaunchKernelGGL(ir_check_kernel, dim3(1), dim3(64), 0, s, (int)nrhs, w.norms, w.normA,
+                         tol, max_iters, w.state);
+      DDLA_HIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, s, 2 * nrhs, w.norms);
-    hipLaunchKernelGGL(col_inf_norms_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 64),
-                                                  (unsigned)nrhs),
-                       dim3(256), 0, s, n, nrhs, w.R, X, ldx, w.norms);
-    DDLA_HIP_CHECK(hipGetLastError());
-    DDLA_HIP_CHECK(hipMemcpyAsync(nh.data(), w.norms, nh.size() * sizeof(unsigned long long),
-                                  hipMemcpyDeviceToHost, s));
+    queued += group;
+    DDLA_HIP_CHECK(hipMemcpyAsync(&st, w.state, sizeof(IrState), hipMemcpyDeviceToHost, s));
     DDLA_HIP_CHECK(hipStreamSynchronize(s));
-    double worst = 0.0;
-    for (int64_t j = 0; j < nrhs; ++j) {
-      const double r = __builtin_bit_cast(double, nh[2 * j]);
-      const double x = __builtin_bit_cast(double, nh[2 * j + 1]);
-      double e = (x > 0.0 && normA > 0.0) ? r / (normA * x) : (r == 0.0 ? 0.0 : INFINITY);
-      if (!std::isfinite(r) || !std::isfinite(x) || std::isnan(e)) e = INFINITY;  // NaN/Inf: failed
-      worst = std::max(worst, e);
-    }
-    berr = worst;
-    if (!(berr > tol)) {
-      ++it;
-      break;
-    }
-    slow = (berr > 0.9 * prev) ? slow + 1 : 0;
-    if (slow >= 3 || !std::isfinite(berr)) {
-      ++it;
-      break;
-    }
-    prev = berr;
+    if (st.done) break;
   }
-  *iters = it;
-  *berr_out = berr;
+  *iters = st.iters;
+  *berr_out = st.berr;
+  return 0;
+}
+
+// max over columns of ||x_j - xref_j||_inf / ||xref_j||_inf
+int forward_error(int64_t n, int64_t nrhs, const double* X, int64_t ldx, const double* Xr,
+                  int64_t ldr, MixedWs& w, double* out) {
+  hipStream_t s = stream();
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, s, 2 * nrhs, w.norms, nullptr);
+  hipLaunchKernelGGL(fwd_err_norms_kernel,
+                     dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 64), (unsigned)nrhs),
+                     dim3(256), 0, s, n, X, ldx, Xr, ldr, w.norms);
+  DDLA_HIP_CHECK(hipGetLastError());
+  std::vector<unsigned long long> h(2 * nrhs);
+  DDLA_HIP_CHECK(hipMemcpyAsync(h.data(), w.norms, h.size() * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, s));
+  DDLA_HIP_CHECK(hipStreamSynchronize(s));
+  double worst = 0.0;
+  for (int64_t j = 0; j < nrhs; ++j) {
+    const double d = __builtin_bit_cast(double, h[2 * j]);
+    const double r = __builtin_bit_cast(double, h[2 * j + 1]);
+    const double e = r > 0.0 ? d / r : (d == 0.0 ? 0.0 : INFINITY);
+    worst = std::max(worst, std::isnan(e) ? INFINITY : e);
+  }
+  *out = worst;
   return 0;
 }
 
@@ -1115,6 +1227,10 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
     set_error("factor precision %d not supported by the mixed solver", prec);
     return DDLA_ERR_UNSUPPORTED;
   }
+  if (opts && opts->gmres != 0) {
+    set_error("GMRES refinement is not available (opts->gmres must be 0)");
+    return DDLA_ERR_UNSUPPORTED;
+  }
   FactorKind kind = lu ? LU_PIVOT : CHOLESKY;
   if (lu && opts && opts->pivoting == DDLA_PIVOT_BUTTERFLY) kind = LU_BUTTERFLY;
   if (lu && opts && opts->pivoting == DDLA_PIVOT_TOURNAMENT) kind = LU_TOURNAMENT;
@@ -1124,8 +1240,7 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
   // default split: FP16 x2 (22 bits) and BF16 x3 (24 bits) reach FP32 product accuracy
   w.terms = (opts && opts->terms > 0) ? std::min(opts->terms, 3)
                                       : (prec == DDLA_PREC_BF16 ? 3 : 2);
-  double normA = 0.0;
-  DDLA_RETURN_IF(norm_inf(n, a, lda, w, &normA));
+  DDLA_RETURN_IF(norm_inf(n, a, lda, w));
   Timer t;
   double factor_ms = 0.0, refine_ms = 0.0;
   int iters = 0;
@@ -1171,7 +1286,7 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
     factor_ms += t.stop_ms(s);
     if (finfo == 0) {
       t.start(s);
-      DDLA_RETURN_IF(refine(kind, n, nrhs, a, lda, b, ldb, x, ldx, w, normA, tol, max_iters, &iters,
+      DDLA_RETURN_IF(refine(kind, n, nrhs, a, lda, b, ldb, x, ldx, w, tol, max_iters, &iters,
                             &berr));
       refine_ms += t.stop_ms(s);
     }
@@ -1194,6 +1309,9 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
   }
   out.iters = iters;
   out.backward_error = berr;
+  out.forward_error = -1.0;
+  if (opts && opts->x_ref)
+    DDLA_RETURN_IF(forward_error(n, nrhs, x, ldx, opts->x_ref, opts->ld_xref, w, &out.forward_error));
   out.status = (berr <= tol) ? 0 : DDLA_ERR_NOT_CONVERGED;
   if (info) *info = out;
   if (out.status) set_error("refinement stopped at backward error %.3e after %d steps", berr, iters);
@@ -1228,6 +1346,7 @@ int ddla_dgesv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc
   DDLA_RETURN_IF(ensure_init());
   DDLA_ARG_CHECK(n >= 0 && nrhs >= 0 && lda >= n && ldb >= nrhs && ldx >= nrhs,
                  "bad gesv arguments");
+  DDLA_ARG_CHECK(!opts || !opts->x_ref || opts->ld_xref >= nrhs, "ld_xref below nrhs");
   if (n == 0 || nrhs == 0) return 0;
   DDLA_ARG_CHECK(n <= 32768, "n = %lld above the supported 32768", (long long)n);
   return mpir(true, n, nrhs, a, lda, b, ldb, x, ldx, opts, info);
@@ -1252,6 +1371,7 @@ int ddla_dposv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc
   DDLA_RETURN_IF(ensure_init());
   DDLA_ARG_CHECK(n >= 0 && nrhs >= 0 && lda >= n && ldb >= nrhs && ldx >= nrhs,
                  "bad posv arguments");
+  DDLA_ARG_CHECK(!opts || !opts->x_ref || opts->ld_xref >= nrhs, "ld_xref below nrhs");
   if (n == 0 || nrhs == 0) return 0;
   return mpir(false, n, nrhs, a, lda, b, ldb, x, ldx, opts, info);
 }

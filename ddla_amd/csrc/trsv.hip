@@ -225,7 +225,10 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
                                                    int64_t lda, const T* __restrict__ Dinv,
                                                    T* __restrict__ B, int64_t ldb,
                                                    u64* __restrict__ XW, unsigned tag,
-                                                   int split) {
+                                                   int split, const int* __restrict__ skip) {
+  // a refinement that has already converged queues its remaining steps as no-ops: every
+  // workgroup reads the same word (written by an earlier kernel) and leaves before any wait
+  if (skip && *skip) return;
   constexpr int Q = 256 / BS;       // threads per row of a block product (k split)
   constexpr int KQ = BS / Q;        // k range per thread
   constexpr int E = BS * BS / 256;  // staged block elements per thread
@@ -665,7 +668,7 @@ size_t trsv_dinv_elems(int64_t n) {
 // op(T) X = B in place, nrhs <= 8, with Dinv from trsv_prepare (same lower/trans).
 template <typename T>
 int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t lda, const T* Dinv,
-               T* B, int64_t ldb, hipStream_t s) {
+               T* B, int64_t ldb, hipStream_t s, const int* skip) {
   if (n <= 0 || nrhs <= 0) return 0;
   if (nrhs > RMAX) {
     set_error("trsv: nrhs %d > %d", nrhs, RMAX);
@@ -688,7 +691,7 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
   int64_t lda_ = lda, ldb_ = ldb;
   unsigned tag_ = tag;
   int split_ = 0;
-  void* args[] = {&n_, &nrhs_, &A, &lda_, &Dinv, &B, &ldb_, &XW, &tag_, &split_};
+  void* args[] = {&n_, &nrhs_, &A, &lda_, &Dinv, &B, &ldb_, &XW, &tag_, &split_, &skip};
   // a helper workgroup per block row when twice the rows fit co-resident: the longest rows
   // stream their blocks with two workgroups
 #define TS(NR, L, TR)                                                                    \
@@ -720,8 +723,8 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
 template int trsv_prepare<float>(bool, bool, bool, int64_t, const float*, int64_t, float*, hipStream_t);
 template int trsv_prepare<double>(bool, bool, bool, int64_t, const double*, int64_t, double*, hipStream_t);
 template int trsv_solve<float>(bool, bool, int64_t, int, const float*, int64_t, const float*, float*,
-                               int64_t, hipStream_t);
+                               int64_t, hipStream_t, const int*);
 template int trsv_solve<double>(bool, bool, int64_t, int, const double*, int64_t, const double*,
-                                double*, int64_t, hipStream_t);
+                                double*, int64_t, hipStream_t, const int*);
 
 }  // namespace ddla

@@ -30,13 +30,14 @@ cint = ctypes.c_int
 
 class MpirOpts(ctypes.Structure):
     _fields_ = [("factor_prec", i32), ("max_iters", i32), ("tol", ctypes.c_double),
-                ("gmres", i32), ("terms", i32), ("pivoting", i32), ("reserved", i32)]
+                ("gmres", i32), ("terms", i32), ("pivoting", i32), ("reserved", i32),
+                ("x_ref", vp), ("ld_xref", i64)]
 
 
 class MpirInfo(ctypes.Structure):
     _fields_ = [("status", i32), ("iters", i32), ("inner_iters", i32), ("fallback", i32),
                 ("backward_error", ctypes.c_double), ("factor_ms", ctypes.c_double),
-                ("refine_ms", ctypes.c_double)]
+                ("refine_ms", ctypes.c_double), ("forward_error", ctypes.c_double)]
 
 
 # name -> argtypes (restype is always int)
@@ -77,6 +78,25 @@ SIGNATURES = {
     "ddla_dtrsm_dev": [cint, cint, cint, cint, i64, i64, dp, vp, i64, vp, i64],
     "ddla_dsyrk_dev": [cint, cint, i64, i64, dp, vp, i64, dp, vp, i64],
     "ddla_damaxv_dev": [i64, vp, i64, ctypes.POINTER(i64)],
+    # level-3 BLAS with BLIS argument lists (host and strided-device forms)
+    "ddla_dgemmt": [cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp, i64, i64],
+    "ddla_dgemmt_dev": [cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp, i64,
+                        i64],
+    "ddla_dsyrk": [cint, cint, i64, i64, dp, vp, i64, i64, dp, vp, i64, i64],
+    "ddla_dsyr2k": [cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp, i64, i64],
+    "ddla_dsyr2k_dev": [cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp, i64,
+                        i64],
+    "ddla_dsymm": [cint, cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp, i64,
+                   i64],
+    "ddla_dsymm_dev": [cint, cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp,
+                       i64, i64],
+    "ddla_dtrmm": [cint, cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64],
+    "ddla_dtrmm_dev": [cint, cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64],
+    "ddla_dtrmm3": [cint, cint, cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp,
+                    i64, i64],
+    "ddla_dtrmm3_dev": [cint, cint, cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp,
+                        vp, i64, i64],
+    "ddla_dtrsm": [cint, cint, cint, cint, i64, i64, dp, vp, i64, i64, vp, i64, i64],
     "ddla_dgesv_mpir_dev": [i64, i64, vp, i64, vp, i64, vp, i64, ctypes.POINTER(MpirOpts),
                             ctypes.POINTER(MpirInfo)],
     "ddla_dposv_mpir_dev": [i64, i64, vp, i64, vp, i64, vp, i64, ctypes.POINTER(MpirOpts),
@@ -89,6 +109,26 @@ SIGNATURES = {
 }
 
 _lib = None
+_finalized = False
+
+
+def finalized() -> bool:
+    """True once the interpreter is exiting and the library context has been released: objects
+    that still hold device memory then leave it to the process's end instead of calling into
+    HIP, whose runtime (or a profiler wrapped around it) may already be torn down."""
+    return _finalized
+
+
+def finalize(at_exit: bool = False):
+    """Releases the library context (streams, events, workspaces; ddla_init sets it up again).
+    Registered with atexit (at_exit=True) when the library is loaded, so the release runs while
+    the HIP runtime is still up, before the interpreter's teardown collects the objects left."""
+    global _finalized
+    if _lib is None or _finalized:
+        return
+    if at_exit:
+        _finalized = True
+    _lib.ddla_finalize()
 
 
 def header_symbols() -> list[str]:
@@ -112,6 +152,8 @@ def lib() -> ctypes.CDLL:
         fn.argtypes = args
         fn.restype = cint
     _lib = L
+    import atexit
+    atexit.register(finalize, True)
     return L
 
 

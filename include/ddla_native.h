@@ -135,7 +135,9 @@ int ddla_dgemm_dev(int transa, int transb, ddla_dim_t m, ddla_dim_t n, ddla_dim_
 int ddla_dgetrf_dev(ddla_dim_t n, double* a, ddla_inc_t lda, int32_t* dpiv);
 /* Solves A X = B (trans == DDLA_NO_TRANSPOSE) or A^T X = B with the factors of
  * ddla_dgetrf_dev; B (n x nrhs row-major, device) is overwritten with X.
- * Replaces LU.PLUSolve (LU.cs:266-287, PLUSolveInternel :431-465). */
+ * Replaces LU.PLUSolve (LU.cs:266-287, PLUSolveInternel :431-465).  The pivots are checked on
+ * the device first (a pivot that would swap outside the matrix returns DDLA_ERR_INVALID_ARG,
+ * where the reference's SwapRow throws), which synchronises the stream. */
 int ddla_dgetrs_dev(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
                     const int32_t* dpiv, double* b, ddla_inc_t ldb);
 /* Host-pointer forms with strided views (synchronous). */
@@ -143,7 +145,8 @@ int ddla_dgetrf(ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t csa, int32_t
 int ddla_dgetrs(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
                 ddla_inc_t csa, const int32_t* piv, double* b, ddla_inc_t rsb, ddla_inc_t csb);
 /* Row interchanges with relative pivots: side DDLA_LEFT swaps rows, DDLA_RIGHT columns;
- * forward applies P, backward P^-1.  Replaces LU.ApplyPiv (LU.cs:237-249, :376-429). */
+ * forward applies P, backward P^-1.  Replaces LU.ApplyPiv (LU.cs:237-249, :376-429).  Pivots
+ * are range-checked on the device first, as in ddla_dgetrs_dev. */
 int ddla_dlaswp_dev(int side, int forward, ddla_dim_t m, ddla_dim_t n, double* x, ddla_inc_t ldx,
                     const int32_t* dpiv, ddla_dim_t npiv);
 
@@ -170,18 +173,82 @@ int ddla_dpotrs(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_i
                 ddla_inc_t csa, double* b, ddla_inc_t rsb, ddla_inc_t csb);
 
 /* ------------------------------------------------------------ level-3 pieces ------- */
-/* B = alpha * op(A)^-1 B (left) or B op(A)^-1 (right), row-major device matrices.
- * Replaces bli_dtrsm (DDLA/BLAS/Native/3m.cs:151-162) behind BlasProvider.TrSM
- * (DDLA/BLAS/3m.cs:287-314). */
+/* B = alpha * op(A)^-1 B (left) or B op(A)^-1 (right), row-major device matrices with leading
+ * dimensions: the device-resident form of bli_dtrsm (DDLA/BLAS/Native/3m.cs:151-162) for the
+ * factorizations' row-major buffers; ddla_dtrsm below has bli_dtrsm's own argument list. */
 int ddla_dtrsm_dev(int side, int uplo, int transa, int diag, ddla_dim_t m, ddla_dim_t n,
                    const double* alpha, const double* a, ddla_inc_t lda, double* b,
                    ddla_inc_t ldb);
-/* uplo triangle of C = beta*C + alpha*op(A) op(A)^T, row-major device matrices.
- * Replaces bli_dsyrk (DDLA/BLAS/Native/3m.cs:96-106) behind BlasProvider.SyRk
- * (DDLA/BLAS/3m.cs:118-147). */
+/* uplo triangle of C = beta*C + alpha*op(A) op(A)^T, row-major device matrices with leading
+ * dimensions: the device-resident form of bli_dsyrk (DDLA/BLAS/Native/3m.cs:96-106);
+ * ddla_dsyrk below has bli_dsyrk's own argument list. */
 int ddla_dsyrk_dev(int uplo, int transa, ddla_dim_t m, ddla_dim_t k, const double* alpha,
                    const double* a, ddla_inc_t lda, const double* beta, double* c,
                    ddla_inc_t ldc);
+/* Level-3 BLAS with exactly BLIS's argument lists (strided views, scalars by pointer), so the
+ * C# `Source` declarations of DDLA/BLAS/Native/3m.cs retarget by changing only the library
+ * name and entry point.  Host-pointer forms are synchronous; `_dev` forms take device
+ * pointers with any strides and are ordered on the library stream.  conja is ConjType
+ * (DDLA/Misc/Flags/ConjType.cs: 0 or 0x10), the identity for real data.
+ * C (m x m, uploc triangle only) = beta*C + alpha*op(A)*op(B), op(A) m x k, op(B) k x m.
+ * Replaces bli_dgemmt (Native/3m.cs:26-38) behind BlasProvider.GeMMT (BLAS/3m.cs:44-70). */
+int ddla_dgemmt(int uploc, int transa, int transb, ddla_dim_t m, ddla_dim_t k, const double* alpha,
+                const double* a, ddla_inc_t rsa, ddla_inc_t csa, const double* b, ddla_inc_t rsb,
+                ddla_inc_t csb, const double* beta, double* c, ddla_inc_t rsc, ddla_inc_t csc);
+int ddla_dgemmt_dev(int uploc, int transa, int transb, ddla_dim_t m, ddla_dim_t k,
+                    const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa,
+                    const double* b, ddla_inc_t rsb, ddla_inc_t csb, const double* beta,
+                    double* c, ddla_inc_t rsc, ddla_inc_t csc);
+/* uploc triangle of C = beta*C + alpha*op(A)*op(A)^T.  Replaces bli_dsyrk (Native/3m.cs:96-106)
+ * behind BlasProvider.SyRk (BLAS/3m.cs:118-138). */
+int ddla_dsyrk(int uploc, int transa, ddla_dim_t m, ddla_dim_t k, const double* alpha,
+               const double* a, ddla_inc_t rsa, ddla_inc_t csa, const double* beta, double* c,
+               ddla_inc_t rsc, ddla_inc_t csc);
+/* uploc triangle of C = beta*C + alpha*op(A)*op(B)^T + alpha*op(B)*op(A)^T, op(A), op(B) m x k.
+ * Replaces bli_dsyr2k (Native/3m.cs:108-120) behind BlasProvider.SyR2k (BLAS/3m.cs:149-172). */
+int ddla_dsyr2k(int uploc, int transa, int transb, ddla_dim_t m, ddla_dim_t k, const double* alpha,
+                const double* a, ddla_inc_t rsa, ddla_inc_t csa, const double* b, ddla_inc_t rsb,
+                ddla_inc_t csb, const double* beta, double* c, ddla_inc_t rsc, ddla_inc_t csc);
+int ddla_dsyr2k_dev(int uploc, int transa, int transb, ddla_dim_t m, ddla_dim_t k,
+                    const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa,
+                    const double* b, ddla_inc_t rsb, ddla_inc_t csb, const double* beta,
+                    double* c, ddla_inc_t rsc, ddla_inc_t csc);
+/* C (m x n) = beta*C + alpha*S*op(B) (sidea LEFT, S m x m) or alpha*op(B)*S (RIGHT, S n x n),
+ * S symmetric with only its uploa triangle read.  Replaces bli_dsymm (Native/3m.cs:81-94)
+ * behind BlasProvider.SyMM (BLAS/3m.cs:83-106). */
+int ddla_dsymm(int sidea, int uploa, int conja, int transb, ddla_dim_t m, ddla_dim_t n,
+               const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa,
+               const double* b, ddla_inc_t rsb, ddla_inc_t csb, const double* beta, double* c,
+               ddla_inc_t rsc, ddla_inc_t csc);
+int ddla_dsymm_dev(int sidea, int uploa, int conja, int transb, ddla_dim_t m, ddla_dim_t n,
+                   const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa,
+                   const double* b, ddla_inc_t rsb, ddla_inc_t csb, const double* beta, double* c,
+                   ddla_inc_t rsc, ddla_inc_t csc);
+/* B (m x n) = alpha*op(T)*B (LEFT) or alpha*B*op(T) (RIGHT), T the uploa triangle of A with
+ * a unit diagonal when diaga is DDLA_UNIT_DIAG.  Replaces bli_dtrmm (Native/3m.cs:122-133)
+ * behind BlasProvider.TrMM (BLAS/3m.cs:185-204); op(T) honours transa as BLIS does (the
+ * managed TrMM, BLAS/Managed/3m.cs:259-287, does not apply aTrans). */
+int ddla_dtrmm(int sidea, int uploa, int transa, int diaga, ddla_dim_t m, ddla_dim_t n,
+               const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa, double* b,
+               ddla_inc_t rsb, ddla_inc_t csb);
+int ddla_dtrmm_dev(int sidea, int uploa, int transa, int diaga, ddla_dim_t m, ddla_dim_t n,
+                   const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa, double* b,
+                   ddla_inc_t rsb, ddla_inc_t csb);
+/* C (m x n) = beta*C + alpha*op(T)*op(B) (LEFT) or alpha*op(B)*op(T) (RIGHT).  Replaces
+ * bli_dtrmm3 (Native/3m.cs:135-149) behind BlasProvider.TrMM3 (BLAS/3m.cs:225-250). */
+int ddla_dtrmm3(int sidea, int uploa, int transa, int diaga, int transb, ddla_dim_t m,
+                ddla_dim_t n, const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa,
+                const double* b, ddla_inc_t rsb, ddla_inc_t csb, const double* beta, double* c,
+                ddla_inc_t rsc, ddla_inc_t csc);
+int ddla_dtrmm3_dev(int sidea, int uploa, int transa, int diaga, int transb, ddla_dim_t m,
+                    ddla_dim_t n, const double* alpha, const double* a, ddla_inc_t rsa,
+                    ddla_inc_t csa, const double* b, ddla_inc_t rsb, ddla_inc_t csb,
+                    const double* beta, double* c, ddla_inc_t rsc, ddla_inc_t csc);
+/* Solves op(T)*X = alpha*B (LEFT) or X*op(T) = alpha*B (RIGHT), X over B, strided host views.
+ * Replaces bli_dtrsm (Native/3m.cs:151-162) behind BlasProvider.TrSM (BLAS/3m.cs:287-305). */
+int ddla_dtrsm(int sidea, int uploa, int transa, int diaga, ddla_dim_t m, ddla_dim_t n,
+               const double* alpha, const double* a, ddla_inc_t rsa, ddla_inc_t csa, double* b,
+               ddla_inc_t rsb, ddla_inc_t csb);
 /* Index of the first max |x_i| (NaN smallest; empty -> 0), written to *index (host).
  * Replaces bli_damaxv (DDLA/BLAS/Native/1v.cs:26-31) behind BlasProvider.AMax
  * (DDLA/BLAS/1v.cs:30-41). */
@@ -199,7 +266,8 @@ typedef struct ddla_mpir_opts {
   int32_t factor_prec;   /* DDLA_PREC_FP16 / _BF16 / _FP32 / _FP64 */
   int32_t max_iters;     /* refinement steps before giving up (default 50 if <= 0) */
   double tol;            /* target normwise backward error (default 1e-14 if <= 0) */
-  int32_t gmres;         /* reserved, must be 0 (classic refinement) */
+  int32_t gmres;         /* must be 0 (classic refinement); other values are rejected with
+                            DDLA_ERR_UNSUPPORTED */
   int32_t terms;         /* low-precision parts per FP32 operand in the trailing updates:
                             1 = one FP16/BF16 rounding, 2 or 3 = split x = h + m (+ l) with
                             the cross products summed; 0 = default (FP16: 2, BF16: 3) */
@@ -211,7 +279,10 @@ typedef struct ddla_mpir_opts {
                             eliminations, as in CALU).  BUTTERFLY and TOURNAMENT fall back to
                             partial pivoting when their factor breaks down or does not refine
                             (fallback = 2 / 3) */
-  int32_t reserved;
+  int32_t reserved;      /* must be 0 */
+  const double* x_ref;   /* optional (device, n x nrhs, leading dimension ld_xref): a reference
+                            solution; info->forward_error is measured against it */
+  int64_t ld_xref;
 } ddla_mpir_opts;
 
 typedef struct ddla_mpir_info {
@@ -224,10 +295,15 @@ typedef struct ddla_mpir_info {
   double backward_error; /* max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf) */
   double factor_ms;      /* GPU time of the factorization */
   double refine_ms;      /* GPU time of the refinement */
+  double forward_error;  /* max over columns of ||x - x_ref||_inf / ||x_ref||_inf when opts->x_ref
+                            is given, else -1 */
 } ddla_mpir_info;
 
 /* Solves A X = B for a row-major n x n FP64 device matrix A (not modified) and n x nrhs
- * B, writing X (n x nrhs, ldx).  LU with partial pivoting in the factor precision. */
+ * B, writing X (n x nrhs, ldx).  LU with partial pivoting in the factor precision.  The
+ * residual, the correction and the convergence test run in FP64 on the GPU; the host queues
+ * refinement steps in groups and reads the test's state once per group (steps queued after
+ * convergence do nothing). */
 int ddla_dgesv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
                         const double* b, ddla_inc_t ldb, double* x, ddla_inc_t ldx,
                         const ddla_mpir_opts* opts, ddla_mpir_info* info);

@@ -138,6 +138,74 @@ def syrk(uplo, trans, alpha, A, beta, C):
     return C
 
 
+def _op(X, trans):
+    return X.T if trans == TRANS else X
+
+
+def _expand(A, uplo, sym=False, unit=False):
+    """The dense matrix a BLIS routine reads from the uplo triangle of A: symmetric (sym) or
+    triangular with zeros outside the triangle (ones on the diagonal when unit)."""
+    A = np.asarray(A, dtype=np.float64)
+    tri = np.tril(A) if uplo == LOWER else np.triu(A)
+    if sym:
+        return tri + tri.T - np.diag(np.diag(A))
+    if unit:
+        tri = tri - np.diag(np.diag(tri)) + np.eye(A.shape[0])
+    return tri
+
+
+def _keep_triangle(C, full, uplo):
+    mask = np.tril(np.ones(C.shape, bool)) if uplo == LOWER else np.triu(np.ones(C.shape, bool))
+    C[mask] = full[mask]
+    return C
+
+
+def gemmt(uplo, transa, transb, alpha, A, B, beta, C):
+    """GeMMT (facade DDLA/BLAS/3m.cs:44-70 -> bli_dgemmt; managed DDLA/BLAS/Managed/3m.cs:51-131):
+    the uplo triangle of C = beta*C + alpha*op(A)*op(B), the other triangle untouched.  The
+    triangle takes the restated managed GEMM's sums."""
+    full = gemm(alpha, np.ascontiguousarray(A), np.ascontiguousarray(B), beta,
+                np.ascontiguousarray(C, dtype=np.float64).copy(), transa, transb)
+    return _keep_triangle(C, full, uplo)
+
+
+def syr2k(uplo, transa, transb, alpha, A, B, beta, C):
+    """SyR2k (facade DDLA/BLAS/3m.cs:149-172; managed Managed/3m.cs:227-245: two GeMMT calls,
+    the second with beta = 1)."""
+    gemmt(uplo, transa, NO_TRANS, alpha, A, np.ascontiguousarray(_op(B, transb).T), beta, C)
+    return gemmt(uplo, transb, NO_TRANS, alpha, B, np.ascontiguousarray(_op(A, transa).T), 1.0, C)
+
+
+def symm(side, uplo, transb, alpha, A, B, beta, C):
+    """SyMM with BLIS's semantics (facade DDLA/BLAS/3m.cs:83-106 -> bli_dsymm): C = beta*C +
+    alpha*S*op(B) (LEFT) or alpha*op(B)*S (RIGHT), S symmetric from the uplo triangle of A."""
+    S = _expand(A, uplo, sym=True)
+    Bo = np.ascontiguousarray(_op(B, transb))
+    if side == LEFT:
+        return gemm(alpha, S, Bo, beta, C)
+    return gemm(alpha, Bo, S, beta, C)
+
+
+def trmm3(side, uplo, transa, diag, transb, alpha, A, B, beta, C):
+    """TrMM3 with BLIS's semantics (facade DDLA/BLAS/3m.cs:225-250 -> bli_dtrmm3): C = beta*C +
+    alpha*op(T)*op(B) (LEFT) or alpha*op(B)*op(T) (RIGHT), T the uplo triangle of A."""
+    T = np.ascontiguousarray(_op(_expand(A, uplo, unit=diag == UNIT), transa))
+    Bo = np.ascontiguousarray(_op(B, transb))
+    if side == LEFT:
+        return gemm(alpha, T, Bo, beta, C)
+    return gemm(alpha, Bo, T, beta, C)
+
+
+def trmm(side, uplo, trans, diag, alpha, A, B):
+    """TrMM with BLIS's semantics (facade DDLA/BLAS/3m.cs:185-204 -> bli_dtrmm): B = alpha*op(T)*B
+    (LEFT) or alpha*B*op(T) (RIGHT).  The managed TrMM (Managed/3m.cs:259-287) does not apply
+    aTrans; the drop-in follows BLIS, which does."""
+    out = np.zeros(B.shape)
+    trmm3(side, uplo, trans, diag, NO_TRANS, alpha, A, np.ascontiguousarray(B), 0.0, out)
+    B[...] = out
+    return B
+
+
 def getrf(A):
     """PLUDecompose (LU.cs:251-264) in place. Returns (info, relative pivots int32[n])."""
     n = A.shape[0]

@@ -231,3 +231,67 @@ def test_cholesky_single_block_matches_unblocked_exactly(ddla, oracle, n):
     ddla.CholDecompose(ddla.LOWER, A)
     np.testing.assert_array_equal(np.tril(A), np.tril(So))
     np.testing.assert_array_equal(np.triu(A, 1), np.triu(S, 1))
+
+
+@pytest.mark.slow
+def test_lu_8320_panel_with_128_rows_per_workgroup(ddla, oracle):
+    # the first panels have more than 8192 rows, so the panel kernel runs with 128 rows per
+    # workgroup (the form the n = 16384 factorizations use); the pivot sequence must still be
+    # the oracle's
+    n = 8320
+    A0 = oracle.uniform_matrix(n, n, SEED_C2 + 7)
+    Ao = A0.copy()
+    info, piv_o = oracle.getrf(Ao)
+    assert info == 0
+    dA = ddla.DeviceMatrix.from_host(A0)
+    piv = ddla.PLUDecompose(dA)
+    mism = np.nonzero(piv != piv_o)[0]
+    assert mism.size == 0, f"first divergent pivot at step {mism[0] if mism.size else -1}"
+    assert rel_fro(dA.to_host(), Ao) <= 1e-13 * n
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("prec", [2, 3])  # FP16, BF16 trailing updates
+def test_mixed_factor_8320_reconstructs(ddla, prec):
+    # the mixed LU's FP32 factor through the 128-rows-per-workgroup panel: P A = L U on a
+    # sample of rows (the full product would cost the host minutes)
+    n = 8320
+    A = np.random.default_rng(prec).uniform(-1, 1, (n, n)).astype(np.float32)
+    LU, perm, info = ddla.sgetrf_mp(A, prec, ddla.PIVOT_PARTIAL)
+    assert info == 0
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert np.abs(np.tril(LU, -1)).max() <= 1.0 + 2.0 ** -22
+    rows = np.sort(np.random.default_rng(9).choice(n, 192, replace=False))
+    rows = np.unique(np.r_[rows, [0, 1, 127, 128, 8191, 8192, n - 1]])
+    Ls = LU[rows].astype(np.float64)
+    L = np.where(np.arange(n)[None, :] < rows[:, None], Ls, 0.0)  # row r of L: columns < r
+    L[np.arange(len(rows)), rows] = 1.0
+    U = np.triu(LU.astype(np.float64))
+    PA = A[perm[rows]].astype(np.float64)
+    assert np.linalg.norm(PA - L @ U) / np.linalg.norm(PA) <= 5e-6 * np.sqrt(n)
+
+
+def test_pivots_out_of_range_are_rejected(ddla):
+    # a relative pivot that would swap outside the matrix: the reference's SwapRow throws, the
+    # library returns DDLA_ERR_INVALID_ARG (host and device forms) and leaves B as it was
+    lib = ddla.native.lib()
+    n = 50
+    A = np.eye(n)
+    piv = np.zeros(n, dtype=np.int32)
+    piv[45] = 5  # 45 + 5 = 50 is outside
+    B = np.ones((n, 2))
+    with pytest.raises(ddla.DDLAError) as e:
+        ddla.PLUSolve(A, piv, B)
+    assert e.value.status == ddla.native.ERR_INVALID_ARG
+    dA, dB = ddla.DeviceMatrix.from_host(A), ddla.DeviceMatrix.from_host(B)
+    dpiv = ddla._DevInts.from_host(piv)
+    st = lib.ddla_dgetrs_dev(0, n, 2, dA.ptr, dA.ld, dpiv.ptr, dB.ptr, dB.ld)
+    assert st == ddla.native.ERR_INVALID_ARG
+    np.testing.assert_array_equal(dB.to_host(), B)
+    st = lib.ddla_dlaswp_dev(ddla.LEFT, 1, n, 2, dB.ptr, dB.ld, dpiv.ptr, n)
+    assert st == ddla.native.ERR_INVALID_ARG
+    # in range (the last possible swap, and a negative entry, which the reference skips)
+    piv[45] = 4
+    piv[10] = -3
+    dpiv = ddla._DevInts.from_host(piv)
+    assert lib.ddla_dlaswp_dev(ddla.LEFT, 1, n, 2, dB.ptr, dB.ld, dpiv.ptr, n) == 0

@@ -392,3 +392,71 @@ def test_tslu_zero_column_reports_info(oracle):
     P[:, 40] = 0.0
     zero, sel, top, pairs = oracle.tslu(P)
     assert zero == 41
+
+
+# ---------------------------------------------------------------- level-3 BLAS semantics ---
+# The oracle's GEMMT / SYR2K / SYMM / TRMM / TRMM3 (BLIS semantics over the restated managed
+# GEMM) against scipy's BLAS, an independent implementation of the same definitions.
+import scipy.linalg.blas as sblas  # noqa: E402
+
+LO, UP = 0xC0, 0x60
+
+
+@pytest.mark.parametrize("uplo", [LO, UP])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (8, 0), (0, 8), (8, 8)])
+def test_oracle_gemmt_triangle(uplo, ta, tb):
+    m, k = 37, 150
+    A = O.uniform_matrix(*(k, m) if ta else (m, k), 1)
+    B = O.uniform_matrix(*(m, k) if tb else (k, m), 2)
+    C0 = O.uniform_matrix(m, m, 3)
+    C = O.gemmt(uplo, ta, tb, 0.7, A, B, 1.3, C0.copy())
+    full = 0.7 * ((A.T if ta else A) @ (B.T if tb else B)) + 1.3 * C0
+    mask = np.tril(np.ones((m, m), bool)) if uplo == LO else np.triu(np.ones((m, m), bool))
+    np.testing.assert_allclose(C[mask], full[mask], rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(C[~mask], C0[~mask])
+
+
+@pytest.mark.parametrize("uplo", [LO, UP])
+@pytest.mark.parametrize("trans", [0, 8])
+def test_oracle_syr2k_vs_blas(uplo, trans):
+    m, k = 41, 90
+    A = O.uniform_matrix(*(k, m) if trans else (m, k), 4)
+    B = O.uniform_matrix(*(k, m) if trans else (m, k), 5)
+    C0 = O.uniform_matrix(m, m, 6)
+    C = O.syr2k(uplo, trans, trans, 1.1, A, B, 0.4, C0.copy())
+    ref = sblas.dsyr2k(1.1, A, B, beta=0.4, c=C0.copy(), trans=1 if trans else 0,
+                       lower=1 if uplo == LO else 0)
+    mask = np.tril(np.ones((m, m), bool)) if uplo == LO else np.triu(np.ones((m, m), bool))
+    np.testing.assert_allclose(C[mask], ref[mask], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("uplo", [LO, UP])
+def test_oracle_symm_vs_blas(side, uplo):
+    m, n = 45, 30
+    k = m if side == 0 else n
+    A = O.uniform_matrix(k, k, 7)  # only the uplo triangle may be read
+    B = O.uniform_matrix(m, n, 8)
+    C0 = O.uniform_matrix(m, n, 9)
+    C = O.symm(side, uplo, 0, 1 / np.pi, A, B, np.e, C0.copy())
+    ref = sblas.dsymm(1 / np.pi, A, B, beta=np.e, c=C0.copy(), side=side,
+                      lower=1 if uplo == LO else 0)
+    np.testing.assert_allclose(C, ref, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("uplo", [LO, UP])
+@pytest.mark.parametrize("trans", [0, 8])
+@pytest.mark.parametrize("diag", [0x000, 0x100])
+def test_oracle_trmm_vs_blas(side, uplo, trans, diag):
+    m, n = 33, 52
+    k = m if side == 0 else n
+    A = O.uniform_matrix(k, k, 10)
+    B0 = O.uniform_matrix(m, n, 11)
+    B = O.trmm(side, uplo, trans, diag, 1.2, A, B0.copy())
+    ref = sblas.dtrmm(1.2, A, B0.copy(), side=side, lower=1 if uplo == LO else 0,
+                      trans_a=1 if trans else 0, diag=1 if diag == 0x100 else 0)
+    np.testing.assert_allclose(B, ref, rtol=0, atol=1e-12)
+    # TRMM3 with beta = 0 and op(B) = B is the same product
+    C = O.trmm3(side, uplo, trans, diag, 0, 1.2, A, B0, 0.0, np.full((m, n), np.nan))
+    np.testing.assert_allclose(C, ref, rtol=0, atol=1e-12)
