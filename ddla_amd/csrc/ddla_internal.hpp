@@ -62,6 +62,9 @@ struct Context {
 
 Context& ctx();
 int ensure_init();
+// Locates (hipGetSymbolAddress) and clears the static scratch of the no-allocation entry
+// points on `device`, which must be current; called once per device by the library's set-up.
+int noalloc_setup(int device, hipStream_t s);
 hipStream_t stream();
 // the auxiliary stream and its events (created on first use)
 int aux_stream(hipStream_t* s, hipEvent_t** ev);
@@ -205,12 +208,24 @@ int launch_dscatter2d(int64_t m, int64_t n, const double* src, int64_t ld, doubl
 int launch_dscale2d(int64_t m, int64_t n, double beta, double* c, int64_t rs, int64_t cs,
                     int tri, hipStream_t s);
 
+// p[0 .. count) = v (a kernel, not hipMemsetAsync, so that a captured graph replays it)
+int launch_fill_u32(void* p, int64_t count, unsigned v, hipStream_t s);
+
 // LU pieces (lu_kernels.hip).  The panel kernel writes relative pivots to piv[0..nb),
 // sets *info = info_base + i + 1 at the first zero pivot (if *info was 0) and emits the
 // list of moved rows as (dst, src) pairs relative to the panel's first row.
+// ext: a caller-provided exchange area of panel_area_bytes() (all zero, or holding only words
+// with tags below first_tag) and the first tag of the launch's range first_tag .. first_tag +
+// nb + 1; null: the library's workspace and tag sequence.
+struct PanelScratch {
+  void* area;
+  unsigned first_tag;
+};
+size_t panel_area_bytes();
 template <typename T>
 int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
-                          int info_base, int* list_out, int* count_out, hipStream_t s);
+                          int info_base, int* list_out, int* count_out, hipStream_t s,
+                          const PanelScratch* ext = nullptr);
 // applies such a list (at most 2 * 128 entries) to columns [0, ncols) minus
 // [skip_lo, skip_hi) of the rows from row0 on
 template <typename T>
@@ -273,7 +288,18 @@ int launch_gather_rows(int64_t n, int64_t ncols, const T* src, int64_t lds,
                        const int32_t* perm, T* dst, int64_t ldd, bool inverse, hipStream_t s);
 
 // internal FP64 drivers
-int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* info_host);
+// Device memory a factorization may take from its caller instead of the library's workspaces
+// (the no-allocation entry point): the panel exchange area, the info word, per-panel counts
+// (npanels ints) and row lists (npanels * 4 * 128 ints), and the tag counter of the area.
+struct LuScratch {
+  void* panel_area;
+  int32_t* dinfo;
+  int* counts;
+  int* lists;
+  unsigned* tag_counter;  // host word: last tag used in panel_area
+};
+int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* info_host,
+                    const LuScratch* sc = nullptr);
 int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_t lda,
                     const int32_t* dpiv, double* b, int64_t ldb);
 int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host);

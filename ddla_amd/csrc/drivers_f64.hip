@@ -68,15 +68,22 @@ int* lu_perm_area(int64_t n) {
 // before the main stream touches them:
 //   main: panel(k); wait E2(k-1); swaps + U12 + update of the next panel's columns; panel(k+1)
 //   aux:  wait E0(k); swaps on [0, j) and the far columns, their U12 and update; record E2(k)
-int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* info_host) {
+int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* info_host,
+                    const LuScratch* sc) {
   hipStream_t s = stream();
   hipStream_t s2;
   hipEvent_t* ev;
   DDLA_RETURN_IF(aux_stream(&s2, &ev));
   int32_t* dinfo;
   int *counts, *lists;
-  DDLA_RETURN_IF(lu_bookkeeping(n, &dinfo, &counts, &lists));
-  DDLA_HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int32_t), s));
+  if (sc) {
+    dinfo = sc->dinfo;
+    counts = sc->counts;
+    lists = sc->lists;
+  } else {
+    DDLA_RETURN_IF(lu_bookkeeping(n, &dinfo, &counts, &lists));
+  }
+  DDLA_RETURN_IF(launch_fill_u32(dinfo, 1, 0u, s));
   // the auxiliary stream starts after everything queued so far on the main stream
   DDLA_HIP_CHECK(hipEventRecord(ev[0], s));
   DDLA_HIP_CHECK(hipStreamWaitEvent(s2, ev[0], 0));
@@ -87,8 +94,13 @@ int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* i
     const int64_t m = n - j;
     double* panel = a + j * lda + j;
     int* list_k = lists + (size_t)k * 4 * NBLK;
+    PanelScratch ps{};
+    if (sc) {  // a fresh tag range of the caller's area for each panel
+      ps = PanelScratch{sc->panel_area, *sc->tag_counter + 1u};
+      *sc->tag_counter += static_cast<unsigned>(b) + 2u;
+    }
     DDLA_RETURN_IF(launch_getf2_panel_ex<double>(m, b, panel, lda, dpiv + j, dinfo, (int)j, list_k,
-                                                 counts + k, s));
+                                                 counts + k, s, sc ? &ps : nullptr));
     const int64_t rest = n - j - b;
     const int64_t nxt = std::min<int64_t>(NBLK, rest);  // the next panel's width
     const int64_t far0 = j + b + nxt, far = n - far0;   // columns after the next panel

@@ -649,18 +649,30 @@ bool fits(int64_t m, int limit) {
 
 }  // namespace
 
+namespace {
+constexpr size_t PANEL_KEY_BYTES = 2 * (size_t)WMAX * KSTRIDE * sizeof(u64);
+constexpr size_t PANEL_CAND_BYTES = 2 * (size_t)WMAX * NB_MAX * 2 * sizeof(u64);
+constexpr size_t PANEL_LIST_BYTES = 4 * (size_t)NB_MAX * sizeof(int) + 256;
+}  // namespace
+
+size_t panel_area_bytes() { return PANEL_KEY_BYTES + PANEL_CAND_BYTES + PANEL_LIST_BYTES; }
+
 // exchange area for panel kernels; *base: the launch's tag range is base + 1 .. base + nb + 2
-// (a fresh range per launch, so words of earlier launches never match)
-int panel_sync(PanelSync* sy, int nb, unsigned* base, hipStream_t s) {
-  const size_t kb = 2 * (size_t)WMAX * KSTRIDE * sizeof(u64);
-  const size_t cb = 2 * (size_t)WMAX * NB_MAX * 2 * sizeof(u64);
-  const size_t lb = 4 * (size_t)NB_MAX * sizeof(int) + 256;
-  const size_t bytes = kb + cb + lb;
+// (a fresh range per launch, so words of earlier launches never match).  ext: an area the
+// caller provides (cleared once, its tag range chosen by the caller) instead of the workspace.
+int panel_sync(PanelSync* sy, int nb, unsigned* base, hipStream_t s, const PanelScratch* ext) {
+  const size_t kb = PANEL_KEY_BYTES, cb = PANEL_CAND_BYTES;
+  const size_t bytes = panel_area_bytes();
   void* ws = nullptr;
-  unsigned first = 0;
-  DDLA_RETURN_IF(next_tag(WS_PANEL, bytes, static_cast<unsigned>(nb) + 2u, s, &first));
-  DDLA_RETURN_IF(workspace(WS_PANEL, bytes, &ws));
-  *base = first - 1u;
+  if (ext) {
+    ws = ext->area;
+    *base = ext->first_tag - 1u;
+  } else {
+    unsigned first = 0;
+    DDLA_RETURN_IF(next_tag(WS_PANEL, bytes, static_cast<unsigned>(nb) + 2u, s, &first));
+    DDLA_RETURN_IF(workspace(WS_PANEL, bytes, &ws));
+    *base = first - 1u;
+  }
   char* p = static_cast<char*>(ws);
   sy->keys = reinterpret_cast<u64*>(p);
   sy->cand = reinterpret_cast<u64*>(p + kb);
@@ -671,14 +683,15 @@ int panel_sync(PanelSync* sy, int nb, unsigned* base, hipStream_t s) {
 
 template <typename T>
 int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
-                          int info_base, int* list_out, int* count_out, hipStream_t s) {
+                          int info_base, int* list_out, int* count_out, hipStream_t s,
+                          const PanelScratch* ext) {
   if (nb > NB_MAX || nb <= 0 || m < nb) {
     set_error("panel %lld x %d outside 1 <= nb <= min(m, %d)", (long long)m, nb, NB_MAX);
     return DDLA_ERR_INVALID_ARG;
   }
   PanelSync sy;
   unsigned base = 0;
-  DDLA_RETURN_IF(panel_sync(&sy, nb, &base, s));
+  DDLA_RETURN_IF(panel_sync(&sy, nb, &base, s, ext));
   if (list_out) {
     sy.list = list_out;
     sy.list_count = count_out;
@@ -751,6 +764,20 @@ int launch_piv_to_perm(int64_t n, const int32_t* piv, int32_t* perm, hipStream_t
   return 0;
 }
 
+__global__ void fill_u32_kernel(unsigned* __restrict__ p, int64_t count, unsigned v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+int launch_fill_u32(void* p, int64_t count, unsigned v, hipStream_t s) {
+  if (count <= 0) return 0;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(count)), dim3(256), 0, s,
+                     static_cast<unsigned*>(p), count, v);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 int check_pivots(int64_t n, const int32_t* piv, hipStream_t s) {
   if (n <= 0) return 0;
   void* ws = nullptr;
@@ -780,9 +807,9 @@ int launch_gather_rows(int64_t n, int64_t ncols, const T* src, int64_t lds, cons
 }
 
 template int launch_getf2_panel_ex<double>(int64_t, int, double*, int64_t, int32_t*, int32_t*,
-                                           int, int*, int*, hipStream_t);
+                                           int, int*, int*, hipStream_t, const PanelScratch*);
 template int launch_getf2_panel_ex<float>(int64_t, int, float*, int64_t, int32_t*, int32_t*, int,
-                                          int*, int*, hipStream_t);
+                                          int*, int*, hipStream_t, const PanelScratch*);
 template int launch_laswp_list<double>(double*, int64_t, int64_t, int64_t, int64_t, int64_t,
                                        const int*, const int*, int, hipStream_t);
 template int launch_laswp_list<float>(float*, int64_t, int64_t, int64_t, int64_t, int64_t,

@@ -46,7 +46,12 @@ int init_device(int device) {
   g_ctx.num_cus = prop.multiProcessorCount;
   DDLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.own_stream, hipStreamNonBlocking));
   g_ctx.stream = g_ctx.own_stream;
+  // the auxiliary stream and its events exist from set-up on, so no entry point creates them
+  DDLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.aux_stream, hipStreamNonBlocking));
+  for (hipEvent_t& e : g_ctx.ev) DDLA_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   g_ctx.ready = true;
+  // the no-allocation entry points' static scratch on this device
+  DDLA_RETURN_IF(noalloc_setup(device, g_ctx.own_stream));
   return 0;
 }
 

@@ -65,6 +65,7 @@ SIGNATURES = {
     "ddla_dgemm": [cint, cint, i64, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp, i64, i64],
     "ddla_dgemm_dev": [cint, cint, i64, i64, i64, dp, vp, i64, i64, vp, i64, i64, dp, vp, i64, i64],
     "ddla_dgetrf_dev": [i64, vp, i64, vp],
+    "ddla_dgetrf_noalloc_dev": [i64, vp, i64, vp],
     "ddla_dgetrs_dev": [cint, i64, i64, vp, i64, vp, vp, i64],
     "ddla_dgetrf": [i64, vp, i64, i64, vp],
     "ddla_dgetrs": [cint, i64, i64, vp, i64, i64, vp, vp, i64, i64],

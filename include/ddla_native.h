@@ -133,6 +133,14 @@ int ddla_dgemm_dev(int transa, int transb, ddla_dim_t m, ddla_dim_t n, ddla_dim_
  * PLUDecUnblock :344-374).  dpiv: device int32[n], relative pivots.  Returns info
  * (first zero pivot, 1-based; the reference throws from InvScal, DDLA/BLAS/1v.cs:99). */
 int ddla_dgetrf_dev(ddla_dim_t n, double* a, ddla_inc_t lda, int32_t* dpiv);
+/* ddla_dgetrf_dev without any device allocation.  No counterpart in the reference: for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Same kernels, pivots and factors as ddla_dgetrf_dev;
+ * the scratch (panel exchange area, info word, row lists: about 1.6 MiB) lives in a static
+ * device array that each GPU holds until the process ends, set up by ddla_init.  Orders above
+ * 32768 return DDLA_ERR_INVALID_ARG.  All calls on one GPU share that array: calls that
+ * overlap on one GPU race. */
+int ddla_dgetrf_noalloc_dev(ddla_dim_t n, double* a, ddla_inc_t lda, int32_t* dpiv);
 /* Solves A X = B (trans == DDLA_NO_TRANSPOSE) or A^T X = B with the factors of
  * ddla_dgetrf_dev; B (n x nrhs row-major, device) is overwritten with X.
  * Replaces LU.PLUSolve (LU.cs:266-287, PLUSolveInternel :431-465).  The pivots are checked on
