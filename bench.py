@@ -371,6 +371,9 @@ def main():
                     help="problems in the batched-factorization measurement (0 skips it)")
     ap.add_argument("--qr-device", choices=["cuda", "cpu"], default="cuda",
                     help="where the test matrix's QR factors are computed")
+    ap.add_argument("--gemm-only", type=int, default=0, metavar="N",
+                    help="only time the FP64 GEMM C = A B at N x N x N (for profiler runs) and "
+                         "print its figures")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other mixed configurations (BF16, 64 right-hand sides, "
                          "Cholesky), one timed solve each")
@@ -381,10 +384,14 @@ def main():
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import ddla_amd as ddla
 
     ddla.init(0)
+    if args.gemm_only:
+        print(json.dumps({"gemm_fp64": time_gemm(ddla, args.gemm_only, max(args.steps, 1))}))
+        return
+    import torch
+
     n, nrhs = args.n, args.nrhs
     prec = {"fp16": ddla.PREC_FP16, "bf16": ddla.PREC_BF16, "fp32": ddla.PREC_FP32}[args.prec]
     log(f"n={n} kappa={args.kappa:g} prec={args.prec} nrhs={nrhs}")

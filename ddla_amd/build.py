@@ -1,6 +1,6 @@
 """Builds ddla_amd/libddla_hip.so from ddla_amd/csrc/*.hip with hipcc for gfx950.
 
-Usage: python -m ddla_amd.build [--jobs N]
+Usage: python -m ddla_amd.build [--jobs N] [--incremental]
 Objects go to ddla_amd/_build/; the shared library is written in-tree next to this file.
 """
 from __future__ import annotations
@@ -49,8 +49,19 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src: str) -> str:
+def _stale(src: str, obj: str) -> bool:
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src, os.path.join(os.path.dirname(HERE), "include", "ddla_native.h")]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [HIPCC, *CFLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -58,12 +69,14 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> str:
+def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> str:
+    """Compiles every source (incremental=True: only sources newer than their objects or
+    than the headers) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4) // 2))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda f: _compile(f, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -78,5 +91,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources changed since their objects were built")
     a = ap.parse_args()
-    build(a.jobs)
+    build(a.jobs, incremental=a.incremental)
