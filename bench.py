@@ -56,6 +56,30 @@ def make_system(n, kappa, nrhs, seed, torch, qr_device="cuda"):
     return A.contiguous(), b.contiguous(), xt
 
 
+def make_system_host_qr(n, kappa, nrhs, seed, ddla):
+    """The same construction without torch: the two QRs by numpy (LAPACK) on the host, the
+    products by the library's GEMM on the device.  Returns device matrices (A, b).  For
+    profiler runs: a process where torch's own HIP runtime also runs ends, under rocprofv3, with
+    a crash after the tool's finalization; processes that use only this library end cleanly."""
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    U, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    log(f"QR 1 done {time.time() - t0:.1f}s")
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    log(f"QR 2 done {time.time() - t0:.1f}s")
+    U *= kappa ** (-np.arange(n) / (n - 1))
+    dU, dV = ddla.DeviceMatrix.from_host(U), ddla.DeviceMatrix.from_host(V)
+    del U, V
+    A = ddla.DeviceMatrix(n, n)
+    ddla.BlasProvider.GeMM(1.0, dU, dV, 0.0, A, ddla.NO_TRANSPOSE, ddla.TRANSPOSE)
+    xt = ddla.DeviceMatrix.from_host(rng.uniform(-1.0, 1.0, (n, nrhs)))
+    b = ddla.DeviceMatrix(n, nrhs)
+    ddla.BlasProvider.GeMM(1.0, A, xt, 0.0, b)
+    ddla.synchronize()
+    log(f"system ready {time.time() - t0:.1f}s")
+    return A, b
+
+
 def backward_error(torch, A, x, b):
     r = b - A @ x
     rn = r.abs().amax(dim=0)
@@ -231,11 +255,18 @@ def bench_batched(torch, ddla, count, steps, seed, check_sample, cpu_sample, thr
     return out
 
 
-def cpu_baseline(n_target, n_sample, nrhs):
-    """The oracle (CPU restatement of DDLA's managed FP64 LU + solve) on a bounded sample."""
-    from oracle import oracle as O
+def cpu_threads():
     nproc = os.cpu_count() or 2
-    threads = max(1, min(nproc // 2, 16))  # reference policy ProcessorCount/2, box share 16
+    return nproc, max(1, min(nproc // 2, 16))  # reference policy ProcessorCount/2, box share 16
+
+
+def cpu_baseline(n_target, n_sample, nrhs):
+    """The oracle (CPU restatement of DDLA's managed FP64 LU + solve) on a bounded sample: the
+    plain FP64 LU + solve configuration itself (n = 8192, A uniform[-1,1] from seed 0xDD1A0002),
+    measured, and scaled by (n_target / n_sample)^3 for the headline's unit.  Returns the
+    baseline object and the sample's relative pivots (compared with the GPU's in the report)."""
+    from oracle import oracle as O
+    nproc, threads = cpu_threads()
     O.set_threads(threads)
     A = O.uniform_matrix(n_sample, n_sample, 0xDD1A0002)
     B = O.uniform_matrix(n_sample, nrhs, 0xDD1A0003)
@@ -244,10 +275,74 @@ def cpu_baseline(n_target, n_sample, nrhs):
     O.getrs(A, piv, B)
     dt = time.perf_counter() - t0
     scale = (n_target / n_sample) ** 3
-    return {"value": 1.0 / (dt * scale), "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle FP64 blocked LU (block 128) + solve at n={n_sample} took "
-                       f"{dt:.2f}s on {threads} threads; scaled by (n/{n_sample})^3={scale:.0f} "
-                       f"to n={n_target}")}
+    return ({"value": 1.0 / (dt * scale), "unit": "solves/s", "cores": threads,
+             "host_nproc": nproc, "kind": "port",
+             "sample": (f"oracle FP64 blocked LU (block 128) + solve at n={n_sample} took "
+                        f"{dt:.2f}s on {threads} threads (host nproc {nproc}); scaled by "
+                        f"(n/{n_sample})^3={scale:.0f} to n={n_target}"),
+             "measured_n": n_sample, "measured_seconds": dt}, piv)
+
+
+def cpu_gemm(n, reps=3):
+    """The oracle's restated managed GEMM (GeMM.cs loop nest, 6x8 AVX2 micro-kernel) at n^3."""
+    from oracle import oracle as O
+    nproc, threads = cpu_threads()
+    O.set_threads(threads)
+    A = O.uniform_matrix(n, n, 0xDD1A0001)
+    B = O.uniform_matrix(n, n, 0xDD1A0001 + 1)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.matmul(A, B)
+        ts.append(time.perf_counter() - t0)
+    dt = float(np.median(ts))
+    return {"n": n, "ms": dt * 1e3, "gflops": 2.0 * n ** 3 / dt / 1e9, "threads": threads,
+            "host_nproc": nproc, "kind": "port"}
+
+
+def bench_c2(ddla, n=8192, reps=2):
+    """The plain FP64 LU + solve configuration on the GPU: A uniform[-1,1] (seed 0xDD1A0002,
+    the oracle's stream), b = A x_true with x_true uniform[-1,1]; solves/s, backward error and
+    the panel / interchange kernels' HBM rates."""
+    A0 = ddla.DeviceMatrix.random(n, n, 0xDD1A0002)
+    xt = ddla.DeviceMatrix.random(n, 1, 0xDD1A0003)
+    b = ddla.DeviceMatrix(n, 1)
+    ddla.BlasProvider.GeMM(1.0, A0, xt, 0.0, b)
+    W = ddla.DeviceMatrix(n, n)
+    X = ddla.DeviceMatrix(n, 1)
+    times = []
+    prof = {}
+    for k in range(reps + 1):
+        ddla.native.check(ddla.native.lib().ddla_memcpy_d2d(W.ptr, A0.ptr, n * n * 8))
+        ddla.native.check(ddla.native.lib().ddla_memcpy_d2d(X.ptr, b.ptr, n * 8))
+        ddla.synchronize()
+        if k == reps:
+            ddla.profile_reset()
+            ddla.profile_enable(True)
+        t0 = time.perf_counter()
+        piv = ddla.PLUDecompose(W)
+        ddla.PLUSolve(W, piv, X)
+        ddla.synchronize()
+        if k >= 1:
+            times.append(time.perf_counter() - t0)
+        if k == reps:
+            ddla.profile_enable(False)
+            prof = ddla.profile_read()
+    dt = float(np.median(times[:-1] if len(times) > 1 else times))
+    Ah, bh, xh = A0.to_host(), b.to_host(), X.to_host()
+    be = float(np.abs(bh - Ah @ xh).max() / (np.abs(Ah).sum(axis=1).max() * np.abs(xh).max()))
+    kern = {}
+    for name in ("getf2_panel_f64", "laswp_list", "laswp_trsm_slab_f64", "dgemm_mfma_f64"):
+        e = prof.get(name)
+        if e and e["ms"] > 0:
+            kern[name] = {"launches": e["count"], "ms": e["ms"],
+                          "gbytes_per_s": e["bytes"] / (e["ms"] / 1e3) / 1e9,
+                          "frac_hbm": e["bytes"] / (e["ms"] / 1e3) / PEAK["hbm"]}
+            if e["flops"] > 0 and name.startswith("dgemm"):
+                kern[name]["tflops"] = e["flops"] / (e["ms"] / 1e3) / 1e12
+    return {"n": n, "ms_per_solve": dt * 1e3, "solves_per_s": 1.0 / dt,
+            "gflops": (2.0 / 3.0 * n ** 3 + 2.0 * n * n) / dt / 1e9, "backward_error": be,
+            "kernels": kern}, piv
 
 
 def timed(fn):
@@ -349,6 +444,40 @@ def bench_extras(torch, ddla, A, A_t, n, args, t_fp64):
     return out
 
 
+def solve_only(ddla, args, A, B, X, n, nrhs, prec):
+    """--solve-only: the mixed solver or the plain FP64 LU + solve, --warmup + --steps times."""
+    piv_codes = {"partial": ddla.PIVOT_PARTIAL, "tournament": ddla.PIVOT_TOURNAMENT,
+                 "butterfly": ddla.PIVOT_BUTTERFLY}
+    if args.solve_only == "fp64":
+        W = ddla.DeviceMatrix(n, n)
+        Xf = ddla.DeviceMatrix(n, nrhs)
+
+        def run():
+            ddla.native.check(ddla.native.lib().ddla_memcpy_d2d(W.ptr, A.ptr, n * n * 8))
+            ddla.native.check(ddla.native.lib().ddla_memcpy_d2d(Xf.ptr, B.ptr, n * nrhs * 8))
+            piv = ddla.PLUDecompose(W)
+            ddla.PLUSolve(W, piv, Xf)
+            return None
+    else:
+        def run():
+            return ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14,
+                                  pivoting=piv_codes[args.pivoting])
+    for _ in range(args.warmup):
+        run()
+    ddla.synchronize()
+    t0 = time.perf_counter()
+    info = None
+    for _ in range(args.steps):
+        info = run()
+    ddla.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    out = {"solve_only": args.solve_only, "n": n, "ms_per_solve": dt * 1e3}
+    if info is not None:
+        out.update({"status": info.status, "iters": info.iters,
+                    "backward_error": info.backward_error})
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,6 +503,9 @@ def main():
     ap.add_argument("--gemm-only", type=int, default=0, metavar="N",
                     help="only time the FP64 GEMM C = A B at N x N x N (for profiler runs) and "
                          "print its figures")
+    ap.add_argument("--solve-only", choices=["mixed", "fp64"], default=None,
+                    help="only run the mixed solver or the plain FP64 LU + solve on the system "
+                         "(--warmup + --steps times; for profiler runs) and print the timing")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other mixed configurations (BF16, 64 right-hand sides, "
                          "Cholesky), one timed solve each")
@@ -384,13 +516,24 @@ def main():
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
+    # The profiler modes (--gemm-only; --solve-only with --qr-device cpu) run without torch.
+    # Otherwise torch is imported before the library, the order in which both HIP runtimes
+    # (torch's own and the system's) also run under rocprofv3.
+    torch_free = bool(args.gemm_only) or (args.solve_only is not None and args.qr_device == "cpu")
+    if not torch_free:
+        import torch
     import ddla_amd as ddla
 
     ddla.init(0)
     if args.gemm_only:
         print(json.dumps({"gemm_fp64": time_gemm(ddla, args.gemm_only, max(args.steps, 1))}))
         return
-    import torch
+    if torch_free:
+        n, nrhs = args.n, args.nrhs
+        prec = {"fp16": ddla.PREC_FP16, "bf16": ddla.PREC_BF16, "fp32": ddla.PREC_FP32}[args.prec]
+        A, B = make_system_host_qr(n, args.kappa, nrhs, args.seed, ddla)
+        X = ddla.DeviceMatrix(n, nrhs)
+        return solve_only(ddla, args, A, B, X, n, nrhs, prec)
 
     n, nrhs = args.n, args.nrhs
     prec = {"fp16": ddla.PREC_FP16, "bf16": ddla.PREC_BF16, "fp32": ddla.PREC_FP32}[args.prec]
@@ -408,6 +551,9 @@ def main():
 
     def solve(piv):
         return ddla.gesv_mpir(A, B, X, prec=prec, tol=1e-14, pivoting=piv)
+
+    if args.solve_only:
+        return solve_only(ddla, args, A, B, X, n, nrhs, prec)
 
     # ---- optionally, other pivoting rules of the mixed solver (one timed solve each)
     other_out = []
@@ -524,11 +670,24 @@ def main():
                       "speedup_vs_fp64_lu": t_fp64 / ts})
         log(f"kappa={kap:g}: {ts * 1e3:.1f} ms iters={si.iters} berr={si.backward_error:.2e}")
 
+    # ---- the plain FP64 LU + solve configuration (n = 8192) on the GPU
+    c2, piv_c2 = bench_c2(ddla, 8192)
+    log("fp64 LU n=8192", c2)
+
     cpu = None
+    cpu_c1 = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(n, min(args.cpu_sample, n), nrhs)
+        cpu, piv_cpu = cpu_baseline(n, min(args.cpu_sample, n), nrhs)
         log("cpu", cpu)
+        if cpu["measured_n"] == c2["n"]:
+            c2["cpu"] = {"seconds": cpu["measured_seconds"],
+                         "solves_per_s": 1.0 / cpu["measured_seconds"],
+                         "threads": cpu["cores"], "host_nproc": cpu["host_nproc"],
+                         "kind": "port"}
+            c2["pivots_identical_to_cpu_oracle"] = bool(np.array_equal(piv_c2, piv_cpu))
+        cpu_c1 = cpu_gemm(1024)
+        log("cpu gemm", cpu_c1)
 
     out = {
         "metric": METRIC,
@@ -561,6 +720,8 @@ def main():
                     "gflops": (2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs) / t_fp64 / 1e9},
         "speedup_vs_fp64_lu": t_fp64 / t_mixed,
         "gemm_fp64": gemm,
+        "gemm_fp64_cpu_oracle": cpu_c1,
+        "fp64_lu_n8192": c2,
         "kernel_breakdown": breakdown,
         "kernel_breakdown_fp64_lu": breakdown64,
         "batched": batched,

@@ -254,6 +254,14 @@ int ddla_sync(void) {
 
 int ddla_version(void) { return 100; }
 
+int ddla_set_wait_limit(uint64_t rounds) {
+  API_INIT();
+  ctx().wait_limit = rounds == 0 ? DEFAULT_WAIT_LIMIT
+                                 : static_cast<unsigned>(std::min<uint64_t>(rounds, 0xFFFFFFFFu));
+  ctx().wait_limit_set = true;
+  return 0;
+}
+
 int ddla_profile_enable(int on) {
   API_INIT();
   profile_enable(on != 0);

@@ -40,6 +40,8 @@ const char* last_error();
   } while (0)
 
 // ---------------------------------------------------------------- context ----------------
+constexpr unsigned DEFAULT_WAIT_LIMIT = 1u << 24;
+
 struct Context {
   bool ready = false;
   int device = 0;
@@ -58,6 +60,11 @@ struct Context {
   // slot holds no live tag)
   size_t ws_cleared[16] = {};
   unsigned ws_tag[16] = {};
+  // poll rounds a cross-workgroup wait (LU panel, triangular solve) makes before it gives up
+  unsigned wait_limit = DEFAULT_WAIT_LIMIT;
+  // set once a caller changed wait_limit: from then on every panel launch first writes the
+  // limit into its exchange area, where the kernel reads it (a cleared word means the default)
+  bool wait_limit_set = false;
 };
 
 Context& ctx();
@@ -76,6 +83,12 @@ int workspace(int slot, size_t bytes, void** out);
 int workspace_cleared(int slot, size_t bytes, void** out, hipStream_t s, bool* fresh = nullptr);
 // Next tag of slot's sequence (never 0); a wrapped sequence clears the slot again first.
 int next_tag(int slot, size_t bytes, unsigned step, hipStream_t s, unsigned* tag);
+// Kernels whose workgroups wait on each other's words (the LU panel, the persistent triangular
+// solve) bound every wait by ctx().wait_limit poll rounds: a workgroup that reaches the bound
+// stores its launch's tag in the launch's abort word, and every workgroup of the launch leaves
+// at its next poll.  check_abort synchronises s, reads such a word and, when it was set, clears
+// it and returns DDLA_ERR_HIP naming `what`.
+int check_abort(unsigned* word, hipStream_t s, const char* what);
 enum WorkspaceSlot {
   WS_PANEL = 0,   // panel-kernel exchange slots
   WS_PIVOT = 1,   // pivots / permutations / info words
@@ -222,6 +235,8 @@ struct PanelScratch {
   unsigned first_tag;
 };
 size_t panel_area_bytes();
+// the abort word of the panel launches in ext's area (null: the library's workspace)
+unsigned* panel_abort_word(const PanelScratch* ext);
 template <typename T>
 int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
                           int info_base, int* list_out, int* count_out, hipStream_t s,
@@ -262,6 +277,8 @@ int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64
 template <typename T>
 int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t lda, const T* Dinv,
                T* B, int64_t ldb, hipStream_t s, const int* skip = nullptr);
+// the abort word of the persistent triangular solves (null before the first solve)
+unsigned* trsv_abort_word();
 // the same solve for many right-hand sides (FP32): GEMMs on the prepared diagonal inverses
 int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float* A, int64_t lda,
                        const float* Dinv, float* B, int64_t ldb, hipStream_t s);

@@ -32,6 +32,12 @@ int tri_solve(bool lower, bool unit, bool trans, int64_t n, int64_t nrhs, const 
   return trsm_left<double>(lower, unit, trans, n, nrhs, a, lda, b, ldb, s);
 }
 
+// after the persistent triangular solves (<= 8 right-hand sides): did a wait give up?
+int solves_done(int64_t nrhs, hipStream_t s) {
+  if (nrhs > 8) return 0;
+  return check_abort(trsv_abort_word(), s, "triangular solve");
+}
+
 int read_info(int32_t* dinfo, int32_t* info_host) {
   DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
   DDLA_HIP_CHECK(hipStreamSynchronize(stream()));
@@ -138,7 +144,9 @@ int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* i
     }
   }
   if (pending) DDLA_HIP_CHECK(hipStreamWaitEvent(s, ev[1], 0));
-  return read_info(dinfo, info_host);
+  DDLA_RETURN_IF(read_info(dinfo, info_host));
+  PanelScratch ps{sc ? sc->panel_area : nullptr, 0};
+  return check_abort(panel_abort_word(sc ? &ps : nullptr), s, "LU panel");
 }
 
 int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_t lda,
@@ -164,7 +172,7 @@ int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_
     DDLA_RETURN_IF(launch_gather_rows<double>(n, nrhs, b, ldb, perm, tmp, nrhs, true, s));
     DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, tmp, nrhs, 1, b, ldb, s));
   }
-  return 0;
+  return solves_done(nrhs, s);
 }
 
 // LUDecBlock (LU.cs:169-186): per 128-block, LU without pivoting of the diagonal block,
@@ -200,7 +208,7 @@ int dgetrs_nopiv_internal(bool trans, int64_t n, int64_t nrhs, const double* a, 
     DDLA_RETURN_IF(tri_solve(false, false, true, n, nrhs, a, lda, b, ldb));
     DDLA_RETURN_IF(tri_solve(true, true, true, n, nrhs, a, lda, b, ldb));
   }
-  return 0;
+  return solves_done(nrhs, stream());
 }
 
 // Blocked lower Cholesky (CholeskyLowerBlock, Cholesky.cs:142-160) with a look-ahead of one
@@ -264,7 +272,7 @@ int dpotrs_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_
     DDLA_RETURN_IF(tri_solve(false, false, true, n, nrhs, a, lda, b, ldb));
     DDLA_RETURN_IF(tri_solve(false, false, false, n, nrhs, a, lda, b, ldb));
   }
-  return 0;
+  return solves_done(nrhs, stream());
 }
 
 }  // namespace ddla

@@ -52,6 +52,16 @@ constexpr int KSTRIDE = 16;      // one 128-byte line per key slot: readers spre
 
 typedef unsigned long long u64;
 
+// The exchange area: keys, candidate rows, the row list and its count, then the abort pair
+// (word 0: the first tag of a launch that gave up waiting; word 1: the poll-round limit,
+// 0 meaning DEFAULT_WAIT_LIMIT).  The kernel finds the abort pair at a fixed offset from the
+// keys, so it costs no kernel argument.
+constexpr size_t PANEL_KEY_BYTES = 2 * (size_t)WMAX * KSTRIDE * sizeof(u64);
+constexpr size_t PANEL_CAND_BYTES = 2 * (size_t)WMAX * NB_MAX * 2 * sizeof(u64);
+constexpr size_t PANEL_LIST_BYTES = 4 * (size_t)NB_MAX * sizeof(int) + 256;
+constexpr size_t PANEL_ABORT_OFF = PANEL_KEY_BYTES + PANEL_CAND_BYTES + PANEL_LIST_BYTES;
+constexpr size_t PANEL_ABORT_BYTES = 256;
+
 template <typename K>
 __device__ __forceinline__ void better(K& k, int& r, K k2, int r2) {
   if (k2 > k || (k2 == k && r2 < r)) {
@@ -81,9 +91,30 @@ __device__ __forceinline__ void put_elem(u64* p, T v, unsigned tag) {
 // is set, the row at `src2` into `dst2`.  Lane l takes elements l, l + 64, ...; every load of
 // a lane is issued before any tag is checked, so a row that is already visible costs one round
 // trip, and a round with a stale word re-reads the whole set.
+// The waits of a wave count their poll rounds together, over the whole launch; every 32nd
+// round reads the abort pair.  A wave that is past the limit stores `id` (the launch's first
+// tag, never 0, so a cleared word names no launch) in word 0 and its wait returns false; so
+// does a wait that finds `id` there (another workgroup of the launch gave up).  The other
+// rounds only count, so a poll loop keeps no more than the counter in registers.
+__device__ __forceinline__ bool wait_expired(const u64* keys, unsigned id, unsigned rounds) {
+  u64* pair = const_cast<u64*>(keys) + PANEL_ABORT_OFF / sizeof(u64);
+  const u64 w = get(pair);
+  if (static_cast<unsigned>(w) == id) return true;
+  const unsigned lim = static_cast<unsigned>(w >> 32);
+  if (rounds < (lim ? lim : DEFAULT_WAIT_LIMIT)) return false;
+  __hip_atomic_store(reinterpret_cast<unsigned*>(pair), id, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+__device__ __forceinline__ bool keep_waiting(const u64* keys, unsigned id, unsigned& rounds) {
+  ++rounds;
+  return (rounds & 31u) != 0 || !wait_expired(keys, id, rounds);
+}
+
 template <typename T, int CPL>
-__device__ __forceinline__ void fetch_rows(const u64* src, const u64* src2, int n, int lane,
-                                           unsigned tag, T* dst, T* dst2) {
+__device__ __forceinline__ bool fetch_rows(const u64* src, const u64* src2, int n, int lane,
+                                           unsigned tag, T* dst, T* dst2, const u64* area,
+                                           unsigned id, unsigned& rounds) {
   constexpr int EW = sizeof(T) == 8 ? 2 : 1;
   u64 a[CPL][EW], b[CPL][EW];
   const bool two = src2 != nullptr;
@@ -111,7 +142,8 @@ __device__ __forceinline__ void fetch_rows(const u64* src, const u64* src2, int 
         }
       }
     }
-    if (ok) break;
+    if (__all(ok)) break;  // wave-uniform loop: the round counter stays scalar
+    if (!keep_waiting(area, id, rounds)) return false;
     __builtin_amdgcn_s_sleep(1);
   }
 #pragma unroll
@@ -127,6 +159,7 @@ __device__ __forceinline__ void fetch_rows(const u64* src, const u64* src2, int 
       }
     }
   }
+  return true;
 }
 
 // Polls the W key slots (slot q at keys + q KSTRIDE: the key in one tagged word for FP32, two
@@ -136,8 +169,9 @@ __device__ __forceinline__ void fetch_rows(const u64* src, const u64* src2, int 
 // W - 1 and are not checked): reads under a per-slot branch would each be merged into fresh
 // registers and waited for one group at a time.
 template <typename K, int S>
-__device__ __forceinline__ void poll_keys(const u64* keys, int W, int lane, unsigned tag, K& k,
-                                          int& r) {
+__device__ __forceinline__ bool poll_keys(const u64* keys, int W, int lane, unsigned tag, K& k,
+                                          int& r, const u64* area, unsigned id,
+                                          unsigned& rounds) {
   constexpr int EW = sizeof(K) == 8 ? 2 : 1;  // words of the key
   u64 a[S], b[S], c[S];
   for (;;) {
@@ -154,7 +188,8 @@ __device__ __forceinline__ void poll_keys(const u64* keys, int W, int lane, unsi
       ok = ok && (lane + 64 * j >= W ||
                   (static_cast<unsigned>(a[j]) == tag && static_cast<unsigned>(c[j]) == tag &&
                    (EW == 1 || static_cast<unsigned>(b[j]) == tag)));
-    if (ok) break;
+    if (__all(ok)) break;  // wave-uniform loop: the round counter stays scalar
+    if (!keep_waiting(area, id, rounds)) return false;
     __builtin_amdgcn_s_sleep(1);
   }
   k = K(-2);
@@ -170,6 +205,7 @@ __device__ __forceinline__ void poll_keys(const u64* keys, int W, int lane, unsi
       better(k, r, kj, static_cast<int>(c[j] >> 32));
     }
   }
+  return true;
 }
 
 // Selects element k of a register array with a uniform run-time index (unrolled compares,
@@ -339,6 +375,7 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
   if (lane < RW) lblw[wave][lane] = gr0 + NW * lane;
   column_to_lds(v0, v1, 0, lane, lbl, -1, colb[wave]);
 
+  unsigned rounds = 0;  // wave 0's poll rounds in this launch (every wait counts)
   for (int i = 0; i < nb; ++i) {
     const int par = i & 1;
     const unsigned tag = base + static_cast<unsigned>(i) + 1u;
@@ -394,23 +431,30 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
       int gp;
       const u64* ks = sy.keys + (size_t)par * WMAX * KSTRIDE;
       const int nj = (W + 63) >> 6;
+      bool ok;
       if (nj <= 1)
-        poll_keys<T, 1>(ks, W, lane, tag, gk, gp);
+        ok = poll_keys<T, 1>(ks, W, lane, tag, gk, gp, sy.keys, base + 1u, rounds);
       else if (nj == 2)
-        poll_keys<T, 2>(ks, W, lane, tag, gk, gp);
+        ok = poll_keys<T, 2>(ks, W, lane, tag, gk, gp, sy.keys, base + 1u, rounds);
       else if (nj == 3)
-        poll_keys<T, 3>(ks, W, lane, tag, gk, gp);
+        ok = poll_keys<T, 3>(ks, W, lane, tag, gk, gp, sy.keys, base + 1u, rounds);
       else
-        poll_keys<T, 4>(ks, W, lane, tag, gk, gp);
-      wave_argmax_dpp(gk, gp);  // DPP across the lanes (every lane ends with the winner)
-      gp = __builtin_amdgcn_readfirstlane(gp);
-      // the winner's row, read by this wave right away, to LDS for the others
-      const u64* cw = sy.cand + ((size_t)par * WMAX + (gp & 255)) * NB_MAX * 2;
-      fetch_rows<T, NB_MAX / 64>(cw, nullptr, NB_MAX, lane, tag, prow, prow);
-      if (lane == 0) swin = gp;
+        ok = poll_keys<T, 4>(ks, W, lane, tag, gk, gp, sy.keys, base + 1u, rounds);
+      if (ok) {
+        wave_argmax_dpp(gk, gp);  // DPP across the lanes (every lane ends with the winner)
+        gp = __builtin_amdgcn_readfirstlane(gp);
+        // the winner's row, read by this wave right away, to LDS for the others
+        const u64* cw = sy.cand + ((size_t)par * WMAX + (gp & 255)) * NB_MAX * 2;
+        ok = fetch_rows<T, NB_MAX / 64>(cw, nullptr, NB_MAX, lane, tag, prow, prow, sy.keys,
+                                        base + 1u, rounds);
+      }
+      if (lane == 0) swin = ok ? gp : -1;
     }
     __syncthreads();  // prow and the winner in LDS
     const int gp = __builtin_amdgcn_readfirstlane(swin);
+    // a wait gave up: leave the column loop as on a zero pivot (the rows are written back in
+    // their current, consistent order; the host reports the failure)
+    if (gp < 0) break;
     const int p = gp >> 8;  // label of the pivot row
     const T pv = prow[i];
     if (w == 0 && tid == 0) piv[i] = p - i;
@@ -649,13 +693,15 @@ bool fits(int64_t m, int limit) {
 
 }  // namespace
 
-namespace {
-constexpr size_t PANEL_KEY_BYTES = 2 * (size_t)WMAX * KSTRIDE * sizeof(u64);
-constexpr size_t PANEL_CAND_BYTES = 2 * (size_t)WMAX * NB_MAX * 2 * sizeof(u64);
-constexpr size_t PANEL_LIST_BYTES = 4 * (size_t)NB_MAX * sizeof(int) + 256;
-}  // namespace
+size_t panel_area_bytes() {
+  return PANEL_KEY_BYTES + PANEL_CAND_BYTES + PANEL_LIST_BYTES + PANEL_ABORT_BYTES;
+}
 
-size_t panel_area_bytes() { return PANEL_KEY_BYTES + PANEL_CAND_BYTES + PANEL_LIST_BYTES; }
+unsigned* panel_abort_word(const PanelScratch* ext) {
+  void* area = ext ? ext->area : ctx().ws[WS_PANEL];
+  if (!area) return nullptr;
+  return reinterpret_cast<unsigned*>(static_cast<char*>(area) + PANEL_ABORT_OFF);
+}
 
 // exchange area for panel kernels; *base: the launch's tag range is base + 1 .. base + nb + 2
 // (a fresh range per launch, so words of earlier launches never match).  ext: an area the
@@ -678,6 +724,11 @@ int panel_sync(PanelSync* sy, int nb, unsigned* base, hipStream_t s, const Panel
   sy->cand = reinterpret_cast<u64*>(p + kb);
   sy->list = reinterpret_cast<int*>(p + kb + cb);
   sy->list_count = sy->list + 4 * NB_MAX;
+  // a limit other than the default travels in the area (once changed, it is written before
+  // every launch: the area may have been cleared since)
+  if (ctx().wait_limit_set)
+    DDLA_RETURN_IF(launch_fill_u32(p + PANEL_ABORT_OFF + sizeof(unsigned), 1,
+                                   ctx().wait_limit, s));
   return 0;
 }
 

@@ -968,7 +968,7 @@ int sgetrf_mixed(int64_t n, int prec, bool tournament, float* a, int64_t lda, Mi
   DDLA_RETURN_IF(launch_perm_compose((int)n, k, NBLK, lists, counts, w.perm, s));
   DDLA_HIP_CHECK(hipMemcpyAsync(info_host, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DDLA_HIP_CHECK(hipStreamSynchronize(s));
-  return 0;
+  return check_abort(panel_abort_word(nullptr), s, "mixed LU panel");
 }
 
 // FP32 LU without pivoting of the butterfly-transformed matrix (same block structure as
@@ -1185,6 +1185,7 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
     queued += group;
     DDLA_HIP_CHECK(hipMemcpyAsync(&st, w.state, sizeof(IrState), hipMemcpyDeviceToHost, s));
     DDLA_HIP_CHECK(hipStreamSynchronize(s));
+    if (few) DDLA_RETURN_IF(check_abort(trsv_abort_word(), s, "refinement triangular solve"));
     if (st.done) break;
   }
   *iters = st.iters;

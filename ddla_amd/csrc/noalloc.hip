@@ -19,9 +19,10 @@ namespace {
 constexpr int64_t NOALLOC_MAX_N = 32768;
 constexpr int64_t NOALLOC_MAX_PANELS = NOALLOC_MAX_N / 128;
 constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
-// panel exchange area (keys, candidate rows and the panel's row list: 1,116,416 bytes, checked
-// against panel_area_bytes() at set-up), then the info word, the per-panel counts and lists
-constexpr size_t AREA_BYTES = 1116416;
+// panel exchange area (keys, candidate rows, the panel's row list and the abort word: 1,116,672
+// bytes, checked against panel_area_bytes() at set-up), then the info word, the per-panel
+// counts and lists
+constexpr size_t AREA_BYTES = 1116672;
 constexpr size_t INFO_OFF = align256(AREA_BYTES);
 constexpr size_t COUNTS_OFF = INFO_OFF + 256;
 constexpr size_t LISTS_OFF = align256(COUNTS_OFF + NOALLOC_MAX_PANELS * sizeof(int));

@@ -131,6 +131,20 @@ int next_tag(int slot, size_t bytes, unsigned step, hipStream_t s, unsigned* tag
   return 0;
 }
 
+int check_abort(unsigned* word, hipStream_t s, const char* what) {
+  if (!word) return 0;
+  unsigned h = 0;
+  DDLA_HIP_CHECK(hipMemcpyAsync(&h, word, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  DDLA_HIP_CHECK(hipStreamSynchronize(s));
+  if (h == 0) return 0;
+  DDLA_RETURN_IF(launch_fill_u32(word, 1, 0u, s));
+  DDLA_HIP_CHECK(hipStreamSynchronize(s));
+  set_error("%s: a workgroup waited more than %u poll rounds for another one and the launch "
+            "gave up (workgroups not co-resident: other work on the device?)", what,
+            g_ctx.wait_limit);
+  return DDLA_ERR_HIP;
+}
+
 // ---------------------------------------------------------------- kernel timing ---------
 namespace {
 struct ProfEntry {

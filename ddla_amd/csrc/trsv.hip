@@ -42,12 +42,37 @@ __device__ __forceinline__ void publish(u64* p, T v, unsigned tag) {
   }
 }
 
+// Bound of a wait for another workgroup's word: past `limit` poll rounds the waiter stores
+// the call's tag in the abort word and gives up; a waiter that finds the tag there (another
+// workgroup gave up) gives up too.  The tag is never 0, so a cleared word names no call.
+struct WaitBound {
+  unsigned* abort;
+  unsigned limit;
+  unsigned tag;
+  __device__ __forceinline__ bool keep_waiting(unsigned& rounds) const {
+    ++rounds;
+    if (rounds > limit) {
+      __hip_atomic_store(abort, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    return (rounds & 63u) != 0 ||
+           __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != tag;
+  }
+};
+
+// the value published at p for this call; `failed` is set (and 0 returned) when the wait gives up
 template <typename T>
-__device__ __forceinline__ T await(const u64* p, unsigned tag) {
+__device__ __forceinline__ T await(const u64* p, const WaitBound& wb, bool& failed) {
+  const unsigned tag = wb.tag;
+  unsigned rounds = 0;
   if constexpr (sizeof(T) == 8) {
     u64 a = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     u64 b = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (static_cast<unsigned>(a) != tag || static_cast<unsigned>(b) != tag) {
+      if (!wb.keep_waiting(rounds)) {
+        failed = true;
+        return T(0);
+      }
       __builtin_amdgcn_s_sleep(1);
       a = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       b = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -56,6 +81,10 @@ __device__ __forceinline__ T await(const u64* p, unsigned tag) {
   } else {
     u64 a = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (static_cast<unsigned>(a) != tag) {
+      if (!wb.keep_waiting(rounds)) {
+        failed = true;
+        return T(0);
+      }
       __builtin_amdgcn_s_sleep(1);
       a = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -225,10 +254,14 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
                                                    int64_t lda, const T* __restrict__ Dinv,
                                                    T* __restrict__ B, int64_t ldb,
                                                    u64* __restrict__ XW, unsigned tag,
-                                                   int split, const int* __restrict__ skip) {
+                                                   int split, const int* __restrict__ skip,
+                                                   unsigned* __restrict__ abort_word,
+                                                   unsigned limit) {
   // a refinement that has already converged queues its remaining steps as no-ops: every
   // workgroup reads the same word (written by an earlier kernel) and leaves before any wait
   if (skip && *skip) return;
+  const WaitBound wb{abort_word, limit, tag};
+  bool failed = false;  // a wait of this thread gave up (the workgroup leaves at the next check)
   constexpr int Q = 256 / BS;       // threads per row of a block product (k split)
   constexpr int KQ = BS / Q;        // k range per thread
   constexpr int E = BS * BS / 256;  // staged block elements per thread
@@ -338,11 +371,16 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
         T v = T(0);
         if (c < nrhs && c0 + k < n) {
           const u64* p = XW + ((c0 + k) * nrhs + c) * NW;
+          unsigned rounds = 0;
           for (;;) {
             bool ok = true;
 #pragma unroll
             for (int w = 0; w < NW; ++w) ok = ok && static_cast<unsigned>(xw[t][w]) == tag;
-            if (ok) break;
+            if (ok || failed) break;
+            if (!wb.keep_waiting(rounds)) {
+              failed = true;
+              break;
+            }
             __builtin_amdgcn_s_sleep(1);
 #pragma unroll
             for (int w = 0; w < NW; ++w)
@@ -406,7 +444,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
       else if (!helper) load_dfull();
       use_x(jc);
       if (jn < nstream) load_x(jn);
-      __syncthreads();
+      if (__syncthreads_or(failed)) return;
 #pragma unroll 4
       for (int k = q * KQ; k < q * KQ + KQ; ++k) {
         const T a = Ts[row][k];
@@ -443,11 +481,12 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
         T sum = T(0);
 #pragma unroll
         for (int qq = 0; qq < Q; ++qq) sum += red[qq][r][c];
-        if (helped && c < nrhs && r < rows) sum += await<T>(HW + ((r0 + r) * nrhs + c) * NW, tag);
+        if (helped && c < nrhs && r < rows)
+          sum += await<T>(HW + ((r0 + r) * nrhs + c) * NW, wb, failed);
         xs[r][c] = (c < nrhs && r < rows) ? breg[t] - sum : T(0);
       }
     }
-    __syncthreads();
+    if (__syncthreads_or(failed)) return;
     T acc[NR];
 #pragma unroll
     for (int c = 0; c < NR; ++c) acc[c] = T(0);
@@ -470,7 +509,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
     if (jb > 1) {
       load_x(jb - 2);  // use_x only re-reads words that do not carry this call's tag yet
       use_x(jb - 2);
-      __syncthreads();
+      if (__syncthreads_or(failed)) return;
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = T(0);
 #pragma unroll
@@ -493,7 +532,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
     if (jb > 0) {
       load_x(jb - 1);
       use_x(jb - 1);
-      __syncthreads();
+      if (__syncthreads_or(failed)) return;
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = T(0);
 #pragma unroll 4
@@ -525,12 +564,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int64_t n, int nrhs, const T*
   }
 }
 
-template <typename T>
-struct TrsvState {
-  unsigned tag = 0;
-  void* ws = nullptr;
-  size_t bytes = 0;
-};
+constexpr size_t TRSV_ABORT_BYTES = 256;
 
 template <typename T, bool LOWER, bool TRANS>
 int max_trsv_groups() {
@@ -660,6 +694,8 @@ int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float*
   return 0;
 }
 
+unsigned* trsv_abort_word() { return static_cast<unsigned*>(ctx().ws[WS_GMRES]); }
+
 // inverted 64-blocks + Dfull + M + N (sized for the larger FP32 solve block)
 size_t trsv_dinv_elems(int64_t n) {
   return (size_t)(inv64_elems(n) + 3 * blocks_elems<128>(n));
@@ -675,8 +711,9 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
     return DDLA_ERR_INVALID_ARG;
   }
   void* ws = nullptr;
-  // X words, then the helpers' partial-sum words (2 words per FP64 value)
-  const size_t wbytes = (size_t)n * nrhs * 4 * sizeof(u64);
+  // the abort word (first 256 bytes), then the X words, then the helpers' partial-sum words
+  // (2 words per FP64 value)
+  const size_t wbytes = TRSV_ABORT_BYTES + (size_t)n * nrhs * 4 * sizeof(u64);
   // one tag sequence for the FP32 and FP64 solves: both use this workspace, so a word left
   // by a solve of either type never carries the tag of a later solve
   unsigned tag = 0;
@@ -685,23 +722,24 @@ int trsv_solve(bool lower, bool trans, int64_t n, int nrhs, const T* A, int64_t 
   const int nblk = ceil_div(n, solve_bs<T>());
   ProfScope prof(sizeof(T) == 8 ? "trsv_f64" : "trsv_f32", (double)n * n * nrhs,
                  (0.5 * n * n + 2.0 * n * nrhs) * sizeof(T), s);
-  u64* XW = static_cast<u64*>(ws);
-  int64_t n_ = n;
-  int nrhs_ = nrhs;
-  int64_t lda_ = lda, ldb_ = ldb;
-  unsigned tag_ = tag;
-  int split_ = 0;
-  void* args[] = {&n_, &nrhs_, &A, &lda_, &Dinv, &B, &ldb_, &XW, &tag_, &split_, &skip};
-  // a helper workgroup per block row when twice the rows fit co-resident: the longest rows
-  // stream their blocks with two workgroups
-#define TS(NR, L, TR)                                                                    \
-  do {                                                                                   \
-    const int maxg = max_trsv_groups<T, L, TR>();                                        \
-    split_ = 2 * nblk <= maxg ? 1 : 0;                                                   \
-    const int G = split_ ? 2 * nblk : std::min(nblk, maxg);                              \
-    DDLA_HIP_CHECK(hipLaunchCooperativeKernel(                                           \
-        reinterpret_cast<const void*>(trsv_kernel<T, solve_bs<T>(), NR, L, TR>), dim3(G), \
-        dim3(256), args, 0, s));                                                         \
+  unsigned* abort_word = static_cast<unsigned*>(ws);
+  u64* XW = reinterpret_cast<u64*>(static_cast<char*>(ws) + TRSV_ABORT_BYTES);
+  const unsigned limit = ctx().wait_limit;
+  // A plain launch of at most the co-resident number of workgroups: block row jb waits only on
+  // the rows before it, which lower workgroup ids (dispatched first) or this workgroup's
+  // previous rows compute, and on its helper, which is launched only when every workgroup fits
+  // at once.  Waits are bounded (the abort word) in case other work keeps workgroups out.
+  // A helper workgroup per block row when twice the rows fit co-resident: the longest rows
+  // stream their blocks with two workgroups.
+#define TS(NR, L, TR)                                                                       \
+  do {                                                                                      \
+    const int maxg = max_trsv_groups<T, L, TR>();                                           \
+    const int split = 2 * nblk <= maxg ? 1 : 0;                                             \
+    const int G = split ? 2 * nblk : std::min(nblk, maxg);                                  \
+    hipLaunchKernelGGL((trsv_kernel<T, solve_bs<T>(), NR, L, TR>), dim3(G), dim3(256), 0, s, \
+                       n, nrhs, A, lda, Dinv, B, ldb, XW, tag, split, skip, abort_word,      \
+                       limit);                                                              \
+    DDLA_HIP_CHECK(hipGetLastError());                                                      \
   } while (0)
 #define TSN(L, TR)                          \
   do {                                      \

@@ -49,6 +49,7 @@ SIGNATURES = {
     "ddla_get_stream": [ctypes.POINTER(vp)],
     "ddla_sync": [],
     "ddla_version": [],
+    "ddla_set_wait_limit": [u64],
     "ddla_profile_enable": [cint],
     "ddla_profile_reset": [],
     "ddla_profile_read": [cint, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(i64),

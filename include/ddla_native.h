@@ -84,6 +84,12 @@ int ddla_get_stream(void** stream);
 int ddla_sync(void);
 /* Library version as 10000*major + 100*minor + patch. */
 int ddla_version(void);
+/* No counterpart in the reference.  The LU panel and the persistent triangular solve run as
+ * workgroups that wait on each other's results; each wait gives up after `rounds` poll rounds
+ * (0 restores the default, 2^24, many seconds), the launch's workgroups leave, and the call
+ * returns DDLA_ERR_HIP instead of hanging -- e.g. when other work keeps the workgroups from
+ * becoming co-resident. */
+int ddla_set_wait_limit(uint64_t rounds);
 
 /* Per-kernel timing, no counterpart in the reference.  While enabled, every launch of the
  * library's kernels is bracketed by HIP events on its stream; the totals per kernel name
@@ -145,7 +151,9 @@ int ddla_dgetrf_noalloc_dev(ddla_dim_t n, double* a, ddla_inc_t lda, int32_t* dp
  * ddla_dgetrf_dev; B (n x nrhs row-major, device) is overwritten with X.
  * Replaces LU.PLUSolve (LU.cs:266-287, PLUSolveInternel :431-465).  The pivots are checked on
  * the device first (a pivot that would swap outside the matrix returns DDLA_ERR_INVALID_ARG,
- * where the reference's SwapRow throws), which synchronises the stream. */
+ * where the reference's SwapRow throws), which synchronises the stream; with at most 8
+ * right-hand sides the stream is synchronised again at the end to report a triangular-solve
+ * wait that gave up (ddla_set_wait_limit). */
 int ddla_dgetrs_dev(int trans, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
                     const int32_t* dpiv, double* b, ddla_inc_t ldb);
 /* Host-pointer forms with strided views (synchronous). */

@@ -295,3 +295,42 @@ def test_pivots_out_of_range_are_rejected(ddla):
     piv[10] = -3
     dpiv = ddla._DevInts.from_host(piv)
     assert lib.ddla_dlaswp_dev(ddla.LEFT, 1, n, 2, dB.ptr, dB.ld, dpiv.ptr, n) == 0
+
+
+def test_bounded_waits_give_up_and_recover(ddla, oracle):
+    # The panel and the triangular solve are workgroups that wait on each other; every wait is
+    # bounded.  A tiny bound set from the host makes the waits give up (every kernel still
+    # exits normally): the call reports DDLA_ERR_HIP, and the next calls with the default bound
+    # run as usual (no state left behind).
+    lib = ddla.native.lib()
+    n = 4096
+    A0 = oracle.uniform_matrix(n, n, 77)
+    d = ddla.DeviceMatrix.from_host(A0)
+    dpiv = ddla._DevInts(n)
+    assert lib.ddla_set_wait_limit(1) == 0
+    try:
+        st = lib.ddla_dgetrf_dev(n, d.ptr, d.ld, dpiv.ptr)
+    finally:
+        assert lib.ddla_set_wait_limit(0) == 0
+    assert st == ddla.native.ERR_HIP
+    assert "gave up" in ddla.native.last_error()
+    # a correct factorization afterwards
+    d = ddla.DeviceMatrix.from_host(A0)
+    piv = ddla.PLUDecompose(d)
+    Ao = A0.copy()
+    _, piv_o = oracle.getrf(Ao)
+    np.testing.assert_array_equal(piv, piv_o)
+    # the solve's waits
+    b = oracle.uniform_matrix(n, 1, 78)
+    db = ddla.DeviceMatrix.from_host(b)
+    dp = ddla._DevInts.from_host(piv)
+    assert lib.ddla_set_wait_limit(1) == 0
+    try:
+        st = lib.ddla_dgetrs_dev(0, n, 1, d.ptr, d.ld, dp.ptr, db.ptr, db.ld)
+    finally:
+        assert lib.ddla_set_wait_limit(0) == 0
+    assert st == ddla.native.ERR_HIP
+    db = ddla.DeviceMatrix.from_host(b)
+    ddla.PLUSolve(d, piv, db)
+    x = db.to_host()
+    assert backward_error(A0, x[:, 0], b[:, 0]) < 1e-14
