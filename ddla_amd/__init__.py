@@ -25,7 +25,7 @@ __all__ = [
     "synchronize", "NO_TRANSPOSE", "TRANSPOSE", "LOWER", "UPPER", "UNIT_DIAG", "NONUNIT_DIAG",
     "LEFT", "RIGHT", "PREC_FP64", "PREC_FP32", "PREC_FP16", "PREC_BF16", "DDLAError",
     "BatchPlan", "batch_factor", "BATCH_LU", "BATCH_CHOL", "sgetrf_mp", "PIVOT_PARTIAL",
-    "PIVOT_BUTTERFLY", "PIVOT_TOURNAMENT",
+    "PIVOT_BUTTERFLY", "PIVOT_TOURNAMENT", "LDLT", "LDLTDecompose", "LDLTSolve",
 ]
 
 
@@ -524,6 +524,76 @@ class Cholesky:
         x = b if inplace else np.array(_as_f64(b), copy=True)
         CholeskySolve(self.uplo, self.matrix, x)
         return x
+
+
+def LDLTDecompose(uplo, A):
+    """Blocked LDL^T without pivoting in place (LDLT.cs:56-73): lower A = L D L^T, upper
+    A = U^T D U.  Raises LinalgException on a zero pivot (the reference checks every pivot
+    but the last of each 128-block, LDLT.cs:145-152; this checks all of them)."""
+    if isinstance(A, DeviceMatrix):
+        if A.rows != A.cols:
+            raise ValueError("Matrix must be square")
+        info = check(_lib().ddla_dldlt_dev(uplo, A.rows, A.ptr, A.ld))
+    else:
+        A = _as_f64(A)
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("Matrix must be square")
+        info = check(_lib().ddla_dldlt(uplo, A.shape[0], _ptr(A), *_strides(A)))
+    if info:
+        raise LinalgException(f"LDLT factorization encountered a zero pivot at {info - 1}")
+    return 0
+
+
+def LDLTSolve(uplo, A, B):
+    """B <- A^-1 B with an LDL^T factor (LDLT.cs:105-126, :267-295)."""
+    if isinstance(A, DeviceMatrix):
+        if B.rows != A.rows:
+            raise ValueError("Matrix dimensions do not match for Cholesky solve.")
+        return check(_lib().ddla_dldlt_solve_dev(uplo, A.rows, B.cols, A.ptr, A.ld, B.ptr, B.ld))
+    A, B = _as_f64(A), _as_f64(B)
+    B2 = B.reshape(-1, 1) if B.ndim == 1 else B
+    if B2.shape[0] != A.shape[0]:
+        raise ValueError("Matrix dimensions do not match for Cholesky solve.")
+    return check(_lib().ddla_dldlt_solve(uplo, A.shape[0], B2.shape[1], _ptr(A), *_strides(A),
+                                         _ptr(B2), *_strides(B2)))
+
+
+class LDLT:
+    """LDL^T factorization (LDLT.cs:9-137); computed once, lazily."""
+
+    def __init__(self, A, uplo=LOWER, inplace=False):
+        A = _as_f64(A)
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("Matrix must be square")
+        self.uplo = uplo
+        self.matrix = A if inplace else np.array(A, order="C", copy=True)
+        self._computed = False
+        self._deconstructed = False
+
+    def ComputeOnce(self):
+        if self._deconstructed:
+            raise RuntimeError("Matrix has been deconstructed, cannot compute again.")
+        if not self._computed:
+            self._computed = True
+            LDLTDecompose(self.uplo, self.matrix)
+
+    def Solve(self, b, inplace=False):
+        self.ComputeOnce()
+        x = b if inplace else np.array(_as_f64(b), copy=True)
+        LDLTSolve(self.uplo, self.matrix, x)
+        return x
+
+    def Deconstruct(self):
+        """(L, D): the unit triangle of the chosen side and the diagonal matrix
+        (LDLT.cs:128-137); for UPPER the triangle returned is U."""
+        self.ComputeOnce()
+        D = np.diag(np.diag(self.matrix).copy())
+        L = np.tril(self.matrix) if self.uplo == LOWER else np.triu(self.matrix)
+        np.fill_diagonal(L, 1.0)
+        self._deconstructed = True
+        return L, D
+
+
 PIVOT_PARTIAL, PIVOT_BUTTERFLY, PIVOT_TOURNAMENT = 0, 1, 2
 
 

@@ -550,6 +550,57 @@ int ddla_dpotrs(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_i
   return download_view(n, nrhs, dB, b, rsb, csb);
 }
 
+int ddla_dldlt_dev(int uplo, ddla_dim_t n, double* a, ddla_inc_t lda) {
+  API_INIT();
+  DDLA_ARG_CHECK(uplo == DDLA_LOWER || uplo == DDLA_UPPER,
+                 "Matrix must be either upper or lower triangular.");
+  DDLA_ARG_CHECK(n >= 0 && lda >= n, "bad ldlt arguments");
+  if (n == 0) return 0;
+  DDLA_ARG_CHECK(a != nullptr, "null matrix pointer");
+  int32_t info = 0;
+  DDLA_RETURN_IF(dldlt_internal(uplo == DDLA_LOWER, n, a, lda, &info));
+  return info;
+}
+
+int ddla_dldlt_solve_dev(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a,
+                         ddla_inc_t lda, double* b, ddla_inc_t ldb) {
+  API_INIT();
+  DDLA_ARG_CHECK(uplo == DDLA_LOWER || uplo == DDLA_UPPER,
+                 "Matrix must be either upper or lower triangular.");
+  DDLA_ARG_CHECK(n >= 0 && nrhs >= 0 && lda >= n && ldb >= nrhs, "bad ldlt solve arguments");
+  if (n == 0 || nrhs == 0) return 0;
+  DDLA_ARG_CHECK(a && b, "null matrix pointer");
+  return dldlt_solve_internal(uplo == DDLA_LOWER, n, nrhs, a, lda, b, ldb);
+}
+
+int ddla_dldlt(int uplo, ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t csa) {
+  API_INIT();
+  DDLA_ARG_CHECK(uplo == DDLA_LOWER || uplo == DDLA_UPPER,
+                 "Matrix must be either upper or lower triangular.");
+  DDLA_ARG_CHECK(n >= 0, "negative n");
+  if (n == 0) return 0;
+  DevBuf dA;
+  DDLA_RETURN_IF(upload_view(n, n, a, rsa, csa, dA));
+  int32_t info = 0;
+  DDLA_RETURN_IF(dldlt_internal(uplo == DDLA_LOWER, n, dA.d(), n, &info));
+  DDLA_RETURN_IF(download_view(n, n, dA, a, rsa, csa));
+  return info;
+}
+
+int ddla_dldlt_solve(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
+                     ddla_inc_t csa, double* b, ddla_inc_t rsb, ddla_inc_t csb) {
+  API_INIT();
+  DDLA_ARG_CHECK(uplo == DDLA_LOWER || uplo == DDLA_UPPER,
+                 "Matrix must be either upper or lower triangular.");
+  DDLA_ARG_CHECK(n >= 0 && nrhs >= 0, "negative size");
+  if (n == 0 || nrhs == 0) return 0;
+  DevBuf dA, dB;
+  DDLA_RETURN_IF(upload_view(n, n, a, rsa, csa, dA));
+  DDLA_RETURN_IF(upload_view(n, nrhs, b, rsb, csb, dB));
+  DDLA_RETURN_IF(dldlt_solve_internal(uplo == DDLA_LOWER, n, nrhs, dA.d(), n, dB.d(), nrhs));
+  return download_view(n, nrhs, dB, b, rsb, csb);
+}
+
 int ddla_dtrsm_dev(int side, int uplo, int transa, int diag, ddla_dim_t m, ddla_dim_t n,
                    const double* alpha, const double* a, ddla_inc_t lda, double* b,
                    ddla_inc_t ldb) {

@@ -282,11 +282,12 @@ __device__ __forceinline__ void rlt_steps(std::integer_sequence<int, Ks...>, Qua
 template <typename T, bool TL>
 __global__ __launch_bounds__(256) void trsm_rlt_kernel(int nb, int64_t nrows,
                                                        const T* __restrict__ L, int64_t ldl,
-                                                       T* __restrict__ B, int64_t ldb) {
+                                                       T* __restrict__ B, int64_t ldb,
+                                                       bool unit) {
   __shared__ Quad4<T> U[NB][NB / 4];  // strictly upper part of U, zero beyond nb
-  __shared__ T rinv[NB];  // 1 / u_kk (1 beyond nb)
+  __shared__ T rinv[NB];  // 1 / u_kk (1 beyond nb, and everywhere for a unit diagonal)
   const int tid = threadIdx.x;
-  const T dv = tid < nb ? L[(int64_t)tid * ldl + tid] : T(1);  // divided after the staging
+  const T dv = (tid < nb && !unit) ? L[(int64_t)tid * ldl + tid] : T(1);  // divided later
   // batches of 16 branch-free reads per thread in flight together
   for (int b0 = 0; b0 < NB * NB / 256; b0 += 16) {
     T v[16];
@@ -350,6 +351,113 @@ __global__ __launch_bounds__(256) void transpose_kernel(int64_t n, double* __res
   }
 }
 
+// LDL^T of one nb x nb diagonal block, lower triangle (LDLTLowerUnblock, LDLT.cs:141-158): for
+// each column i with pivot d = a_ii, l = a21 * (1/d) (InvScaled multiplies by the reciprocal)
+// and the lower triangle of A22 gets a_rc += (-d l_r) l_c.  The block's lower triangle sits in
+// LDS; column i of the block is also kept in a double-buffered LDS vector, written by the
+// previous step, so one barrier per column suffices.  Thread (wave, lane) updates rows
+// wave + 4k and columns i + 1 + lane, i + 65 + lane.  Only the lower triangle (with the
+// diagonal) is read and written back.  A zero pivot sets *info = info_base + i + 1 (first
+// failure wins) and leaves the block as it stands.
+__global__ __launch_bounds__(256) void ldlt_diag_kernel(int nb, double* __restrict__ A,
+                                                        int64_t lda, int32_t* __restrict__ info,
+                                                        int info_base) {
+  constexpr int LD = NB + 1;
+  __shared__ double t[NB * LD];
+  __shared__ double colv[2][NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  for (int b0 = 0; b0 < NB * NB / 256; b0 += 16) {  // 16 reads in flight per thread
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int idx = (b0 + q) * 256 + tid, r = idx / NB, c = idx % NB;
+      v[q] = A[(r < nb && c <= r) ? (int64_t)r * lda + c : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int idx = (b0 + q) * 256 + tid, r = idx / NB, c = idx % NB;
+      if (r < nb && c <= r) {
+        t[r * LD + c] = v[q];
+        if (c == 0) colv[0][r] = v[q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = 0; i < nb; ++i) {
+    const int par = i & 1;
+    const double d = colv[par][i];
+    if (d == 0.0) {
+      if (tid == 0) atomicCAS(info, 0, info_base + i + 1);
+      break;
+    }
+    const double rinv = 1.0 / d;
+    const int c0 = i + 1 + lane, c1 = c0 + 64;
+    const bool h0 = c0 < nb, h1 = c1 < nb;
+    const double l0 = h0 ? colv[par][c0] * rinv : 0.0;
+    const double l1 = h1 ? colv[par][c1] * rinv : 0.0;
+    constexpr int CH = 8;
+    for (int k0 = 0; k0 < NB / 4; k0 += CH) {
+      double a0[CH], a1[CH], x[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int r = wave + 4 * (k0 + k);
+        const double* tr = t + r * LD;
+        x[k] = colv[par][r];
+        a0[k] = (h0 && c0 <= r) ? tr[c0] : 0.0;
+        a1[k] = (h1 && c1 <= r) ? tr[c1] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int r = wave + 4 * (k0 + k);
+        if (r > i && r < nb) {
+          double* tr = t + r * LD;
+          const double lr = x[k] * rinv;
+          const double ar = -d * lr;  // alpha x_r with alpha = -d
+          if (h0 && c0 <= r) {
+            const double v = fma(ar, l0, a0[k]);
+            tr[c0] = v;
+            if (c0 == i + 1) colv[par ^ 1][r] = v;  // the next column, for the next step
+          }
+          if (h1 && c1 <= r) tr[c1] = fma(ar, l1, a1[k]);
+          if (lane == 0) tr[i] = lr;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int r = idx / nb, c = idx % nb;
+    if (c <= r) A[(int64_t)r * lda + c] = t[r * LD + c];
+  }
+}
+
+// Y = A21 (rows x b) and A21[:, j] *= 1 / d_j with d_j = A11[j, j] (LDLT.cs:188-191)
+__global__ __launch_bounds__(256) void ldlt_scale_kernel(int64_t rows, int b,
+                                                         const double* __restrict__ A11,
+                                                         int64_t lda, double* __restrict__ A21,
+                                                         double* __restrict__ Y, int64_t ldy) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= rows * b) return;
+  const int64_t r = e / b;
+  const int c = static_cast<int>(e % b);
+  const double v = A21[r * lda + c];
+  Y[r * ldy + c] = v;
+  A21[r * lda + c] = v * (1.0 / A11[(int64_t)c * lda + c]);
+}
+
+// X[i, :] *= 1 / A[i, i] (LDLT.cs:274-275)
+__global__ __launch_bounds__(256) void scale_rows_by_inv_diag_kernel(int64_t n, int64_t nrhs,
+                                                                     const double* __restrict__ A,
+                                                                     int64_t lda,
+                                                                     double* __restrict__ X,
+                                                                     int64_t ldx) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * nrhs) return;
+  const int64_t r = e / nrhs, c = e % nrhs;
+  X[r * ldx + c] *= 1.0 / A[r * lda + r];
+}
+
 }  // namespace
 
 template <typename T>
@@ -368,7 +476,7 @@ int launch_potf2_ex(int nb, T* A, int64_t lda, int32_t* info, int info_base, hip
 
 template <typename T>
 int launch_trsm_rlt_panel(int nb, int64_t nrows, const T* L, int64_t ldl, T* B, int64_t ldb,
-                          hipStream_t s, bool l_transposed) {
+                          hipStream_t s, bool l_transposed, bool unit) {
   if (nrows <= 0 || nb <= 0) return 0;
   if (nb > NB) {
     set_error("trsm block %d > %d", nb, NB);
@@ -378,10 +486,10 @@ int launch_trsm_rlt_panel(int nb, int64_t nrows, const T* L, int64_t ldl, T* B, 
                  sizeof(T) * (2.0 * nrows * nb + 0.5 * nb * nb), s);
   if (l_transposed)
     hipLaunchKernelGGL((trsm_rlt_kernel<T, true>), dim3(ceil_div(nrows, 64)), dim3(256), 0, s, nb,
-                       nrows, L, ldl, B, ldb);
+                       nrows, L, ldl, B, ldb, unit);
   else
     hipLaunchKernelGGL((trsm_rlt_kernel<T, false>), dim3(ceil_div(nrows, 64)), dim3(256), 0, s, nb,
-                       nrows, L, ldl, B, ldb);
+                       nrows, L, ldl, B, ldb, unit);
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -408,12 +516,45 @@ int launch_dtranspose_inplace(int64_t n, double* a, int64_t lda, hipStream_t s) 
   return 0;
 }
 
+int launch_ldlt_diag(int nb, double* A, int64_t lda, int32_t* info, int info_base,
+                     hipStream_t s) {
+  if (nb <= 0) return 0;
+  if (nb > NB) {
+    set_error("ldlt block %d > %d", nb, NB);
+    return DDLA_ERR_INVALID_ARG;
+  }
+  ProfScope prof("ldlt_diag_f64", (double)nb * nb * nb / 3.0, 8.0 * nb * nb, s);
+  hipLaunchKernelGGL(ldlt_diag_kernel, dim3(1), dim3(256), 0, s, nb, A, lda, info, info_base);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_ldlt_scale(int64_t rows, int b, const double* A11, int64_t lda, double* A21,
+                      double* Y, int64_t ldy, hipStream_t s) {
+  const int64_t total = rows * b;
+  if (total <= 0) return 0;
+  hipLaunchKernelGGL(ldlt_scale_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, s, rows, b, A11,
+                     lda, A21, Y, ldy);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_scale_rows_by_inv_diag(int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                                  double* X, int64_t ldx, hipStream_t s) {
+  const int64_t total = n * nrhs;
+  if (total <= 0) return 0;
+  hipLaunchKernelGGL(scale_rows_by_inv_diag_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, s,
+                     n, nrhs, A, lda, X, ldx);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 template int launch_potf2_ex<double>(int, double*, int64_t, int32_t*, int, hipStream_t);
 template int launch_potf2_ex<float>(int, float*, int64_t, int32_t*, int, hipStream_t);
 template int launch_trsm_rlt_panel<double>(int, int64_t, const double*, int64_t, double*, int64_t,
-                                           hipStream_t, bool);
+                                           hipStream_t, bool, bool);
 template int launch_trsm_rlt_panel<float>(int, int64_t, const float*, int64_t, float*, int64_t,
-                                          hipStream_t, bool);
+                                          hipStream_t, bool, bool);
 template int launch_getf2_nopiv<double>(int, double*, int64_t, int32_t*, int, hipStream_t);
 template int launch_getf2_nopiv<float>(int, float*, int64_t, int32_t*, int, hipStream_t);
 

@@ -286,14 +286,23 @@ int trsv_solve_blocked(bool lower, bool trans, int64_t n, int nrhs, const float*
 // Cholesky pieces (chol_kernels.hip)
 template <typename T>
 int launch_potf2_ex(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s);
-// B <- B L^-T for lower L (l_transposed: B <- B U^-1 for the upper U stored at L)
+// B <- B L^-T for lower L (l_transposed: B <- B U^-1 for the upper U stored at L); unit: the
+// triangle's diagonal is taken as ones (not read)
 template <typename T>
 int launch_trsm_rlt_panel(int nb, int64_t nrows, const T* L, int64_t ldl, T* B, int64_t ldb,
-                          hipStream_t s, bool l_transposed);
+                          hipStream_t s, bool l_transposed, bool unit = false);
 // LU without pivoting of one nb x nb diagonal block (nb <= 128)
 template <typename T>
 int launch_getf2_nopiv(int nb, T* A, int64_t lda, int32_t* info, int info_base, hipStream_t s);
 int launch_dtranspose_inplace(int64_t n, double* a, int64_t lda, hipStream_t s);
+// LDL^T pieces (chol_kernels.hip): the nb x nb diagonal block (nb <= 128, lower triangle);
+// Y = A21, A21 *= diag(A11)^-1 by columns; X[i, :] *= 1 / A[i, i]
+int launch_ldlt_diag(int nb, double* A, int64_t lda, int32_t* info, int info_base,
+                     hipStream_t s);
+int launch_ldlt_scale(int64_t rows, int b, const double* A11, int64_t lda, double* A21,
+                      double* Y, int64_t ldy, hipStream_t s);
+int launch_scale_rows_by_inv_diag(int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                                  double* X, int64_t ldx, hipStream_t s);
 
 // relative pivots that stay inside n rows (0 <= i + piv[i] < n wherever piv[i] > 0), checked on
 // the device; synchronises s and returns DDLA_ERR_INVALID_ARG otherwise
@@ -320,6 +329,9 @@ int dgetrf_internal(int64_t n, double* a, int64_t lda, int32_t* dpiv, int32_t* i
 int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_t lda,
                     const int32_t* dpiv, double* b, int64_t ldb);
 int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host);
+int dldlt_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host);
+int dldlt_solve_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda,
+                         double* b, int64_t ldb);
 int dpotrs_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda, double* b,
                     int64_t ldb);
 int dgetrf_nopiv_internal(int64_t n, double* a, int64_t lda, int32_t* info_host);

@@ -7,6 +7,9 @@
 //   dpotrf_internal  CholeskyLowerBlock (DDLA/Factorizations/Cholesky.cs:142-160); the upper
 //                    form factors the transposed storage with the same kernels.
 //   dpotrs_internal  CholeskySolveUncheck (Cholesky.cs:210-226).
+//   dldlt_internal   LDLTLowerBlock (DDLA/Factorizations/LDLT.cs:160-202); the upper form
+//                    (LDLTUpperBlock, :223-265) factors the transposed storage.
+//   dldlt_solve_internal  LDLTSolveUncheck (LDLT.cs:267-295).
 #include <algorithm>
 
 #include "ddla_internal.hpp"
@@ -272,6 +275,48 @@ int dpotrs_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_
     DDLA_RETURN_IF(tri_solve(false, false, true, n, nrhs, a, lda, b, ldb));
     DDLA_RETURN_IF(tri_solve(false, false, false, n, nrhs, a, lda, b, ldb));
   }
+  return solves_done(nrhs, stream());
+}
+
+// Per 128-column block: the diagonal block's LDL^T (one workgroup), A21 <- A21 L11^-T (unit
+// diagonal), Y21 = A21 and A21 <- A21 D11^-1 (the multipliers), then the lower triangle of A22
+// takes -Y21 A21^T on the GEMM (GeMMT, LDLT.cs:185-193).  A zero pivot d_k gives info k + 1.
+int dldlt_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host) {
+  hipStream_t s = stream();
+  int32_t* dinfo;
+  int *counts, *lists;
+  DDLA_RETURN_IF(lu_bookkeeping(1, &dinfo, &counts, &lists));
+  DDLA_RETURN_IF(launch_fill_u32(dinfo, 1, 0u, s));
+  double* y = nullptr;
+  if (n > NBLK) {
+    void* ws = nullptr;
+    DDLA_RETURN_IF(workspace(WS_TEMP, (size_t)(n - NBLK) * NBLK * sizeof(double), &ws));
+    y = static_cast<double*>(ws);
+  }
+  if (!lower) DDLA_RETURN_IF(launch_dtranspose_inplace(n, a, lda, s));
+  for (int64_t j = 0; j < n; j += NBLK) {
+    const int b = (int)std::min<int64_t>(NBLK, n - j);
+    double* a11 = a + j * lda + j;
+    DDLA_RETURN_IF(launch_ldlt_diag(b, a11, lda, dinfo, (int)j, s));
+    const int64_t rest = n - j - b;
+    if (rest <= 0) break;
+    double* a21 = a11 + b * lda;
+    DDLA_RETURN_IF(launch_trsm_rlt_panel<double>(b, rest, a11, lda, a21, lda, s, false, true));
+    DDLA_RETURN_IF(launch_ldlt_scale(rest, b, a11, lda, a21, y, b, s));
+    // lower(A22) -= Y21 * A21^T
+    DDLA_RETURN_IF(launch_dgemm(rest, rest, b, -1.0, y, b, true, a21, lda, false, 1.0, a21 + b,
+                                lda, 1, s));
+  }
+  if (!lower) DDLA_RETURN_IF(launch_dtranspose_inplace(n, a, lda, s));
+  return read_info(dinfo, info_host);
+}
+
+// lower: X <- L^-1 X, X <- D^-1 X, X <- L^-T X; upper: the same with L = U^T
+int dldlt_solve_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda,
+                         double* b, int64_t ldb) {
+  DDLA_RETURN_IF(tri_solve(lower, true, !lower, n, nrhs, a, lda, b, ldb));
+  DDLA_RETURN_IF(launch_scale_rows_by_inv_diag(n, nrhs, a, lda, b, ldb, stream()));
+  DDLA_RETURN_IF(tri_solve(lower, true, lower, n, nrhs, a, lda, b, ldb));
   return solves_done(nrhs, stream());
 }
 

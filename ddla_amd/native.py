@@ -77,6 +77,10 @@ SIGNATURES = {
     "ddla_dpotrs_dev": [cint, i64, i64, vp, i64, vp, i64],
     "ddla_dpotrf": [cint, i64, vp, i64, i64],
     "ddla_dpotrs": [cint, i64, i64, vp, i64, i64, vp, i64, i64],
+    "ddla_dldlt_dev": [cint, i64, vp, i64],
+    "ddla_dldlt_solve_dev": [cint, i64, i64, vp, i64, vp, i64],
+    "ddla_dldlt": [cint, i64, vp, i64, i64],
+    "ddla_dldlt_solve": [cint, i64, i64, vp, i64, i64, vp, i64, i64],
     "ddla_dtrsm_dev": [cint, cint, cint, cint, i64, i64, dp, vp, i64, vp, i64],
     "ddla_dsyrk_dev": [cint, cint, i64, i64, dp, vp, i64, dp, vp, i64],
     "ddla_damaxv_dev": [i64, vp, i64, ctypes.POINTER(i64)],

@@ -188,6 +188,24 @@ int ddla_dpotrf(int uplo, ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t cs
 int ddla_dpotrs(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
                 ddla_inc_t csa, double* b, ddla_inc_t rsb, ddla_inc_t csb);
 
+/* ---------------------------------------------------------------------- LDL^T -------- */
+/* Blocked LDL^T without pivoting (block 128) of a row-major n x n symmetric device matrix in
+ * place: lower A = L D L^T, upper A = U^T D U, the unit triangle stored below (above) the
+ * diagonal and D on it; only the uplo triangle is read and written.  Replaces
+ * LDLT.LDLTDecompose (DDLA/Factorizations/LDLT.cs:56-73; lower :141-202, upper :204-265).
+ * Returns info = 1-based index of the first zero pivot d_k (the reference throws
+ * LinalgException for a zero pivot, LDLT.cs:148-152, but does not test the last pivot of
+ * each 128-block; every pivot is tested here). */
+int ddla_dldlt_dev(int uplo, ddla_dim_t n, double* a, ddla_inc_t lda);
+/* Solves A X = B with the factor of ddla_dldlt_dev (LDLT.LDLTSolve, LDLT.cs:105-126,
+ * :267-295). */
+int ddla_dldlt_solve_dev(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a,
+                         ddla_inc_t lda, double* b, ddla_inc_t ldb);
+/* Host forms with (row stride, column stride) views. */
+int ddla_dldlt(int uplo, ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t csa);
+int ddla_dldlt_solve(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
+                     ddla_inc_t csa, double* b, ddla_inc_t rsb, ddla_inc_t csb);
+
 /* ------------------------------------------------------------ level-3 pieces ------- */
 /* B = alpha * op(A)^-1 B (left) or B op(A)^-1 (right), row-major device matrices with leading
  * dimensions: the device-resident form of bli_dtrsm (DDLA/BLAS/Native/3m.cs:151-162) for the

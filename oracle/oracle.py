@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: a CPU restatement of DDLA's managed FP64 GEMM, blocked LU with
-partial pivoting and blocked Cholesky, used to check the HIP library and as the timed CPU
+partial pivoting, blocked Cholesky and (in numpy) blocked LDL^T, used to check the HIP library and as the timed CPU
 baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import it.
 Every function mirrors the reference routine named in its docstring (paths relative to
 the DDLA repository root); the C source cites the same lines.
@@ -272,6 +272,69 @@ def potrs(A, B, uplo=LOWER):
     n, nrhs = B.shape
     lib().orc_dpotrs(uplo, n, nrhs, _dp(A), *_strides(A), _dp(B), *_strides(B))
     return B
+
+
+def _ldlt_unblocked(At):
+    """LDLTLowerUnblock (DDLA/Factorizations/LDLT.cs:141-158) on the lower triangle of At, in
+    place: for i < n - 1 a zero a_ii stops with i + 1, else a21 *= 1/a_ii (InvScaled) and the
+    lower triangle of A22 gets a_rc += (-a_ii l_r) l_c.  The last pivot is not tested (as in
+    the reference)."""
+    n = At.shape[0]
+    for i in range(n - 1):
+        d = At[i, i]
+        if d == 0.0:
+            return i + 1
+        At[i + 1:, i] *= 1.0 / d
+        l = At[i + 1:, i].copy()
+        for r in range(i + 1, n):
+            At[r, i + 1:r + 1] += (-d * l[r - i - 1]) * l[:r - i]
+    return 0
+
+
+def ldlt(A, uplo=LOWER, block=128):
+    """LDLTDecompose (LDLT.cs:56-73) in place; returns info (0, or k + 1 for the first zero
+    pivot the reference tests).  Lower: LDLTLowerBlock (:160-202): per block, the unblocked
+    factor of A11, A21 <- A21 L11^-T (unit), Y21 = A21, A21[:, j] *= 1/d_j, lower(A22) -=
+    Y21 A21^T.  Upper (LDLTUpperBlock, :223-265) is the same sequence on the transposed
+    storage (A = U^T D U), which is how it is run here.  Unblocked when n <= block."""
+    import scipy.linalg as sla
+    At = A if uplo == LOWER else A.T
+    n = At.shape[0]
+    if n <= block:
+        return _ldlt_unblocked(At)
+    for j in range(0, n, block):
+        b = min(block, n - j)
+        info = _ldlt_unblocked(At[j:j + b, j:j + b])
+        if info:
+            return j + info
+        if j + b >= n:
+            break
+        A11 = At[j:j + b, j:j + b]
+        A21 = At[j + b:, j:j + b]
+        A21[...] = sla.solve_triangular(A11, A21.T, lower=True, unit_diagonal=True).T
+        Y = A21.copy()
+        A21 *= 1.0 / np.diag(A11)[None, :]
+        A22 = At[j + b:, j + b:]
+        A22[...] = np.where(np.tril(np.ones(A22.shape, bool)), A22 - Y @ A21.T, A22)
+    return 0
+
+
+def ldlt_solve(A, B, uplo=LOWER):
+    """LDLTSolveUncheck (LDLT.cs:267-295): X <- L^-1 X, X[i, :] *= 1/d_i, X <- L^-T X with
+    L the unit lower triangle (U^T for UPPER); B overwritten with X."""
+    import scipy.linalg as sla
+    At = A if uplo == LOWER else A.T
+    X = sla.solve_triangular(At, B, lower=True, unit_diagonal=True)
+    X *= (1.0 / np.diag(At))[:, None]
+    B[...] = sla.solve_triangular(At.T, X, lower=False, unit_diagonal=True)
+    return B
+
+
+def ldlt_factors(F, uplo=LOWER):
+    """(T, d): the unit triangle of the chosen side and the diagonal of an LDL^T factor
+    (LDLT.Deconstruct, LDLT.cs:128-137)."""
+    T = np.tril(F, -1) + np.eye(F.shape[0]) if uplo == LOWER else np.triu(F, 1) + np.eye(F.shape[0])
+    return T, np.diag(F).copy()
 
 
 def lu_factors(LU):

@@ -460,3 +460,86 @@ def test_oracle_trmm_vs_blas(side, uplo, trans, diag):
     # TRMM3 with beta = 0 and op(B) = B is the same product
     C = O.trmm3(side, uplo, trans, diag, 0, 1.2, A, B0, 0.0, np.full((m, n), np.nan))
     np.testing.assert_allclose(C, ref, rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------------------- LDL^T restatement ----
+# Pinned by: a hand-computed 3 x 3 factor; the identity L sqrt(D) = chol(A) for SPD A
+# (numpy's LAPACK Cholesky); reconstruction L D L^T = A for a symmetric indefinite matrix;
+# the blocked path (n > 128) against the pure-loop unblocked path on the same matrix; the
+# zero-pivot index of a designed matrix; and solves against numpy.linalg.solve.
+
+def _sym_indefinite(n, seed):
+    rng = np.random.default_rng(seed)
+    M = rng.uniform(-1, 1, (n, n))
+    S = (M + M.T) / 2
+    # diagonally dominant with mixed signs: every leading minor is far from zero
+    S[np.diag_indices(n)] = (n / 2 + rng.uniform(0, 1, n)) * np.where(np.arange(n) % 3 == 1, -1, 1)
+    return S
+
+
+def test_ldlt_oracle_hand_computed():
+    A = np.array([[4.0, 2.0, -2.0], [2.0, 5.0, 1.0], [-2.0, 1.0, 3.0]])
+    # d1 = 4, l21 = 0.5, l31 = -0.5; d2 = 5 - 4 (0.25) = 4, l32 = (1 - 4 (0.5)(-0.5)) / 4 = 0.5;
+    # d3 = 3 - 4 (0.25) - 4 (0.25) = 1
+    F = A.copy()
+    assert O.ldlt(F) == 0
+    L, d = O.ldlt_factors(F)
+    np.testing.assert_array_equal(d, [4.0, 4.0, 1.0])
+    np.testing.assert_array_equal(L, [[1, 0, 0], [0.5, 1, 0], [-0.5, 0.5, 1]])
+    np.testing.assert_array_equal(np.triu(F, 1), np.triu(A, 1))  # other triangle untouched
+    Fu = A.copy()
+    assert O.ldlt(Fu, O.UPPER) == 0
+    U, du = O.ldlt_factors(Fu, O.UPPER)
+    np.testing.assert_array_equal(du, d)
+    np.testing.assert_array_equal(U, L.T)
+
+
+@pytest.mark.parametrize("n", [7, 128, 300])
+def test_ldlt_oracle_matches_cholesky_on_spd(n):
+    rng = np.random.default_rng(n)
+    M = rng.uniform(-1, 1, (n, n))
+    A = M @ M.T + n * np.eye(n)
+    for uplo in (O.LOWER, O.UPPER):
+        F = A.copy()
+        assert O.ldlt(F, uplo) == 0
+        T, d = O.ldlt_factors(F, uplo)
+        Lc = np.linalg.cholesky(A)
+        G = T * np.sqrt(d)[None, :] if uplo == O.LOWER else (np.sqrt(d)[:, None] * T).T
+        assert np.max(np.abs(G - Lc)) < 1e-12 * np.max(np.abs(Lc))
+
+
+@pytest.mark.parametrize("n", [5, 130, 300])
+def test_ldlt_oracle_reconstructs_indefinite_and_blocked_equals_unblocked(n):
+    A = _sym_indefinite(n, 1000 + n)
+    F = A.copy()
+    assert O.ldlt(F) == 0
+    L, d = O.ldlt_factors(F)
+    assert np.linalg.norm(L @ np.diag(d) @ L.T - A) < 1e-13 * n * np.linalg.norm(A)
+    assert (d < 0).any() and (d > 0).any()
+    Fu = A.copy()
+    assert O.ldlt(Fu, block=10 ** 6) == 0  # one unblocked pass over the whole matrix
+    assert np.max(np.abs(np.tril(Fu) - np.tril(F))) < 1e-12
+
+
+def test_ldlt_oracle_zero_pivot_index():
+    A = _sym_indefinite(10, 5)
+    # row and column 3 decoupled from the first three pivots and a zero diagonal: the first
+    # three steps leave a_33 = 0, so d_3 = 0 exactly
+    A[3, :4] = 0.0
+    A[:4, 3] = 0.0
+    assert O.ldlt(A.copy()) == 4
+    # the last pivot of a block is not tested (LDLT.cs:145 stops at Rows - 1): no status
+    B = _sym_indefinite(6, 9)
+    B[5, :] = 0.0
+    B[:, 5] = 0.0
+    assert O.ldlt(B.copy()) == 0
+
+
+def test_ldlt_oracle_solve():
+    A = _sym_indefinite(150, 7)
+    B = np.random.default_rng(8).uniform(-1, 1, (150, 3))
+    for uplo in (O.LOWER, O.UPPER):
+        F = A.copy()
+        assert O.ldlt(F, uplo) == 0
+        X = O.ldlt_solve(F, B.copy(), uplo)
+        np.testing.assert_allclose(X, np.linalg.solve(A, B), rtol=1e-10, atol=1e-12)
