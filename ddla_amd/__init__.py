@@ -25,7 +25,8 @@ __all__ = [
     "synchronize", "NO_TRANSPOSE", "TRANSPOSE", "LOWER", "UPPER", "UNIT_DIAG", "NONUNIT_DIAG",
     "LEFT", "RIGHT", "PREC_FP64", "PREC_FP32", "PREC_FP16", "PREC_BF16", "DDLAError",
     "BatchPlan", "batch_factor", "BATCH_LU", "BATCH_CHOL", "sgetrf_mp", "PIVOT_PARTIAL",
-    "PIVOT_BUTTERFLY", "PIVOT_TOURNAMENT", "LDLT", "LDLTDecompose", "LDLTSolve",
+    "PIVOT_BUTTERFLY", "PIVOT_TOURNAMENT", "LDLT", "LDLTDecompose", "LDLTSolve", "QR",
+    "QRDecompose", "QRApply", "QRSolve", "qr_t_rows",
 ]
 
 
@@ -592,6 +593,109 @@ class LDLT:
         np.fill_diagonal(L, 1.0)
         self._deconstructed = True
         return L, D
+
+
+def qr_t_rows(m, n):
+    """Rows of the T matrix of a QR (QR.CreateT, QR.cs:124-128): min(256, max(m, n))."""
+    return int(_lib().ddla_qr_t_rows(m, n))
+
+
+def QRDecompose(A, T=None):
+    """Blocked Householder QR in the UT form, in place (QR.cs:130-149): A gets R and the
+    Householder vectors; returns T (qr_t_rows(m, n) x n, the block factors).  A and T are
+    numpy arrays (any strides) or DeviceMatrix objects (T required then)."""
+    if isinstance(A, DeviceMatrix):
+        if T is None or T.rows < qr_t_rows(A.rows, A.cols) or T.cols != A.cols:
+            raise ValueError("Matrix T and A must have the same number of columns.")
+        check(_lib().ddla_dgeqrf_dev(A.rows, A.cols, A.ptr, A.ld, T.ptr, T.ld))
+        return T
+    A = _as_f64(A)
+    m, n = A.shape
+    if T is None:
+        T = np.zeros((qr_t_rows(m, n), n))
+    if T.shape != (qr_t_rows(m, n), n):
+        raise ValueError("Matrix T and A must have the same number of columns.")
+    check(_lib().ddla_dgeqrf(m, n, _ptr(A), *_strides(A), _ptr(T), *_strides(T)))
+    return T
+
+
+def QRApply(A, T, B, trans):
+    """B <- Q^T B (trans, ApplyQlhfc) or Q B (ApplyQlnfc) on DeviceMatrix operands."""
+    return check(_lib().ddla_dqr_apply_dev(TRANSPOSE if trans else NO_TRANSPOSE, A.rows, A.cols,
+                                           A.ptr, A.ld, T.ptr, T.ld, B.cols, B.ptr, B.ld))
+
+
+def QRSolve(A, T, B, X=None):
+    """Least squares X = argmin |A X - B| (QR.Solve, QR.cs:151-167); B unchanged."""
+    if isinstance(A, DeviceMatrix):
+        check(_lib().ddla_dqr_solve_dev(A.rows, A.cols, B.cols, A.ptr, A.ld, T.ptr, T.ld, B.ptr,
+                                        B.ld, X.ptr, X.ld))
+        return X
+    A, T, B = _as_f64(A), _as_f64(T), _as_f64(B)
+    B2 = B.reshape(-1, 1) if B.ndim == 1 else B
+    m, n = A.shape
+    if B2.shape[0] != m:
+        raise ValueError("Matrix A and B must have the same number of rows.")
+    X2 = np.zeros((n, B2.shape[1])) if X is None else X.reshape(n, -1)
+    check(_lib().ddla_dqr_solve(m, n, B2.shape[1], _ptr(A), *_strides(A), _ptr(T), *_strides(T),
+                                _ptr(B2), *_strides(B2), _ptr(X2), *_strides(X2)))
+    return X2.reshape(-1) if B.ndim == 1 and X is None else X2
+
+
+class QR:
+    """Householder QR (QR.cs:14-122); computed once, lazily, on the GPU."""
+
+    def __init__(self, A, inplace=False):
+        A = _as_f64(A)
+        self.matrix = A if inplace else np.array(A, order="C", copy=True)
+        m, n = self.matrix.shape
+        self.aux = np.zeros((qr_t_rows(m, n), n))
+        self._computed = False
+        self._deconstructed = False
+
+    def ComputeOnce(self):
+        if self._deconstructed:
+            raise RuntimeError("Matrix has been deconstructed, cannot compute again.")
+        if not self._computed:
+            self._computed = True
+            QRDecompose(self.matrix, self.aux)
+
+    def Solve(self, B):
+        self.ComputeOnce()
+        return QRSolve(self.matrix, self.aux, B)
+
+    def _apply(self, B, trans):
+        B = _as_f64(B)
+        B2 = B.reshape(-1, 1) if B.ndim == 1 else B
+        m = self.matrix.shape[0]
+        if B2.shape[0] != m:
+            raise ValueError("B must have as many rows as A")
+        dA = DeviceMatrix.from_host(self.matrix)
+        dT = DeviceMatrix.from_host(self.aux)
+        dB = DeviceMatrix.from_host(np.ascontiguousarray(B2))
+        QRApply(dA, dT, dB, trans)
+        B2[...] = dB.to_host()
+        return B
+
+    def ApplyQ(self, B):
+        """B <- Q B in place (QR.cs:78-90)."""
+        self.ComputeOnce()
+        return self._apply(B, False)
+
+    def ApplyQT(self, B):
+        """B <- Q^T B in place (QR.cs:92-104)."""
+        self.ComputeOnce()
+        return self._apply(B, True)
+
+    def Deconstruct(self):
+        """(Q, R): Q = H_0 ... H_{k-1} applied to the m x m identity, R the upper triangle
+        (QR.cs:106-118)."""
+        self.ComputeOnce()
+        m = self.matrix.shape[0]
+        Q = self._apply(np.eye(m), False)
+        R = np.triu(self.matrix)
+        self._deconstructed = True
+        return Q, R
 
 
 PIVOT_PARTIAL, PIVOT_BUTTERFLY, PIVOT_TOURNAMENT = 0, 1, 2

@@ -601,6 +601,76 @@ int ddla_dldlt_solve(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, d
   return download_view(n, nrhs, dB, b, rsb, csb);
 }
 
+int ddla_qr_t_rows(ddla_dim_t m, ddla_dim_t n) {
+  if (m < 0 || n < 0) return 0;
+  return qr_t_rows(m, n);
+}
+
+int ddla_dgeqrf_dev(ddla_dim_t m, ddla_dim_t n, double* a, ddla_inc_t lda, double* t,
+                    ddla_inc_t ldt) {
+  API_INIT();
+  DDLA_ARG_CHECK(m >= 0 && n >= 0 && lda >= n && ldt >= n, "bad geqrf arguments");
+  if (m == 0 || n == 0) return 0;
+  DDLA_ARG_CHECK(a && t, "null matrix pointer");
+  return dgeqrf_internal(m, n, a, lda, t, ldt);
+}
+
+int ddla_dqr_apply_dev(int trans, ddla_dim_t m, ddla_dim_t n, const double* a, ddla_inc_t lda,
+                       const double* t, ddla_inc_t ldt, ddla_dim_t nrhs, double* b,
+                       ddla_inc_t ldb) {
+  API_INIT();
+  DDLA_ARG_CHECK(valid_trans(trans), "bad trans");
+  DDLA_ARG_CHECK(m >= 0 && n >= 0 && nrhs >= 0 && lda >= n && ldt >= n && ldb >= nrhs,
+                 "bad QR apply arguments");
+  if (m == 0 || n == 0 || nrhs == 0) return 0;
+  DDLA_ARG_CHECK(a && t && b, "null matrix pointer");
+  return dqr_apply_internal(is_trans(trans), m, n, a, lda, t, ldt, nrhs, b, ldb);
+}
+
+int ddla_dqr_solve_dev(ddla_dim_t m, ddla_dim_t n, ddla_dim_t nrhs, const double* a,
+                       ddla_inc_t lda, const double* t, ddla_inc_t ldt, const double* b,
+                       ddla_inc_t ldb, double* x, ddla_inc_t ldx) {
+  API_INIT();
+  DDLA_ARG_CHECK(m >= n, "QR solve needs m >= n (least squares)");
+  DDLA_ARG_CHECK(n >= 0 && nrhs >= 0 && lda >= n && ldt >= n && ldb >= nrhs && ldx >= nrhs,
+                 "bad QR solve arguments");
+  if (n == 0 || nrhs == 0) return 0;
+  DDLA_ARG_CHECK(a && t && b && x, "null matrix pointer");
+  return dqr_solve_internal(m, n, nrhs, a, lda, t, ldt, b, ldb, x, ldx);
+}
+
+int ddla_dgeqrf(ddla_dim_t m, ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t csa,
+                double* t, ddla_inc_t rst, ddla_inc_t cst) {
+  API_INIT();
+  DDLA_ARG_CHECK(m >= 0 && n >= 0, "negative size");
+  if (m == 0 || n == 0) return 0;
+  const int64_t tr = qr_t_rows(m, n);
+  DevBuf dA, dT;
+  DDLA_RETURN_IF(upload_view(m, n, a, rsa, csa, dA));
+  DDLA_RETURN_IF(upload_view(tr, n, t, rst, cst, dT));
+  DDLA_RETURN_IF(dgeqrf_internal(m, n, dA.d(), n, dT.d(), n));
+  DDLA_RETURN_IF(download_view(m, n, dA, a, rsa, csa));
+  return download_view(tr, n, dT, t, rst, cst);
+}
+
+int ddla_dqr_solve(ddla_dim_t m, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
+                   ddla_inc_t csa, const double* t, ddla_inc_t rst, ddla_inc_t cst,
+                   const double* b, ddla_inc_t rsb, ddla_inc_t csb, double* x, ddla_inc_t rsx,
+                   ddla_inc_t csx) {
+  API_INIT();
+  DDLA_ARG_CHECK(m >= n, "QR solve needs m >= n (least squares)");
+  DDLA_ARG_CHECK(n >= 0 && nrhs >= 0, "negative size");
+  if (n == 0 || nrhs == 0) return 0;
+  const int64_t tr = qr_t_rows(m, n);
+  DevBuf dA, dT, dB, dX;
+  DDLA_RETURN_IF(upload_view(m, n, a, rsa, csa, dA));
+  DDLA_RETURN_IF(upload_view(tr, n, t, rst, cst, dT));
+  DDLA_RETURN_IF(upload_view(m, nrhs, b, rsb, csb, dB));
+  DDLA_RETURN_IF(dX.alloc((size_t)n * nrhs * sizeof(double)));
+  DDLA_RETURN_IF(dqr_solve_internal(m, n, nrhs, dA.d(), n, dT.d(), n, dB.d(), nrhs, dX.d(), nrhs));
+  return download_view(n, nrhs, dX, x, rsx, csx);
+}
+
 int ddla_dtrsm_dev(int side, int uplo, int transa, int diag, ddla_dim_t m, ddla_dim_t n,
                    const double* alpha, const double* a, ddla_inc_t lda, double* b,
                    ddla_inc_t ldb) {

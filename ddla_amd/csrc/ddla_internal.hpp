@@ -42,6 +42,29 @@ const char* last_error();
 // ---------------------------------------------------------------- context ----------------
 constexpr unsigned DEFAULT_WAIT_LIMIT = 1u << 24;
 
+enum WorkspaceSlot {
+  WS_PANEL = 0,   // panel-kernel exchange slots
+  WS_PIVOT = 1,   // pivots / permutations / info words
+  WS_TEMP = 2,    // general temporaries (packed operands, transposes)
+  WS_MIXED = 3,   // low-precision working copies
+  WS_MIXED2 = 4,
+  WS_REFINE = 5,  // refinement vectors
+  WS_GMRES = 6,
+  WS_SMALL = 7,   // small reduction outputs
+  WS_DINV = 8,    // inverted diagonal blocks for the FP64 solves
+  WS_SPLIT = 9,   // low-precision parts of the trailing-update operands
+  WS_SPLIT2 = 10, // ... for updates on the auxiliary stream
+  WS_TEMP2 = 11,  // row-interchange staging on the auxiliary stream
+  WS_TSLU = 12,   // tournament-pivoting tree (candidate sets, arrival words)
+  WS_SPLITP = 13, // split panel operand shared by the updates of one panel (two in flight)
+  WS_BLAS3 = 14,  // level-3 BLAS: the expanded triangular / symmetric operand
+  WS_BLAS3B = 15, // level-3 BLAS: the product before it is written back over B
+  WS_QR_SYNC = 16, // QR panel exchange area
+  WS_QR_WORK = 17, // QR: explicit Householder vectors of a block, products, tau
+  WS_QR_Y = 18,    // QR solve: Q^T B
+  WS_COUNT = 19,
+};
+
 struct Context {
   bool ready = false;
   int device = 0;
@@ -53,13 +76,13 @@ struct Context {
   hipStream_t aux_stream = nullptr;
   hipEvent_t ev[4] = {};
   // growable scratch buffers (device memory kept for the life of the context)
-  void* ws[16] = {};
-  size_t ws_bytes[16] = {};
+  void* ws[WS_COUNT] = {};
+  size_t ws_bytes[WS_COUNT] = {};
   // bytes of each slot cleared since it was allocated, and the tag sequence of the exchange
   // words kept there (kernels that wait on each other's words compare 32-bit tags; a cleared
   // slot holds no live tag)
-  size_t ws_cleared[16] = {};
-  unsigned ws_tag[16] = {};
+  size_t ws_cleared[WS_COUNT] = {};
+  unsigned ws_tag[WS_COUNT] = {};
   // poll rounds a cross-workgroup wait (LU panel, triangular solve) makes before it gives up
   unsigned wait_limit = DEFAULT_WAIT_LIMIT;
   // set once a caller changed wait_limit: from then on every panel launch first writes the
@@ -89,25 +112,7 @@ int next_tag(int slot, size_t bytes, unsigned step, hipStream_t s, unsigned* tag
 // at its next poll.  check_abort synchronises s, reads such a word and, when it was set, clears
 // it and returns DDLA_ERR_HIP naming `what`.
 int check_abort(unsigned* word, hipStream_t s, const char* what);
-enum WorkspaceSlot {
-  WS_PANEL = 0,   // panel-kernel exchange slots
-  WS_PIVOT = 1,   // pivots / permutations / info words
-  WS_TEMP = 2,    // general temporaries (packed operands, transposes)
-  WS_MIXED = 3,   // low-precision working copies
-  WS_MIXED2 = 4,
-  WS_REFINE = 5,  // refinement vectors
-  WS_GMRES = 6,
-  WS_SMALL = 7,   // small reduction outputs
-  WS_DINV = 8,    // inverted diagonal blocks for the FP64 solves
-  WS_SPLIT = 9,   // low-precision parts of the trailing-update operands
-  WS_SPLIT2 = 10, // ... for updates on the auxiliary stream
-  WS_TEMP2 = 11,  // row-interchange staging on the auxiliary stream
-  WS_TSLU = 12,   // tournament-pivoting tree (candidate sets, arrival words)
-  WS_SPLITP = 13, // split panel operand shared by the updates of one panel (two in flight)
-  WS_BLAS3 = 14,  // level-3 BLAS: the expanded triangular / symmetric operand
-  WS_BLAS3B = 15, // level-3 BLAS: the product before it is written back over B
-  WS_COUNT = 16,
-};
+
 
 // Optional per-kernel timing (ddla_profile_*): while enabled, a ProfScope around a launch
 // records HIP events on the launch stream and accumulates the kernel's time together with
@@ -330,6 +335,14 @@ int dgetrs_internal(bool trans, int64_t n, int64_t nrhs, const double* a, int64_
                     const int32_t* dpiv, double* b, int64_t ldb);
 int dpotrf_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host);
 int dldlt_internal(bool lower, int64_t n, double* a, int64_t lda, int32_t* info_host);
+// Householder QR in the UT form (qr.hip): T has qr_t_rows(m, n) rows and n columns
+int qr_t_rows(int64_t m, int64_t n);
+int dgeqrf_internal(int64_t m, int64_t n, double* a, int64_t lda, double* t, int64_t ldt);
+int dqr_apply_internal(bool trans, int64_t m, int64_t n, const double* a, int64_t lda,
+                       const double* t, int64_t ldt, int64_t nrhs, double* b, int64_t ldb);
+int dqr_solve_internal(int64_t m, int64_t n, int64_t nrhs, const double* a, int64_t lda,
+                       const double* t, int64_t ldt, const double* b, int64_t ldb, double* x,
+                       int64_t ldx);
 int dldlt_solve_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda,
                          double* b, int64_t ldb);
 int dpotrs_internal(bool lower, int64_t n, int64_t nrhs, const double* a, int64_t lda, double* b,

@@ -81,6 +81,12 @@ SIGNATURES = {
     "ddla_dldlt_solve_dev": [cint, i64, i64, vp, i64, vp, i64],
     "ddla_dldlt": [cint, i64, vp, i64, i64],
     "ddla_dldlt_solve": [cint, i64, i64, vp, i64, i64, vp, i64, i64],
+    "ddla_qr_t_rows": [i64, i64],
+    "ddla_dgeqrf_dev": [i64, i64, vp, i64, vp, i64],
+    "ddla_dqr_apply_dev": [cint, i64, i64, vp, i64, vp, i64, i64, vp, i64],
+    "ddla_dqr_solve_dev": [i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64],
+    "ddla_dgeqrf": [i64, i64, vp, i64, i64, vp, i64, i64],
+    "ddla_dqr_solve": [i64, i64, i64, vp, i64, i64, vp, i64, i64, vp, i64, i64, vp, i64, i64],
     "ddla_dtrsm_dev": [cint, cint, cint, cint, i64, i64, dp, vp, i64, vp, i64],
     "ddla_dsyrk_dev": [cint, cint, i64, i64, dp, vp, i64, dp, vp, i64],
     "ddla_damaxv_dev": [i64, vp, i64, ctypes.POINTER(i64)],

@@ -206,6 +206,35 @@ int ddla_dldlt(int uplo, ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t csa
 int ddla_dldlt_solve(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
                      ddla_inc_t csa, double* b, ddla_inc_t rsb, ddla_inc_t csb);
 
+/* ------------------------------------------------------------------------- QR ------- */
+/* Blocked Householder QR in the UT form (block 256).  T holds ddla_qr_t_rows(m, n) =
+ * min(256, max(m, n)) rows and n columns: block k (columns k .. k+b-1) keeps its b x b upper
+ * triangular factor in T[0:b, k:k+b] (tau_j on the diagonal, u_i^T u_j above); other entries of
+ * T are unspecified.  Replaces QR.CreateT / QR.QRDecompose (DDLA/Factorizations/QR.cs:124-149,
+ * :326-395): A overwritten with R on and above the diagonal and the Householder vectors u2
+ * below (implicit 1 on the diagonal; H = I - u u^T / tau, HouseHolder.cs:34-56).  The column
+ * norm is sqrt of the sum of squares (the reference uses BLIS's scaled norm); m <= 32768. */
+int ddla_qr_t_rows(ddla_dim_t m, ddla_dim_t n);
+int ddla_dgeqrf_dev(ddla_dim_t m, ddla_dim_t n, double* a, ddla_inc_t lda, double* t,
+                    ddla_inc_t ldt);
+/* B (m x nrhs) <- Q^T B (trans = DDLA_TRANSPOSE; QR.ApplyQlhfc, QR.cs:423-464) or Q B
+ * (DDLA_NO_TRANSPOSE; QR.ApplyQlnfc, :466-521), with the factor of ddla_dgeqrf_dev. */
+int ddla_dqr_apply_dev(int trans, ddla_dim_t m, ddla_dim_t n, const double* a, ddla_inc_t lda,
+                       const double* t, ddla_inc_t ldt, ddla_dim_t nrhs, double* b,
+                       ddla_inc_t ldb);
+/* Least squares X (n x nrhs) = argmin |A X - B| for m >= n: X = R^-1 (Q^T B)[0:n]; B is left
+ * unchanged (QR.Solve, QR.cs:151-167, :397-409). */
+int ddla_dqr_solve_dev(ddla_dim_t m, ddla_dim_t n, ddla_dim_t nrhs, const double* a,
+                       ddla_inc_t lda, const double* t, ddla_inc_t ldt, const double* b,
+                       ddla_inc_t ldb, double* x, ddla_inc_t ldx);
+/* Host forms with (row stride, column stride) views. */
+int ddla_dgeqrf(ddla_dim_t m, ddla_dim_t n, double* a, ddla_inc_t rsa, ddla_inc_t csa,
+                double* t, ddla_inc_t rst, ddla_inc_t cst);
+int ddla_dqr_solve(ddla_dim_t m, ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t rsa,
+                   ddla_inc_t csa, const double* t, ddla_inc_t rst, ddla_inc_t cst,
+                   const double* b, ddla_inc_t rsb, ddla_inc_t csb, double* x, ddla_inc_t rsx,
+                   ddla_inc_t csx);
+
 /* ------------------------------------------------------------ level-3 pieces ------- */
 /* B = alpha * op(A)^-1 B (left) or B op(A)^-1 (right), row-major device matrices with leading
  * dimensions: the device-resident form of bli_dtrsm (DDLA/BLAS/Native/3m.cs:151-162) for the

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: a CPU restatement of DDLA's managed FP64 GEMM, blocked LU with
-partial pivoting, blocked Cholesky and (in numpy) blocked LDL^T, used to check the HIP library and as the timed CPU
+partial pivoting, blocked Cholesky and (in numpy) blocked LDL^T and Householder QR, used to check the HIP library and as the timed CPU
 baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import it.
 Every function mirrors the reference routine named in its docstring (paths relative to
 the DDLA repository root); the C source cites the same lines.
@@ -335,6 +335,112 @@ def ldlt_factors(F, uplo=LOWER):
     (LDLT.Deconstruct, LDLT.cs:128-137)."""
     T = np.tril(F, -1) + np.eye(F.shape[0]) if uplo == LOWER else np.triu(F, 1) + np.eye(F.shape[0])
     return T, np.diag(F).copy()
+
+
+QR_BLOCK = 256  # QR.BlockSize (DDLA/Factorizations/QR.cs:16)
+
+
+def qr_t_rows(m, n):
+    """Rows of the T matrix QR.CreateT allocates (QR.cs:124-128): min(256, max(m, n))."""
+    return min(QR_BLOCK, max(m, n))
+
+
+def _qr_unblocked(A, T):
+    """QRDecUnblock (QR.cs:326-363) on the view A (m x n), T (>= min(m, n) rows): per column
+    j, BuildHouseHolder (Transformations/HouseHolder.cs:34-56: rho = -copysign(hypot(chi,
+    |x2|), chi), u2 = x2 * (1/(chi - rho)), tau = (1 + |u2|^2 / (chi - rho)^2) / 2; tau = 1/2
+    and chi <- -chi when x2 = 0), ApplyHouseHolder from the left (:95-111: w = (a12 + A22^T
+    u2) * (1/tau), a12 -= w, A22 -= u2 w^T), T[j, j] = tau and T[:j, j] = a10 + A20^T u2.
+    The norm is sqrt of the sum of squares (the reference calls BLIS's scaled norm)."""
+    m, n = A.shape
+    for j in range(min(m, n)):
+        chi = A[j, j]
+        x2 = A[j + 1:, j]
+        len2 = float(np.sqrt(np.dot(x2, x2)))
+        if len2 == 0.0:
+            A[j, j] = -chi
+            tau = 0.5
+        else:
+            rho = -np.copysign(np.hypot(chi, len2), chi)
+            miu = chi - rho
+            A[j, j] = rho
+            x2 *= 1.0 / miu
+            s = len2 / abs(miu)
+            tau = (1.0 + s * s) / 2.0
+        u2 = A[j + 1:, j]
+        if j + 1 < n:
+            w = A[j, j + 1:] + u2 @ A[j + 1:, j + 1:]
+            w *= 1.0 / tau
+            A[j, j + 1:] -= w
+            A[j + 1:, j + 1:] -= np.outer(u2, w)
+        T[j, j] = tau
+        T[:j, j] = A[j, :j] + A[j + 1:, :j].T @ u2
+    return A, T
+
+
+def _unit_lower(P):
+    """The Householder vectors of a factored panel P (m x b): unit diagonal, zeros above."""
+    U = np.tril(P, -1)
+    k = min(P.shape)
+    U[np.arange(k), np.arange(k)] = 1.0
+    return U
+
+
+def _qr_blocks(m, n, tb):
+    k, out = 0, []
+    while k < min(m, n):
+        b = min(tb, min(m, n) - k)
+        out.append((k, b))
+        k += b
+    return out
+
+
+def qr(A, T=None):
+    """QRDecompose (QR.cs:130-149) in place: unblocked when n <= 256, else QRDecBlock
+    (:365-395): per block of b = min(T rows, remaining) columns the unblocked factor of the
+    panel A[k:, k:k+b] (its T factor in T[:b, k:k+b]), then ApplyQlhfc of the panel to
+    A[k:, k+b:].  Returns T (qr_t_rows(m, n) x n; entries outside the factors' upper
+    triangles are unspecified)."""
+    m, n = A.shape
+    tb = qr_t_rows(m, n)
+    if T is None:
+        T = np.zeros((tb, n))
+    if n <= QR_BLOCK:
+        _qr_unblocked(A, T)
+        return T
+    for k, b in _qr_blocks(m, n, tb):
+        _qr_unblocked(A[k:, k:k + b], T[:b, k:k + b])
+        if k + b < n:
+            _apply_block(A[k:, k:k + b], T[:b, k:k + b], A[k:, k + b:], trans=True)
+    return T
+
+
+def _apply_block(P, Tb, B, trans):
+    """B <- (I - U Tb^-1 U^T)^T B (trans) or (I - U Tb^-1 U^T) B, U the unit lower part of the
+    panel P (ApplyQlhfc / ApplyQlnfc's step, QR.cs:444-463, :505-519)."""
+    import scipy.linalg as sla
+    U = _unit_lower(P)
+    W = U.T @ B
+    W = sla.solve_triangular(np.triu(Tb), W, lower=False, trans=1 if trans else 0)
+    B -= U @ W
+
+
+def qr_apply(A, T, B, trans):
+    """ApplyQlhfc (trans: B <- Q^T B, QR.cs:423-464) or ApplyQlnfc (B <- Q B, :466-521) with
+    the blocks of the factorization (forward for Q^T, backward for Q)."""
+    m, n = A.shape
+    blocks = _qr_blocks(m, n, T.shape[0])
+    for k, b in (blocks if trans else blocks[::-1]):
+        _apply_block(A[k:, k:k + b], T[:b, k:k + b], B[k:], trans)
+    return B
+
+
+def qr_solve(A, T, B):
+    """SolveUncheck (QR.cs:397-409): Y = Q^T B, X = R^-1 Y[:n] (least squares, m >= n)."""
+    import scipy.linalg as sla
+    n = A.shape[1]
+    Y = qr_apply(A, T, np.array(B, dtype=np.float64, copy=True), True)
+    return sla.solve_triangular(np.triu(A[:n, :n]), Y[:n], lower=False)
 
 
 def lu_factors(LU):

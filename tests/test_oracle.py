@@ -543,3 +543,57 @@ def test_ldlt_oracle_solve():
         assert O.ldlt(F, uplo) == 0
         X = O.ldlt_solve(F, B.copy(), uplo)
         np.testing.assert_allclose(X, np.linalg.solve(A, B), rtol=1e-10, atol=1e-12)
+
+
+# ------------------------------------------------------------------ QR restatement ----
+# Pinned by LAPACK's dgeqrf (scipy mode='raw'), which uses the same Householder convention
+# (beta = -sign(alpha) |x|, v = x / (alpha - beta), v_1 = 1): R and the vectors agree, and the
+# reference's tau is LAPACK's 1/tau.  Where a column has no entries below the diagonal
+# (the last one when m <= n) LAPACK leaves H = I (tau = 0) while the reference reflects
+# (tau = 1/2, the row of R negated).  T is pinned by its definition (UT transform:
+# T = strictly-upper(U^T U) + diag(tau)); Q by orthogonality and A = Q R; the solve by
+# numpy's least squares.
+
+@pytest.mark.parametrize("m,n", [(5, 3), (300, 300), (700, 520), (100, 300), (3, 5), (600, 257)])
+def test_qr_oracle_matches_lapack(m, n):
+    A0 = np.random.default_rng(m * 7 + n).uniform(-1, 1, (m, n))
+    A = A0.copy()
+    T = O.qr(A)
+    assert T.shape == (min(256, max(m, n)), n)
+    (h, tau), _ = sla.qr(A0, mode="raw")
+    k = min(m, n)
+    full = m > n  # every column has entries below its diagonal
+    rows = m if full else m - 1
+    assert np.max(np.abs(np.triu(A)[:rows] - np.triu(h)[:rows])) < 1e-13
+    if not full:
+        np.testing.assert_allclose(A[m - 1, m - 1:], -h[m - 1, m - 1:], rtol=0, atol=1e-13)
+    assert np.max(np.abs(np.tril(A, -1)[:, :k] - np.tril(h, -1)[:, :k])) < 1e-13
+    taus = np.array([T[j % 256, j] for j in range(k)])
+    kk = k if full else k - 1
+    np.testing.assert_allclose(taus[:kk], 1.0 / tau[:kk], rtol=1e-13)
+    if not full:
+        assert taus[k - 1] == 0.5
+    # T: strictly upper part of U^T U per block, tau on the diagonal
+    for k0 in range(0, k, 256 if n > 256 else max(k, 1)):
+        b = min(256 if n > 256 else k, k - k0)
+        U = O._unit_lower(A[k0:, k0:k0 + b])
+        G = U.T @ U
+        Tb = T[:b, k0:k0 + b]
+        assert np.max(np.abs(np.triu(Tb, 1) - np.triu(G, 1))) < 1e-12 * max(1.0, np.max(np.abs(G)))
+    Q = O.qr_apply(A, T, np.eye(m), False)
+    assert np.max(np.abs(Q.T @ Q - np.eye(m))) < 1e-13
+    assert np.max(np.abs(Q @ np.triu(A) - A0)) < 1e-13
+    Qt = O.qr_apply(A, T, np.eye(m), True)
+    assert np.max(np.abs(Qt - Q.T)) < 1e-13
+
+
+@pytest.mark.parametrize("m,n,nrhs", [(40, 40, 1), (500, 300, 3), (800, 600, 2)])
+def test_qr_oracle_least_squares(m, n, nrhs):
+    rng = np.random.default_rng(m + n)
+    A0 = rng.uniform(-1, 1, (m, n))
+    B = rng.uniform(-1, 1, (m, nrhs))
+    A = A0.copy()
+    T = O.qr(A)
+    X = O.qr_solve(A, T, B)
+    Xl = np.linalg.lstsq(A0, B, rcond=None)[0]
+    np.testing.assert_allclose(X, Xl, rtol=1e-9, atol=1e-11)
