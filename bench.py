@@ -345,6 +345,65 @@ def bench_c2(ddla, n=8192, reps=2):
             "kernels": kern}, piv
 
 
+def bench_ldlt_qr(ddla, n=8192, reps=2, cpu_n=None):
+    """LDL^T (no pivoting) of a symmetric positive definite A = R R^T / n + I and Householder
+    QR of a uniform[-1,1] n x n matrix on the GPU (median of `reps` after one warm-up), with
+    the panel kernels' share; optionally the oracle's numpy restatements at cpu_n on the host."""
+    lib = ddla.native.lib()
+    R = ddla.DeviceMatrix.random(n, n, 0xDD1A0010)
+    S = ddla.DeviceMatrix.from_host(np.eye(n))
+    ddla.BlasProvider.GeMM(1.0 / n, R, R, 1.0, S, transB=ddla.TRANSPOSE)
+    W = ddla.DeviceMatrix(n, n)
+    T = ddla.DeviceMatrix(ddla.qr_t_rows(n, n), n)
+    out = {}
+    for name, src, flops, kern in (
+            ("ldlt", S, n ** 3 / 3.0, "ldlt_diag_f64"),
+            ("qr", R, 4.0 * n ** 3 / 3.0, "qr_panel_f64")):
+        ts, prof = [], {}
+        for k in range(reps + 1):
+            ddla.native.check(lib.ddla_memcpy_d2d(W.ptr, src.ptr, n * n * 8))
+            ddla.synchronize()
+            if k == reps:
+                ddla.profile_reset()
+                ddla.profile_enable(True)
+            t0 = time.perf_counter()
+            if name == "ldlt":
+                ddla.LDLTDecompose(ddla.LOWER, W)
+            else:
+                ddla.QRDecompose(W, T)
+            ddla.synchronize()
+            if k >= 1:
+                ts.append(time.perf_counter() - t0)
+            if k == reps:
+                ddla.profile_enable(False)
+                prof = ddla.profile_read()
+        dt = float(np.median(ts[:-1] if len(ts) > 1 else ts))
+        e = prof.get(kern, {"ms": 0.0, "count": 0})
+        g = prof.get("dgemm_mfma_f64", {"ms": 0.0, "flops": 0.0})
+        out[name] = {"n": n, "ms": dt * 1e3, "gflops": flops / dt / 1e9,
+                     "panel_kernel": kern, "panel_ms": e["ms"], "panel_launches": e["count"],
+                     "gemm_ms": g["ms"],
+                     "gemm_tflops": g["flops"] / (g["ms"] / 1e3) / 1e12 if g["ms"] else None}
+    if cpu_n:
+        from oracle import oracle as O
+        rng = np.random.default_rng(0xDD1A0011)
+        M = rng.uniform(-1, 1, (cpu_n, cpu_n))
+        Sc = M @ M.T / cpu_n + np.eye(cpu_n)
+        t0 = time.perf_counter()
+        O.ldlt(Sc)
+        t_l = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.qr(M)
+        t_q = time.perf_counter() - t0
+        out["cpu_oracle"] = {"n": cpu_n, "ldlt_ms": t_l * 1e3, "qr_ms": t_q * 1e3,
+                             "ldlt_gflops": cpu_n ** 3 / 3.0 / t_l / 1e9,
+                             "qr_gflops": 4.0 * cpu_n ** 3 / 3.0 / t_q / 1e9,
+                             "kind": "port",
+                             "note": "numpy restatements (column loops in Python, BLAS-3 "
+                                     "steps in numpy's BLAS) of LDLT.cs / QR.cs"}
+    return out
+
+
 def timed(fn):
     fn()
     import ddla_amd as ddla
@@ -673,6 +732,9 @@ def main():
     # ---- the plain FP64 LU + solve configuration (n = 8192) on the GPU
     c2, piv_c2 = bench_c2(ddla, 8192)
     log("fp64 LU n=8192", c2)
+    ldlt_qr = None if args.no_extras else bench_ldlt_qr(ddla, 8192,
+                                                       cpu_n=None if args.no_cpu else 1024)
+    log("ldlt / qr", ldlt_qr)
 
     cpu = None
     cpu_c1 = None
@@ -722,6 +784,7 @@ def main():
         "gemm_fp64": gemm,
         "gemm_fp64_cpu_oracle": cpu_c1,
         "fp64_lu_n8192": c2,
+        "ldlt_qr_n8192": ldlt_qr,
         "kernel_breakdown": breakdown,
         "kernel_breakdown_fp64_lu": breakdown64,
         "batched": batched,

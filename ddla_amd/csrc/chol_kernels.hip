@@ -397,7 +397,9 @@ __global__ __launch_bounds__(256) void ldlt_diag_kernel(int nb, double* __restri
     const double l0 = h0 ? colv[par][c0] * rinv : 0.0;
     const double l1 = h1 ? colv[par][c1] * rinv : 0.0;
     constexpr int CH = 8;
-    for (int k0 = 0; k0 < NB / 4; k0 += CH) {
+    // rows wave + 4k > i start at k = (i - wave) / 4 + 1; chunks below hold no such row
+    const int kmin = i >= wave ? (i - wave) / 4 + 1 : 0;
+    for (int k0 = kmin / CH * CH; k0 < NB / 4; k0 += CH) {
       double a0[CH], a1[CH], x[CH];
 #pragma unroll
       for (int k = 0; k < CH; ++k) {

@@ -62,7 +62,8 @@ enum WorkspaceSlot {
   WS_QR_SYNC = 16, // QR panel exchange area
   WS_QR_WORK = 17, // QR: explicit Householder vectors of a block, products, tau
   WS_QR_Y = 18,    // QR solve: Q^T B
-  WS_COUNT = 19,
+  WS_SPLITK = 19,  // split-K GEMM partial products (main stream only)
+  WS_COUNT = 20,
 };
 
 struct Context {

@@ -176,11 +176,21 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, typen
                                                       int64_t ldb, typename C::E beta,
                                                       typename C::E* __restrict__ Cm, int64_t ldc,
                                                       int tiles_m, int tiles_n, bool veca,
-                                                      bool vecb) {
+                                                      bool vecb, int kchunk, int64_t a_zstep,
+                                                      int64_t b_zstep, int64_t c_zstep) {
   typedef typename C::E E;
   typedef typename C::v4 v4;
   __shared__ __attribute__((aligned(16))) E As[2][C::AS];
   __shared__ __attribute__((aligned(16))) E Bs[2][C::BS];
+  // split K: workgroup row z of the grid takes K range [z kchunk, (z + 1) kchunk) and writes
+  // its own partial product (the operands and C advanced by z steps)
+  {
+    const int z = blockIdx.y;
+    K = min(kchunk, K - z * kchunk);
+    A += z * a_zstep;
+    B += z * b_zstep;
+    Cm += z * c_zstep;
+  }
 
   int tmi, tni;
   gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
@@ -297,7 +307,11 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, typen
 template <class C, bool AK, bool BNC>
 int launch_cfg(int M, int N, int K, typename C::E alpha, const typename C::E* A, int64_t lda,
                const typename C::E* B, int64_t ldb, typename C::E beta, typename C::E* Cm,
-               int64_t ldc, int tri, hipStream_t s) {
+               int64_t ldc, int tri, hipStream_t s, int splits = 1, int kchunk = 0) {
+  if (splits <= 1) kchunk = K;
+  const int64_t az = AK ? (int64_t)kchunk : (int64_t)kchunk * lda;
+  const int64_t bz = BNC ? (int64_t)kchunk * ldb : (int64_t)kchunk;
+  const int64_t cz = splits > 1 ? (int64_t)M * ldc : 0;
   const int tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
   const int64_t tiles = (int64_t)tm * tn;
   if (tiles > INT32_MAX) {
@@ -306,10 +320,10 @@ int launch_cfg(int M, int N, int K, typename C::E alpha, const typename C::E* A,
   }
   const bool veca = (reinterpret_cast<uintptr_t>(A) % (2 * sizeof(typename C::E)) == 0) && (lda % 2 == 0);
   const bool vecb = (reinterpret_cast<uintptr_t>(B) % (2 * sizeof(typename C::E)) == 0) && (ldb % 2 == 0);
-  dim3 grid(static_cast<unsigned>(tiles)), block(256);
+  dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(splits)), block(256);
 #define DDLA_GEMM_GO(B0, T)                                                                 \
   hipLaunchKernelGGL((gemm_kernel<C, AK, BNC, B0, T>), grid, block, 0, s, M, N, K, alpha, A, \
-                     lda, B, ldb, beta, Cm, ldc, tm, tn, veca, vecb)
+                     lda, B, ldb, beta, Cm, ldc, tm, tn, veca, vecb, kchunk, az, bz, cz)
   const bool b0 = (beta == typename C::E(0));
   if (tri == 0) {
     if (b0) DDLA_GEMM_GO(true, 0); else DDLA_GEMM_GO(false, 0);
@@ -326,11 +340,38 @@ int launch_cfg(int M, int N, int K, typename C::E alpha, const typename C::E* A,
 template <class C>
 int launch_layout(int M, int N, int K, typename C::E alpha, const typename C::E* A, int64_t lda,
                   bool ak, const typename C::E* B, int64_t ldb, bool bn, typename C::E beta,
-                  typename C::E* Cm, int64_t ldc, int tri, hipStream_t s) {
-  if (ak && bn) return launch_cfg<C, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s);
-  if (ak && !bn) return launch_cfg<C, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s);
-  if (!ak && bn) return launch_cfg<C, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s);
-  return launch_cfg<C, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s);
+                  typename C::E* Cm, int64_t ldc, int tri, hipStream_t s, int splits = 1,
+                  int kchunk = 0) {
+  if (ak && bn)
+    return launch_cfg<C, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s,
+                                     splits, kchunk);
+  if (ak && !bn)
+    return launch_cfg<C, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s,
+                                      splits, kchunk);
+  if (!ak && bn)
+    return launch_cfg<C, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s,
+                                      splits, kchunk);
+  return launch_cfg<C, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, tri, s,
+                                     splits, kchunk);
+}
+
+// C = alpha sum_z P[z] + beta C over the partial products of a split-K GEMM (z in order, so
+// the result does not depend on the schedule); tri: 0 all, 1 lower, 2 upper triangle of C
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int S,
+                                                            const T* __restrict__ P, T alpha,
+                                                            T beta, T* __restrict__ Cm,
+                                                            int64_t ldc, int tri) {
+  const int64_t mn = (int64_t)M * N;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < mn;
+       e += (int64_t)gridDim.x * 256) {
+    const int i = static_cast<int>(e / N), j = static_cast<int>(e % N);
+    if ((tri == 1 && j > i) || (tri == 2 && j < i)) continue;
+    T acc = P[e];
+    for (int z = 1; z < S; ++z) acc += P[z * mn + e];
+    T* c = Cm + (int64_t)i * ldc + j;
+    *c = beta == T(0) ? alpha * acc : fma(alpha, acc, beta * *c);
+  }
 }
 
 // C = beta*C over a strided m x n view (tri: 0 all, 1 lower, 2 upper); beta == 0 stores 0.
@@ -533,6 +574,27 @@ int gemm_any(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, 
   if (big_tiles >= 2 * cus)
     return launch_layout<Cfg<T, 128, 128>>(M, N, K, alpha, A, lda, a_kcontig, B, ldb, b_ncontig,
                                            beta, C, ldc, tri, s);
+  // few output tiles and a long K (products like U^T C of tall panels): split K over a second
+  // grid dimension, partial products in a workspace, then one ordered reduction.  Only on the
+  // library's main stream (one workspace).
+  const int64_t tiles64 = (int64_t)ceil_div(M, 64) * ceil_div(N, 64);
+  if (s == stream() && tiles64 * 2 < cus && K >= 1024) {
+    int splits = (int)std::min<int64_t>(K / 512, ceil_div(2 * cus, tiles64));
+    if (splits >= 2) {
+      const int kchunk = (ceil_div(K, splits) + 31) / 32 * 32;
+      splits = ceil_div(K, kchunk);
+      void* ws = nullptr;
+      DDLA_RETURN_IF(workspace(WS_SPLITK, (size_t)splits * M * N * sizeof(T), &ws));
+      T* P = static_cast<T*>(ws);
+      DDLA_RETURN_IF((launch_layout<Cfg<T, 64, 64, 32>>(M, N, K, T(1), A, lda, a_kcontig, B, ldb,
+                                                        b_ncontig, T(0), P, N, 0, s, splits,
+                                                        kchunk)));
+      hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3(grid_for((int64_t)M * N)), dim3(256), 0, s,
+                         M, N, splits, P, alpha, beta, C, ldc, tri);
+      DDLA_HIP_CHECK(hipGetLastError());
+      return 0;
+    }
+  }
   // small tiles: one wave per SIMD, so each K step is made long enough (32) for the MFMA work
   // to cover the latency of the next step's global loads
   if constexpr (sizeof(T) == 8) {
