@@ -26,7 +26,7 @@ __all__ = [
     "LEFT", "RIGHT", "PREC_FP64", "PREC_FP32", "PREC_FP16", "PREC_BF16", "DDLAError",
     "BatchPlan", "batch_factor", "BATCH_LU", "BATCH_CHOL", "sgetrf_mp", "PIVOT_PARTIAL",
     "PIVOT_BUTTERFLY", "PIVOT_TOURNAMENT", "LDLT", "LDLTDecompose", "LDLTSolve", "QR",
-    "QRDecompose", "QRApply", "QRSolve", "qr_t_rows",
+    "QRDecompose", "QRApply", "QRSolve", "qr_t_rows", "Contract", "ContractNew", "contract_dev",
 ]
 
 
@@ -696,6 +696,60 @@ class QR:
         R = np.triu(self.matrix)
         self._deconstructed = True
         return Q, R
+
+
+def _dev_tensor(x):
+    """A contiguous device copy of a numpy tensor: (DeviceMatrix keeping it, shape, strides)."""
+    x = np.array(_as_f64(x), order="C", copy=True)  # keeps a 0-d tensor 0-d
+    flat = x.reshape(1, max(1, x.size)) if x.size else np.zeros((1, 1))
+    d = DeviceMatrix.from_host(flat)
+    strides = [st // 8 for st in x.strides]
+    return d, list(x.shape), strides
+
+
+def _i64s(v):
+    return (ctypes.c_int64 * max(1, len(v)))(*v)
+
+
+def contract_dev(expression, alpha, a_ptr, a_shape, a_strides, b_ptr, b_shape, b_strides, beta,
+                 c_ptr, c_shape, c_strides):
+    """C = beta C + alpha contract(expression, A, B) on device tensors given by pointer,
+    lengths and element strides (MArray.Contract, DDLA/Einsum/MArray.Contract.cs:35-125)."""
+    return check(_lib().ddla_dcontract_dev(
+        expression.encode(), alpha, a_ptr, len(a_shape), _i64s(a_shape), _i64s(a_strides),
+        b_ptr, len(b_shape), _i64s(b_shape), _i64s(b_strides), beta, c_ptr, len(c_shape),
+        _i64s(c_shape), _i64s(c_strides)))
+
+
+def Contract(result, expression, alpha, left, right, beta=1.0):
+    """result = beta*result + alpha*contract(expression, left, right) on the GPU
+    (MArray.Contract(beta, result, expression, alpha, left, right), MArray.Contract.cs:35-58);
+    numpy tensors of any strides, result updated in place."""
+    C = _as_f64(result)
+    dA, sa, ta = _dev_tensor(left)
+    dB, sb, tb = _dev_tensor(right)
+    dC, sc, tc = _dev_tensor(C)
+    contract_dev(expression, alpha, dA.ptr, sa, ta, dB.ptr, sb, tb, beta, dC.ptr, sc, tc)
+    if C.size:
+        C[...] = dC.to_host().reshape(C.shape)
+    return C
+
+
+def ContractNew(expression, alpha, left, right):
+    """A new result tensor (MArray.Contract(expression, alpha, left, right), :131-250): its
+    lengths from the symbols' lengths in left or right."""
+    lhs, res = [t.strip() for t in expression.split("->")]
+    sa, sb = [t.strip() for t in lhs.split(",")]
+    left, right = _as_f64(left), _as_f64(right)
+    lens = {}
+    for sym, arr in ((sb, right), (sa, left)):
+        for ch, L in zip(sym, arr.shape):
+            lens[ch] = L
+    missing = [ch for ch in res if ch not in lens]
+    if missing:
+        raise ValueError(f"Symbol '{missing[0]}' not found in either tensor.")
+    C = np.zeros([lens[ch] for ch in res])
+    return Contract(C, expression, alpha, left, right, beta=0.0)
 
 
 PIVOT_PARTIAL, PIVOT_BUTTERFLY, PIVOT_TOURNAMENT = 0, 1, 2

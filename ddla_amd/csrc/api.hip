@@ -601,6 +601,20 @@ int ddla_dldlt_solve(int uplo, ddla_dim_t n, ddla_dim_t nrhs, const double* a, d
   return download_view(n, nrhs, dB, b, rsb, csb);
 }
 
+int ddla_dcontract_dev(const char* expr, double alpha, const double* a, int na,
+                       const int64_t* len_a, const int64_t* str_a, const double* b, int nb,
+                       const int64_t* len_b, const int64_t* str_b, double beta, double* c,
+                       int nc, const int64_t* len_c, const int64_t* str_c) {
+  API_INIT();
+  DDLA_ARG_CHECK(expr != nullptr, "null expression");
+  DDLA_ARG_CHECK((na == 0 || (len_a && str_a)) && (nb == 0 || (len_b && str_b)) &&
+                     (nc == 0 || (len_c && str_c)),
+                 "null shape arrays");
+  DDLA_ARG_CHECK(a && b && c, "null tensor pointer");
+  return dcontract_internal(expr, alpha, a, na, len_a, str_a, b, nb, len_b, str_b, beta, c, nc,
+                            len_c, str_c);
+}
+
 int ddla_qr_t_rows(ddla_dim_t m, ddla_dim_t n) {
   if (m < 0 || n < 0) return 0;
   return qr_t_rows(m, n);

@@ -63,7 +63,11 @@ enum WorkspaceSlot {
   WS_QR_WORK = 17, // QR: explicit Householder vectors of a block, products, tau
   WS_QR_Y = 18,    // QR solve: Q^T B
   WS_SPLITK = 19,  // split-K GEMM partial products (main stream only)
-  WS_COUNT = 20,
+  WS_EIN_IDX = 20, // tensor contraction: scatter vectors
+  WS_EIN_A = 21,   // tensor contraction: gathered operands and product
+  WS_EIN_B = 22,
+  WS_EIN_C = 23,
+  WS_COUNT = 24,
 };
 
 struct Context {
@@ -341,6 +345,10 @@ int qr_t_rows(int64_t m, int64_t n);
 int dgeqrf_internal(int64_t m, int64_t n, double* a, int64_t lda, double* t, int64_t ldt);
 int dqr_apply_internal(bool trans, int64_t m, int64_t n, const double* a, int64_t lda,
                        const double* t, int64_t ldt, int64_t nrhs, double* b, int64_t ldb);
+int dcontract_internal(const char* expr, double alpha, const double* a, int na,
+                       const int64_t* len_a, const int64_t* str_a, const double* b, int nb,
+                       const int64_t* len_b, const int64_t* str_b, double beta, double* c,
+                       int nc, const int64_t* len_c, const int64_t* str_c);
 int dqr_solve_internal(int64_t m, int64_t n, int64_t nrhs, const double* a, int64_t lda,
                        const double* t, int64_t ldt, const double* b, int64_t ldb, double* x,
                        int64_t ldx);

@@ -81,6 +81,8 @@ SIGNATURES = {
     "ddla_dldlt_solve_dev": [cint, i64, i64, vp, i64, vp, i64],
     "ddla_dldlt": [cint, i64, vp, i64, i64],
     "ddla_dldlt_solve": [cint, i64, i64, vp, i64, i64, vp, i64, i64],
+    "ddla_dcontract_dev": [ctypes.c_char_p, ctypes.c_double, vp, cint, vp, vp, vp, cint, vp, vp,
+                           ctypes.c_double, vp, cint, vp, vp],
     "ddla_qr_t_rows": [i64, i64],
     "ddla_dgeqrf_dev": [i64, i64, vp, i64, vp, i64],
     "ddla_dqr_apply_dev": [cint, i64, i64, vp, i64, vp, i64, i64, vp, i64],

@@ -235,6 +235,22 @@ int ddla_dqr_solve(ddla_dim_t m, ddla_dim_t n, ddla_dim_t nrhs, const double* a,
                    const double* b, ddla_inc_t rsb, ddla_inc_t csb, double* x, ddla_inc_t rsx,
                    ddla_inc_t csx);
 
+/* ------------------------------------------------------------- tensor contraction ---- */
+/* C = beta C + alpha contract(expr, A, B) for strided FP64 device tensors (lengths and
+ * element strides per dimension, at most 16 dimensions).  expr is "<A symbols>,<B
+ * symbols>-><C symbols>", one character per dimension.  Replaces MArray.Contract(beta,
+ * result, expression, alpha, left, right) (DDLA/Einsum/MArray.Contract.cs:35-125, the
+ * block-scatter contraction :252-405) with the index rules of IndiceUtils (Utilities/
+ * IndiceUtils.cs:18-130): a repeated symbol in one tensor is its diagonal; symbols of A and B
+ * only are summed products, of A alone or B alone summed, of C alone broadcast.  beta scales
+ * every element of C first (beta = 0 stores 0); alpha = 0 leaves C scaled only.  Errors
+ * (DDLA_ERR_INVALID_ARG) for a malformed expression, a rank that does not match its symbols,
+ * or a symbol with different lengths. */
+int ddla_dcontract_dev(const char* expr, double alpha, const double* a, int na,
+                       const int64_t* len_a, const int64_t* str_a, const double* b, int nb,
+                       const int64_t* len_b, const int64_t* str_b, double beta, double* c,
+                       int nc, const int64_t* len_c, const int64_t* str_c);
+
 /* ------------------------------------------------------------ level-3 pieces ------- */
 /* B = alpha * op(A)^-1 B (left) or B op(A)^-1 (right), row-major device matrices with leading
  * dimensions: the device-resident form of bli_dtrsm (DDLA/BLAS/Native/3m.cs:151-162) for the

@@ -443,6 +443,66 @@ def qr_solve(A, T, B):
     return sla.solve_triangular(np.triu(A[:n, :n]), Y[:n], lower=False)
 
 
+def contract(expression, alpha, A, B, beta, C):
+    """C = beta C + alpha contract(expression, A, B) with MArray.Contract's index rules
+    (DDLA/Einsum/MArray.Contract.cs:35-125; IndiceUtils.Diagonal / Divide,
+    Utilities/IndiceUtils.cs:18-130), restated as explicit loops over the index classes:
+    a repeated symbol within one tensor is its diagonal; symbols of A and B only are summed
+    products (K), of A and C the rows (M), of B and C the columns (N), of all three a batch;
+    symbols of A alone or B alone are summed, of C alone broadcast.  C is updated in place
+    (every element scaled by beta first, then alpha = 0 returns)."""
+    arrow = expression.split("->")
+    if len(arrow) != 2:
+        raise ValueError("Expression must contain exactly one '->' symbol.")
+    ops = arrow[0].split(",")
+    if len(ops) != 2:
+        raise ValueError("Expression must contain exactly two symbols before '->'.")
+    sa, sb, sc = ops[0].strip(), ops[1].strip(), arrow[1].strip()
+
+    def diag(sym, T):
+        if len(sym) != T.ndim:
+            raise ValueError("The rank of the tensor must match the length of the symbol.")
+        out = {}
+        for i, ch in enumerate(sym):
+            L, st = T.shape[i], T.strides[i] // T.itemsize
+            if ch in out:
+                if out[ch][0] != L:
+                    raise ValueError(f"The character '{ch}' already exists with different lengths.")
+                out[ch] = (L, out[ch][1] + st)
+            else:
+                out[ch] = (L, st)
+        return out
+
+    ia, ib, ic = diag(sa, A), diag(sb, B), diag(sc, C)
+    for ch in set(ia) | set(ib) | set(ic):
+        ls = {d[ch][0] for d in (ia, ib, ic) if ch in d}
+        if len(ls) > 1:
+            raise ValueError(f"The character '{ch}' exists in both indices with different lengths.")
+    C[...] = 0.0 if beta == 0.0 else beta * C
+    if alpha == 0.0:
+        return C
+    syms = sorted(set(ia) | set(ib) | set(ic))
+    lens = [next(d[ch][0] for d in (ia, ib, ic) if ch in d) for ch in syms]
+    # a flat view over the memory each tensor spans (non-negative strides), indexed by the
+    # element strides of the merged indices
+    def flat(T):
+        st = np.array(T.strides, dtype=np.int64) // T.itemsize
+        if (st < 0).any():
+            raise ValueError("negative strides are not handled here")
+        hi = int(sum((n - 1) * s for n, s in zip(T.shape, st))) if T.size else 0
+        return np.lib.stride_tricks.as_strided(T, shape=(hi + 1,), strides=(T.itemsize,)), 0
+
+    (fa_, oa), (fb_, ob), (fc_, oc) = flat(A), flat(B), flat(C)
+    if any(L == 0 for L in lens):
+        return C
+    for multi in np.ndindex(*lens):
+        offa = oa + sum(i * ia[ch][1] for i, ch in zip(multi, syms) if ch in ia)
+        offb = ob + sum(i * ib[ch][1] for i, ch in zip(multi, syms) if ch in ib)
+        offc = oc + sum(i * ic[ch][1] for i, ch in zip(multi, syms) if ch in ic)
+        fc_[offc] += alpha * (fa_[offa] * fb_[offb])
+    return C
+
+
 def lu_factors(LU):
     """Split a packed LU into unit-lower L and upper U (LU.Deconstruct, LU.cs:112-128)."""
     L = np.tril(LU, -1) + np.eye(LU.shape[0])

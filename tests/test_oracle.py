@@ -597,3 +597,51 @@ def test_qr_oracle_least_squares(m, n, nrhs):
     X = O.qr_solve(A, T, B)
     Xl = np.linalg.lstsq(A0, B, rcond=None)[0]
     np.testing.assert_allclose(X, Xl, rtol=1e-9, atol=1e-11)
+
+
+# ------------------------------------------------------- tensor contraction restatement ----
+# Pinned by numpy.einsum wherever einsum can state the same contraction (products, batches,
+# transposed outputs, diagonals of inputs, indices summed in one operand), and by hand-built
+# answers for what einsum cannot state (an index of the result alone is a broadcast, a
+# repeated index of the result its diagonal).
+
+@pytest.mark.parametrize("expr,sa,sb", [
+    ("ij,jk->ik", (4, 5), (5, 3)),
+    ("ij,jk->ki", (4, 5), (5, 3)),
+    ("bij,bjk->bik", (3, 4, 5), (3, 5, 2)),
+    ("abcd,cdef->abef", (2, 3, 4, 2), (4, 2, 3, 2)),
+    ("iij,jk->ik", (4, 4, 3), (3, 2)),
+    ("ijx,jk->ik", (3, 4, 2), (4, 5)),
+    ("ij,kl->ijkl", (2, 3), (2, 2)),
+    ("i,i->", (7,), (7,)),
+])
+def test_contract_oracle_matches_einsum(expr, sa, sb):
+    rng = np.random.default_rng(len(expr))
+    A = rng.uniform(-1, 1, sa)
+    B = rng.uniform(-1, 1, sb)
+    ref = np.einsum(expr, A, B)
+    C0 = rng.uniform(-1, 1, ref.shape)
+    C = C0.copy()
+    O.contract(expr, 0.75, A, B, 0.5, C)
+    np.testing.assert_allclose(C, 0.5 * C0 + 0.75 * ref, rtol=1e-13, atol=1e-14)
+
+
+def test_contract_oracle_strided_views_broadcast_and_result_diagonal():
+    rng = np.random.default_rng(5)
+    big = rng.uniform(-1, 1, (6, 10))
+    A = big[:, ::2]          # 6 x 5 view, column stride 2
+    B = rng.uniform(-1, 1, (3, 5)).T  # 5 x 3 transposed view
+    C = np.zeros((6, 3, 2))
+    O.contract("ij,jk->ikz", 1.0, A, B, 0.0, C)  # z only in C: broadcast
+    for z in range(2):
+        np.testing.assert_allclose(C[:, :, z], A @ B, rtol=1e-13)
+    D = np.zeros((4, 4))
+    X = rng.uniform(-1, 1, (4, 3))
+    Y = rng.uniform(-1, 1, (3, 4))
+    O.contract("ij,ji->ii", 1.0, X, Y, 0.0, D)  # a repeated result index: its diagonal
+    np.testing.assert_allclose(np.diag(D), np.einsum("ij,ji->i", X, Y), rtol=1e-13)
+    assert np.count_nonzero(D - np.diag(np.diag(D))) == 0
+    with pytest.raises(ValueError):
+        O.contract("ij,jk", 1.0, X, Y, 0.0, D)
+    with pytest.raises(ValueError):
+        O.contract("ij,jk->ik", 1.0, X, rng.uniform(-1, 1, (4, 4)), 0.0, D)
