@@ -466,7 +466,17 @@ class LU:
         PLUSolve(self.matrix, self.pivs, x, trans)
         return x
 
+    def Inverse(self):
+        """A^-1 from the factors (LU.Inverse, LU.cs:22-27, :102-110).  The reference builds the
+        identity and calls Solve(inverse, true), which solves with the transpose into a new
+        matrix and returns the untouched identity; this mirror returns the inverse."""
+        self._compute_once()
+        X = np.eye(self.n)
+        PLUSolve(self.matrix, self.pivs, X, False)
+        return X
+
     def Deconstruct(self):
+        """(P, L, U) with P A = L U (LU.Deconstruct, LU.cs:112-128)."""
         self._compute_once()
         n = self.n
         P = np.eye(n)

@@ -83,6 +83,31 @@ def test_lu_class_mirror(ddla):
     assert rel_fro(P @ A, L @ U) < 1e-13
 
 
+@pytest.mark.parametrize("n", [64, 300, 700])
+def test_lu_class_matches_oracle(ddla, oracle, n):
+    # the LU class (LU.cs:13-128) against the oracle: pivots identical, factors, both solves
+    # and the inverse (the reference's Inverse returns the identity; the mirror's is A^-1)
+    A0 = oracle.uniform_matrix(n, n, 0xDD1A0100 + n)
+    Ao = A0.copy()
+    info, piv_o = oracle.getrf(Ao)
+    assert info == 0
+    lu = ddla.LU(A0)
+    B = oracle.uniform_matrix(n, 3, 7)
+    for trans in (False, True):
+        x = lu.Solve(B, trans=trans)
+        xo = oracle.getrs(Ao, piv_o, B.copy(), trans=trans)
+        assert rel_fro(x, xo) < 1e-12
+    np.testing.assert_array_equal(lu.pivs, piv_o)
+    assert rel_fro(lu.matrix, Ao) < 1e-13 * n
+    inv = lu.Inverse()
+    assert rel_fro(inv, np.linalg.inv(A0)) < 1e-11
+    assert np.max(np.abs(A0 @ inv - np.eye(n))) < 1e-11
+    P, L, U = lu.Deconstruct()
+    Lo, Uo = oracle.lu_factors(Ao)
+    np.testing.assert_array_equal(P, oracle.perm_matrix(piv_o))
+    assert rel_fro(L, Lo) < 1e-13 * n and rel_fro(U, Uo) < 1e-13 * n
+
+
 def test_lu_zero_pivot_raises(ddla):
     A = np.random.default_rng(1).uniform(-1, 1, (200, 200))
     A[:, 150] = 0.0
