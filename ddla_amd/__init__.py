@@ -854,7 +854,9 @@ class BatchPlan:
     schedules them over one persistent launch captured in a hipGraph (ddla_batch_create).
 
     a_ptr / piv_ptr / info_ptr are device addresses (ints or c_void_p) of the packed matrices,
-    pivots (sum n) and per-problem status (count)."""
+    pivots (sum n) and per-problem status (count).  tune: 0 (or False) a fixed shape, 1 (or
+    True) the tuning cache's shape for the batch's bucket or a timed choice that is then
+    stored, 2 always timed and stored."""
 
     def __init__(self, n, kind, prec, a_ptr, piv_ptr, info_ptr, tune=False):
         self.n = np.ascontiguousarray(n, dtype=np.int32)
@@ -865,7 +867,7 @@ class BatchPlan:
         check(_lib().ddla_batch_create(ctypes.byref(h), prec, self.n.size, _ptr(self.n),
                                        _ptr(self.kind), ctypes.c_void_p(int(_addr(a_ptr))),
                                        ctypes.c_void_p(int(_addr(piv_ptr))),
-                                       ctypes.c_void_p(int(_addr(info_ptr))), 1 if tune else 0))
+                                       ctypes.c_void_p(int(_addr(info_ptr))), int(tune)))
         self.handle = h
 
     def execute(self):

@@ -26,9 +26,17 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
 #include <map>
 #include <queue>
+#include <sstream>
+#include <string>
 #include <vector>
+
+#include <sys/stat.h>
+#include <unistd.h>
 
 namespace ddla {
 namespace {
@@ -426,11 +434,109 @@ struct BatchPlan {
   int64_t total_elems = 0, total_piv = 0;
   double flops = 0.0, bytes = 0.0;
   std::vector<double> tune_ms;  // per candidate, in candidate order (if tuned)
+  std::string tune_key;         // the tuning cache's key of this batch's bucket
+  bool from_cache = false;      // shape taken from the tuning cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 namespace {
+
+// ---- tuning cache: one JSON file of (bucket key -> chosen shape) entries, one per line:
+//   {"key": "<arch>|batch|<f64|f32>|n<max order, next power of 2>|c<log2 count>|<lu|chol|mixed>",
+//    "nb": 32, "wg_per_cu": 4, "ms": 16.1}
+// at $DDLA_TUNE_CACHE, else $XDG_CACHE_HOME/ddla_amd/tune.json, else
+// $HOME/.cache/ddla_amd/tune.json.  A missing or unwritable file only means no cache.
+std::string tune_cache_path() {
+  if (const char* p = std::getenv("DDLA_TUNE_CACHE")) return p;
+  if (const char* x = std::getenv("XDG_CACHE_HOME")) return std::string(x) + "/ddla_amd/tune.json";
+  if (const char* h = std::getenv("HOME")) return std::string(h) + "/.cache/ddla_amd/tune.json";
+  return "";
+}
+
+std::string tune_bucket_key(int prec, const std::vector<int32_t>& n, const int32_t* kind,
+                            int64_t count) {
+  hipDeviceProp_t prop{};
+  std::string arch = "unknown";
+  if (hipGetDeviceProperties(&prop, ctx().device) == hipSuccess) {
+    arch = prop.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch = arch.substr(0, colon);
+  }
+  int nmax = 1, lu = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    nmax = std::max(nmax, (int)n[(size_t)i]);
+    lu += kind[i] == DDLA_BATCH_LU;
+  }
+  int nb2 = 1;
+  while (nb2 < nmax) nb2 *= 2;
+  int c2 = 0;
+  while ((int64_t(1) << (c2 + 1)) <= count) ++c2;
+  const char* mix = lu == count ? "lu" : (lu == 0 ? "chol" : "mixed");
+  std::ostringstream k;
+  k << arch << "|batch|" << (prec == DDLA_PREC_FP64 ? "f64" : "f32") << "|n" << nb2 << "|c" << c2
+    << "|" << mix;
+  return k.str();
+}
+
+struct TuneEntry {
+  int nb = 0, wg = 0;
+  double ms = 0.0;
+};
+
+std::map<std::string, TuneEntry> tune_cache_read(const std::string& path) {
+  std::map<std::string, TuneEntry> out;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    const size_t k = line.find("\"key\": \"");
+    if (k == std::string::npos) continue;
+    const size_t k0 = k + 8, k1 = line.find('"', k0);
+    if (k1 == std::string::npos) continue;
+    TuneEntry e;
+    const size_t nbp = line.find("\"nb\": "), wgp = line.find("\"wg_per_cu\": "),
+                 msp = line.find("\"ms\": ");
+    if (nbp == std::string::npos || wgp == std::string::npos) continue;
+    e.nb = std::atoi(line.c_str() + nbp + 6);
+    e.wg = std::atoi(line.c_str() + wgp + 13);
+    if (msp != std::string::npos) e.ms = std::atof(line.c_str() + msp + 6);
+    if ((e.nb == 16 || e.nb == 32) && e.wg >= 1 && e.wg <= 8) out[line.substr(k0, k1 - k0)] = e;
+  }
+  return out;
+}
+
+void tune_cache_write(const std::string& path, const std::map<std::string, TuneEntry>& m) {
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {
+    // create the directory chain (best effort)
+    std::string dir = path.substr(0, slash), cur;
+    std::stringstream ss(dir);
+    std::string part;
+    if (!dir.empty() && dir[0] == '/') cur = "/";
+    while (std::getline(ss, part, '/')) {
+      if (part.empty()) continue;
+      cur += part + "/";
+      (void)mkdir(cur.c_str(), 0755);
+    }
+  }
+  const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+  {
+    std::ofstream f(tmp);
+    if (!f) return;
+    f << "{\"entries\": [\n";
+    size_t i = 0;
+    for (const auto& [key, e] : m) {
+      f << "  {\"key\": \"" << key << "\", \"nb\": " << e.nb << ", \"wg_per_cu\": " << e.wg
+        << ", \"ms\": " << e.ms << "}" << (++i < m.size() ? "," : "") << "\n";
+    }
+    f << "]}\n";
+    if (!f) {
+      std::remove(tmp.c_str());
+      return;
+    }
+  }
+  if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
 
 int max_wg_per_cu(int prec, int nb) {
   static std::map<std::pair<int, int>, int> cache;
@@ -557,7 +663,25 @@ int batch_create(BatchPlan** out, int prec, int64_t count, const int32_t* n_host
   p->wg_per_cu = std::min(2, cand[0].second);
   for (auto& c : cand)
     if (c.first == 32 && c.second == 2) p->wg_per_cu = 2;
+  // tune = 1: the tuning cache's shape for this batch's bucket if it has one, else time the
+  // candidates and store the winner; tune = 2: time and store even on a hit
   if (tune && count > 0) {
+    p->tune_key = tune_bucket_key(prec, p->n, kind_host, count);
+    const std::string path = tune_cache_path();
+    if (tune == 1 && !path.empty()) {
+      const auto m = tune_cache_read(path);
+      auto it = m.find(p->tune_key);
+      if (it != m.end()) {
+        for (auto& c : cand)
+          if (c.first == it->second.nb && c.second == it->second.wg) {
+            p->nb = c.first;
+            p->wg_per_cu = c.second;
+            p->from_cache = true;
+          }
+      }
+    }
+  }
+  if (tune && count > 0 && !p->from_cache) {
     // time each candidate on a copy of the batch (the input stays untouched)
     void* ta = nullptr;
     int32_t *tp = nullptr, *ti = nullptr;
@@ -603,6 +727,12 @@ int batch_create(BatchPlan** out, int prec, int64_t count, const int32_t* n_host
     (void)hipFree(tp);
     (void)hipFree(ti);
     if (st) return fail(st);
+    const std::string path = tune_cache_path();
+    if (!path.empty()) {
+      auto m = tune_cache_read(path);
+      m[p->tune_key] = TuneEntry{p->nb, p->wg_per_cu, best};
+      tune_cache_write(path, m);
+    }
   }
   p->G = (int)std::max<int64_t>(1, std::min<int64_t>(cus * p->wg_per_cu, count));
   if (count > 0) {
@@ -660,8 +790,10 @@ int batch_describe(const BatchPlan* p, char* buf, size_t len) {
   }
   snprintf(buf, len,
            "{\"count\": %lld, \"nb\": %d, \"wg_per_cu\": %d, \"workgroups\": %d, "
-           "\"flops\": %.6g, \"bytes\": %.6g, \"tune_ms\": [%s]}",
-           (long long)p->count, p->nb, p->wg_per_cu, p->G, p->flops, p->bytes, t.c_str());
+           "\"flops\": %.6g, \"bytes\": %.6g, \"tune_ms\": [%s], \"tune_key\": \"%s\", "
+           "\"from_cache\": %s}",
+           (long long)p->count, p->nb, p->wg_per_cu, p->G, p->flops, p->bytes, t.c_str(),
+           p->tune_key.c_str(), p->from_cache ? "true" : "false");
   return 0;
 }
 

@@ -410,8 +410,13 @@ int ddla_sgetrf_mp_dev(ddla_dim_t n, float* a, ddla_inc_t lda, int32_t prec, int
  * pivot / non-positive Cholesky pivot, where that problem stopped).
  *
  * A plan binds the buffers, sorts and balances the problems over one persistent launch,
- * chooses the panel width and workgroups per CU (timing every candidate on a copy of the
- * batch when tune != 0; the input is not modified), and captures the launch in a hipGraph.
+ * chooses the panel width and workgroups per CU, and captures the launch in a hipGraph.
+ * tune = 0: a fixed default shape.  tune = 1: the shape stored for this batch's bucket (device
+ * architecture, precision, largest order rounded up to a power of two, log2 of the count,
+ * LU / Cholesky / mixed) in the tuning cache, else every candidate is timed on a copy of the
+ * batch (the input is not modified) and the winner is stored; tune = 2: always time, and store.
+ * The cache is a JSON file at $DDLA_TUNE_CACHE, else $XDG_CACHE_HOME/ddla_amd/tune.json, else
+ * $HOME/.cache/ddla_amd/tune.json; an unreadable or unwritable file only disables it.
  * ddla_batch_execute replays the graph on the library stream (asynchronous; ddla_sync waits).
  * n and kind are host arrays; a, piv and info are device buffers. */
 #define DDLA_BATCH_LU 0   /* LU with partial pivoting */
@@ -420,8 +425,8 @@ typedef struct ddla_batch_s* ddla_batch_t;
 int ddla_batch_create(ddla_batch_t* plan, int prec, int64_t count, const int32_t* n,
                       const int32_t* kind, void* a, int32_t* piv, int32_t* info, int tune);
 int ddla_batch_execute(ddla_batch_t plan);
-/* JSON text: chosen panel width, workgroups per CU, workgroups, flop and byte counts, and the
- * autotuner's candidate timings (ms) */
+/* JSON text: chosen panel width, workgroups per CU, workgroups, flop and byte counts, the
+ * autotuner's candidate timings (ms), the bucket key and whether the shape came from the cache */
 int ddla_batch_describe(ddla_batch_t plan, char* buf, size_t len);
 int ddla_batch_destroy(ddla_batch_t plan);
 

@@ -133,3 +133,47 @@ def test_batched_plan_graph_replays(ddla, oracle):
                 np.testing.assert_array_equal(x, y)
     d = plan.describe()
     assert d["count"] == 64 and d["nb"] in (16, 32) and d["workgroups"] >= 1
+
+
+def test_batched_tuning_cache(ddla, oracle, tmp_path, monkeypatch):
+    # tune = 1 times the candidates once per bucket and stores the winner; the next plan of the
+    # same bucket takes it from the cache without timing; tune = 2 times again
+    path = tmp_path / "tune.json"
+    monkeypatch.setenv("DDLA_TUNE_CACHE", str(path))
+    rng = np.random.default_rng(12)
+    count = 200
+    sizes = rng.integers(8, 129, count)
+    kinds = rng.integers(0, 2, count)
+    mats = make_batch(oracle, sizes, kinds, 1200)
+    n = np.array(sizes, dtype=np.int32)
+    flat = np.concatenate([m.ravel() for m in mats])
+    a = ddla._DevBuf(flat.nbytes)
+    piv = ddla._DevBuf(int(n.sum()) * 4)
+    info = ddla._DevBuf(count * 4)
+    a.upload(flat)
+    first = ddla.BatchPlan(n, kinds, ddla.PREC_FP64, a.ptr, piv.ptr, info.ptr, tune=1).describe()
+    assert first["from_cache"] is False and len(first["tune_ms"]) >= 2
+    text = path.read_text()
+    assert first["tune_key"] in text and "|f64|n128|c7|mixed" in first["tune_key"]
+    second = ddla.BatchPlan(n, kinds, ddla.PREC_FP64, a.ptr, piv.ptr, info.ptr, tune=1)
+    d2 = second.describe()
+    assert d2["from_cache"] is True and d2["tune_ms"] == []
+    assert (d2["nb"], d2["wg_per_cu"]) == (first["nb"], first["wg_per_cu"])
+    second.execute()
+    ddla.synchronize()
+    out = a.download(np.float64, flat.size)
+    pv = piv.download(np.int32, int(n.sum()))
+    st = info.download(np.int32, count)
+    F, P, o, po = [], [], 0, 0
+    for i, ni in enumerate(n):
+        F.append(out[o:o + ni * ni].reshape(ni, ni).copy())
+        P.append(pv[po:po + ni].copy() if kinds[i] == ddla.BATCH_LU else None)
+        o += ni * ni
+        po += ni
+    check_fp64(ddla, oracle, mats, kinds, F, P, st)
+    third = ddla.BatchPlan(n, kinds, ddla.PREC_FP64, a.ptr, piv.ptr, info.ptr, tune=2).describe()
+    assert third["from_cache"] is False and len(third["tune_ms"]) >= 2
+    # an unwritable cache location only disables the cache
+    monkeypatch.setenv("DDLA_TUNE_CACHE", "/proc/ddla_no_such_dir/tune.json")
+    d4 = ddla.BatchPlan(n, kinds, ddla.PREC_FP64, a.ptr, piv.ptr, info.ptr, tune=1).describe()
+    assert d4["from_cache"] is False
