@@ -534,6 +534,16 @@ def solve_only(ddla, args, A, B, X, n, nrhs, prec):
     if info is not None:
         out.update({"status": info.status, "iters": info.iters,
                     "backward_error": info.backward_error})
+    if args.breakdown:
+        # one more solve with per-launch event timing: kernel time per solve by kernel name
+        ddla.profile_reset()
+        ddla.profile_enable(True)
+        run()
+        ddla.synchronize()
+        ddla.profile_enable(False)
+        prof = ddla.profile_read()
+        out["kernels"] = {k: {"launches": v["count"], "ms": round(v["ms"], 3)}
+                          for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
     print(json.dumps(out))
 
 
@@ -565,6 +575,8 @@ def main():
     ap.add_argument("--solve-only", choices=["mixed", "fp64"], default=None,
                     help="only run the mixed solver or the plain FP64 LU + solve on the system "
                          "(--warmup + --steps times; for profiler runs) and print the timing")
+    ap.add_argument("--breakdown", action="store_true",
+                    help="with --solve-only: also time one solve per kernel (event profiling)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other mixed configurations (BF16, 64 right-hand sides, "
                          "Cholesky), one timed solve each")

@@ -816,6 +816,23 @@ def sgetrf_mp(A, prec=PREC_FP32, pivoting=PIVOT_TOURNAMENT):
     return (buf.download(np.float32, n * n).reshape(n, n), perm.download(np.int32, n), st)
 
 
+def sgetf2(P):
+    """The FP32 panel of the mixed LU on its own (PLUDecUnblock, LU.cs:344-374, in FP32; no
+    reference counterpart in this precision).
+
+    P: m x nb float32 host array (not modified), 1 <= nb <= min(m, 128).  Returns (LU, piv,
+    info): the factored panel, relative pivots (nb entries) and the first zero pivot column + 1
+    (0 if none)."""
+    a = np.ascontiguousarray(P, dtype=np.float32)
+    m, nb = a.shape
+    buf, piv = _DevBuf(a.nbytes), _DevBuf(4 * nb)
+    buf.upload(a)
+    st = _lib().ddla_sgetf2_dev(m, nb, buf.ptr, nb, piv.ptr)
+    if st < 0:
+        check(st)
+    return buf.download(np.float32, m * nb).reshape(m, nb), piv.download(np.int32, nb), st
+
+
 # ------------------------------------------------------------ batched factorizations ---
 BATCH_LU, BATCH_CHOL = 0, 1
 

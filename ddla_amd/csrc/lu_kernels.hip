@@ -33,6 +33,8 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
+#include <cstring>
 
 namespace ddla {
 
@@ -507,6 +509,432 @@ __global__ __launch_bounds__(NW * 64) void getf2_panel_kernel(int m, int nb, T* 
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// FP32 panel with one row per lane (getf2_rows_kernel): the same steps, pivots, labels and
+// outputs as getf2_panel_kernel<float>, another distribution of the panel.  Lane l of wave v
+// of workgroup w holds panel row w * 64 NW + 64 v + l, all of its (at most 128) columns, in
+// registers: the arg-max of a column is a reduction across lanes, and the rank-1 update of a
+// row is a run of fused multiply-adds against the pivot row broadcast from LDS.  With 64 NW
+// rows per workgroup few workgroups hold the panel (32 at m = 16384), and each publishes its
+// candidate row together with its key, so a column step costs one round trip between them.
+// Column step i:
+//   1. arg-max of column i: an unsigned key per lane (0: not a candidate, 1: NaN, else the
+//      bits of |x| + 2, so NaN ranks below every number as in AMax) reduced by DPP max across
+//      the lanes, ties broken by the smallest label (a DPP min only when several lanes tie);
+//      the waves' winners meet in an LDS atomic max on (key << 32 | ~label), one barrier;
+//   2. the wave that holds the workgroup's candidate stages that row in LDS and publishes key,
+//      packed label (label * 256 + workgroup) and the row's columns from the current 16-column
+//      block on as tagged words in the workgroup's slot (no candidate: key 0 and zeros);
+//   3. wave v reads slots v, v + NW, ... -- keys in lanes [0, S), rows in every lane, all
+//      loads in flight before any tag is checked -- keeps the best, writes its row to LDS
+//      (columns <= i zeroed, the column-i value aside) and its key to a second LDS atomic max;
+//   4. after a barrier every workgroup reads the same winner; rows are relabelled (no data
+//      moves): the pivot row takes label i, the row labelled i the pivot's label;
+//   5. every row not yet pivotal forms its multiplier (column i times 1/pivot) and updates
+//      column i + 1 only -- the next arg-max needs no more.  The update of its other columns
+//      after i + 1 is applied in step i + 1, between issuing the slot reads and checking them,
+//      so it runs in the shadow of the exchange; the candidate row published in step i + 1
+//      takes that pending update while it is staged (the same fma on the same operands).
+// The update of a column by a pivot row is fma(-l, u, a) in the order of the pivots, as in
+// the reference; columns before the pivot keep their values (a select, not an fma with zero).
+// Columns are addressed by the static block (one code copy per 16-column block) and by the
+// uniform step inside the block, a dynamic element index of a 16-wide vector.  With a single
+// workgroup (m <= 64 NW) nothing is published: the candidate row goes to LDS directly.
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// slot of a workgroup: word 0 the key, word 1 the packed label, words 8 .. 135 the row's 128
+// values; 144 words keep slots on 128-byte lines
+constexpr int RS_WORDS = 144;
+static_assert(2 * (size_t)WMAX * RS_WORDS * sizeof(u64) <= PANEL_CAND_BYTES,
+              "row-panel slots exceed the candidate area");
+
+// max (MAX) or min of v over lanes [0, NL) by DPP, the result in every lane (uniform)
+template <int NL, bool MAX>
+__device__ __forceinline__ unsigned lanes_reduce_u32(unsigned v) {
+  constexpr unsigned id = MAX ? 0u : 0xFFFFFFFFu;
+  auto f = [](unsigned a, unsigned b) { return MAX ? max(a, b) : min(a, b); };
+  if constexpr (NL > 1) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x111, 0xf, 0xf, false));
+  if constexpr (NL > 2) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x112, 0xf, 0xf, false));
+  if constexpr (NL > 4) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x114, 0xf, 0xf, false));
+  if constexpr (NL > 8) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x118, 0xf, 0xf, false));
+  if constexpr (NL > 16) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
+  if constexpr (NL > 32) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, NL - 1);
+}
+
+// key of a column entry: 0 for rows that are not candidates, 1 for NaN, bits(|x|) + 2
+__device__ __forceinline__ unsigned arg_key(float x, bool cand) {
+  const unsigned b = __builtin_bit_cast(unsigned, x) & 0x7FFFFFFFu;
+  return !cand ? 0u : (b > 0x7F800000u ? 1u : b + 2u);
+}
+
+// The winner over lanes [0, NL): the largest key, among equal keys the smallest tie value
+// (label); returns the winning lane (-1 when every key is 0) and the key.
+template <int NL>
+__device__ __forceinline__ int lanes_winner(unsigned key, unsigned tie, int lane, unsigned& kmax) {
+  kmax = lanes_reduce_u32<NL, true>(key);
+  if (kmax == 0u) return -1;
+  const unsigned long long bm = __ballot(lane < NL && key == kmax);
+  if (__popcll(bm) == 1) return __ffsll(bm) - 1;
+  const unsigned t = lanes_reduce_u32<NL, false>(lane < NL && key == kmax ? tie : 0xFFFFFFFFu);
+  return __ffsll(__ballot(lane < NL && key == kmax && tie == t)) - 1;
+}
+
+template <int NW, int S>
+struct RowsPanel {
+  // uniform over the workgroup
+  int m, nb, W, w, wave, lane, row;
+  unsigned base, rounds;
+  int qwp;  // wrow slot of the previous step's pivot row
+  int32_t* piv;
+  int32_t* info;
+  int info_base;
+  PanelSync sy;
+  float (*stage)[NB_MAX + 4];      // per wave: the raw candidate row, its pending multiplier
+  float (*wrow)[NW][NB_MAX];       // [parity][wave]: the best row the wave read (cols <= i: 0)
+  float (*pval)[NW];               // [parity][wave]: its column-i value
+  unsigned long long* best;        // [parity]: workgroup winner, [2 + parity]: global winner
+  int* gaveup;
+  // per lane: the row, its label (INT_MAX beyond m), its column-i value and its multiplier of
+  // the previous step (the update of its columns after i + 1 still pending)
+  f16v blk[NB_MAX / 16];
+  int lbl;
+  float cur, lprev;
+
+  // the previous step's update of the columns after i (rows that took part in it)
+  template <int JB>
+  __device__ __forceinline__ void apply_prev(int i) {
+    if (i == 0 || !(lbl >= i && lbl != INT_MAX)) return;
+    const float nl = -lprev;
+    const float* u = wrow[(i - 1) & 1][qwp];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const f4v uu = *reinterpret_cast<const f4v*>(u + 16 * JB + 4 * q);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float y = fmaf(nl, uu[e], blk[JB][4 * q + e]);
+        blk[JB][4 * q + e] = 16 * JB + 4 * q + e > i ? y : blk[JB][4 * q + e];
+      }
+    }
+    // one 16-column block at a time: the scheduler would otherwise hoist every LDS read of
+    // the pivot row ahead of the multiply-adds and run out of registers
+#pragma unroll
+    for (int B = JB + 1; B < NB_MAX / 16; ++B) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const f4v uu = *reinterpret_cast<const f4v*>(u + 16 * B + 4 * q);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) blk[B][4 * q + e] = fmaf(nl, uu[e], blk[B][4 * q + e]);
+      }
+    }
+  }
+
+  // The candidate row (label lab, held by this wave) with its pending update applied: lane l
+  // ends with columns 2l and 2l + 1 (lanes >= 8 JB; zeros when has is false).
+  template <int JB>
+  __device__ __forceinline__ float2 staged_pair(int i, bool has, int lab) {
+    if (has && lbl == lab) {
+#pragma unroll
+      for (int B = JB; B < NB_MAX / 16; ++B)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          f4v x;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) x[e] = blk[B][4 * q + e];
+          *reinterpret_cast<f4v*>(&stage[wave][16 * B + 4 * q]) = x;
+        }
+      stage[wave][NB_MAX] = lprev;
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the staged row
+    __builtin_amdgcn_wave_barrier();
+    float2 x = make_float2(0.f, 0.f);
+    if (has && lane >= 8 * JB) {
+      const int c0 = 2 * lane;
+      x = *reinterpret_cast<const float2*>(&stage[wave][c0]);
+      if (i > 0) {
+        const float nl = -stage[wave][NB_MAX];
+        const float* u = wrow[(i - 1) & 1][qwp];
+        const float2 uu = *reinterpret_cast<const float2*>(u + c0);
+        if (c0 > i) x.x = fmaf(nl, uu.x, x.x);
+        if (c0 + 1 > i) x.y = fmaf(nl, uu.y, x.y);
+      }
+    }
+    return x;
+  }
+
+  // One column step; true when the panel stops (zero pivot, or a wait gave up).
+  template <int JB>
+  __device__ __forceinline__ bool step(int ii) {
+    const int i = 16 * JB + ii;
+    const int par = i & 1;
+    const unsigned tag = base + static_cast<unsigned>(i) + 1u;
+    // 1. the workgroup's candidate
+    {
+      const bool cand = lbl >= i && lbl != INT_MAX;
+      const unsigned key = arg_key(cur, cand);
+      unsigned kw;
+      const int wl = lanes_winner<64>(key, static_cast<unsigned>(lbl), lane, kw);
+      if (wl >= 0 && lane == 0) {
+        const unsigned lab = __builtin_amdgcn_readlane(lbl, wl);
+        __hip_atomic_fetch_max(&best[par], (static_cast<u64>(kw) << 32) | (0xFFFFFFFFu - lab),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    __syncthreads();
+    const u64 wb = best[par];
+    if (wave == 0 && lane == 0) {
+      // the words of the other parity were last read before this barrier
+      best[par ^ 1] = 0;
+      best[2 + (par ^ 1)] = 0;
+    }
+    const unsigned kwg = static_cast<unsigned>(wb >> 32);
+    const bool has = kwg != 0u;
+    const int labwg = has ? static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(wb)) : INT_MAX;
+    const bool holder = __ballot(has && lbl == labwg) != 0;  // this wave holds the candidate
+    bool gave = false;
+    // 2. publish (several workgroups) or stage (one): the candidate row with its pending
+    //    update, from the wave that holds it
+    const int nv = W > 1 && wave < W ? (W - wave + NW - 1) / NW : 0;  // slots this wave reads
+    const bool rl = lane >= 8 * JB;  // lanes that hold columns from block JB on
+    const u64* area = sy.cand + (size_t)par * WMAX * RS_WORDS;
+    u64 kw0 = 0, kw1 = 0, va[S], vb[S];
+    // every lane reads its pair of every slot (lanes < 8 JB too: their words are not checked)
+    auto issue = [&]() {
+      const u64* ks = area + (size_t)(wave + NW * min(lane, nv - 1)) * RS_WORDS;
+      kw0 = get(ks);
+      kw1 = get(ks + 1);
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const u64* vs = area + (size_t)(wave + NW * min(j, nv - 1)) * RS_WORDS + 8 + 2 * lane;
+        va[j] = get(vs);
+        vb[j] = get(vs + 1);
+      }
+    };
+    if (W > 1) {
+      if (holder || (!has && wave == 0)) {
+        const float2 x = staged_pair<JB>(i, has, labwg);
+        u64* slot = sy.cand + ((size_t)par * WMAX + w) * RS_WORDS;
+        if (rl) {
+          put(slot + 8 + 2 * lane, __builtin_bit_cast(unsigned, x.x), tag);
+          put(slot + 9 + 2 * lane, __builtin_bit_cast(unsigned, x.y), tag);
+        }
+        if (lane == 0) {
+          put(slot, kwg, tag);
+          put(slot + 1, static_cast<unsigned>(has ? labwg * 256 + w : INT_MAX), tag);
+        }
+      }
+      // 3. the first reads of this wave's slots go out before the pending update
+      if (nv > 0) issue();
+    } else {
+      if (holder) {
+        const float2 x = staged_pair<JB>(i, true, labwg);
+        if (rl) {
+          const int c0 = 2 * lane;
+          *reinterpret_cast<float2*>(&wrow[par][0][c0]) =
+              make_float2(c0 > i ? x.x : 0.f, c0 + 1 > i ? x.y : 0.f);
+          if (c0 == i) pval[par][0] = x.x;
+          if (c0 + 1 == i) pval[par][0] = x.y;
+        }
+      }
+      if (has && wave == 0 && lane == 0)
+        best[2 + par] = (static_cast<u64>(kwg) << 32) |
+                        (0xFFFFFFFFu - static_cast<unsigned>(labwg) * 256u);
+    }
+    apply_prev<JB>(i);
+    if (nv > 0) {
+      for (;;) {
+        bool ok = lane >= nv || (static_cast<unsigned>(kw0) == tag &&
+                                 static_cast<unsigned>(kw1) == tag);
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+          ok = ok && (!rl || j >= nv ||
+                      (static_cast<unsigned>(va[j]) == tag && static_cast<unsigned>(vb[j]) == tag));
+        if (__all(ok)) break;  // wave-uniform loop: the round counter stays scalar
+        if (!keep_waiting(sy.keys, base + 1u, rounds)) {
+          gave = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        issue();
+      }
+      if (!gave) {
+        unsigned kb;
+        const unsigned kj = lane < nv ? static_cast<unsigned>(kw0 >> 32) : 0u;
+        const unsigned pj = lane < nv ? static_cast<unsigned>(kw1 >> 32) : 0xFFFFFFFFu;
+        const int jb = lanes_winner<S>(kj, pj, lane, kb);
+        if (jb >= 0) {
+          const unsigned pk = __builtin_amdgcn_readlane(pj, jb);
+          if (rl) {
+            const float x0 = __builtin_bit_cast(float, static_cast<unsigned>(reg_get(va, jb) >> 32));
+            const float x1 = __builtin_bit_cast(float, static_cast<unsigned>(reg_get(vb, jb) >> 32));
+            const int c0 = 2 * lane;
+            *reinterpret_cast<float2*>(&wrow[par][wave][c0]) =
+                make_float2(c0 > i ? x0 : 0.f, c0 + 1 > i ? x1 : 0.f);
+            if (c0 == i) pval[par][wave] = x0;
+            if (c0 + 1 == i) pval[par][wave] = x1;
+          }
+          if (lane == 0)
+            __hip_atomic_fetch_max(&best[2 + par], (static_cast<u64>(kb) << 32) | (0xFFFFFFFFu - pk),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      } else if (lane == 0) {
+        *gaveup = 1;
+      }
+    }
+    __syncthreads();
+    // 4. the winner, the same in every workgroup
+    if (*gaveup) return true;
+    const u64 gb = best[2 + par];
+    if (static_cast<unsigned>(gb >> 32) == 0u) return true;  // no candidate (m >= nb: never)
+    const unsigned gp = 0xFFFFFFFFu - static_cast<unsigned>(gb);
+    const int p = static_cast<int>(gp >> 8);  // label of the pivot row
+    const int qw = static_cast<int>(gp & 255u) % NW;
+    const float pv = pval[par][qw];
+    if (w == 0 && wave == 0 && lane == 0) piv[i] = p - i;
+    const bool stop = pv == 0.f;
+    if (stop && w == 0 && wave == 0 && lane == 0) atomicCAS(info, 0, info_base + i + 1);
+    lbl = lbl == i ? p : (lbl == p ? i : lbl);
+    qwp = qw;
+    if (stop) return true;
+    // 5. multiplier, and the update of column i + 1 for the next arg-max
+    constexpr int JN = JB + 1 < NB_MAX / 16 ? JB + 1 : JB;
+    if (lbl > i && lbl != INT_MAX) {
+      const float l = cur * (1.f / pv);
+      blk[JB][ii] = l;
+      lprev = l;
+      if (i + 1 < nb) {
+        const float un = wrow[par][qw][i + 1];
+        if (ii < 15) {
+          const float y = fmaf(-l, un, blk[JB][min(ii + 1, 15)]);
+          blk[JB][min(ii + 1, 15)] = y;
+          cur = y;
+        } else {
+          blk[JN][0] = fmaf(-l, un, blk[JN][0]);
+          cur = blk[JN][0];
+        }
+      }
+    } else {
+      cur = ii < 15 ? blk[JB][min(ii + 1, 15)] : blk[JN][0];
+    }
+    return false;
+  }
+
+  template <int JB>
+  __device__ __forceinline__ bool block() {
+    const int ie = min(16, nb - 16 * JB);
+    for (int ii = 0; ii < ie; ++ii)
+      if (step<JB>(ii)) return true;
+    return false;
+  }
+};
+
+template <int NW, int S>
+__global__ __launch_bounds__(NW * 64, 1) void getf2_rows_kernel(int m, int nb,
+                                                                float* __restrict__ P, int64_t ld,
+                                                                int32_t* __restrict__ piv,
+                                                                int32_t* __restrict__ info,
+                                                                PanelSync sy, unsigned base,
+                                                                int info_base) {
+  __shared__ __attribute__((aligned(16))) float stage[NW][NB_MAX + 4];
+  __shared__ __attribute__((aligned(16))) float wrow[2][NW][NB_MAX];
+  __shared__ float pval[2][NW];
+  __shared__ unsigned long long best[4];
+  __shared__ int gaveup;
+
+  RowsPanel<NW, S> st;
+  st.m = m;
+  st.nb = nb;
+  st.W = gridDim.x;
+  st.w = blockIdx.x;
+  const int tid = threadIdx.x;
+  st.lane = tid & 63;
+  st.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  st.row = st.w * 64 * NW + 64 * st.wave + st.lane;
+  st.base = base;
+  st.rounds = 0;
+  st.qwp = 0;
+  st.piv = piv;
+  st.info = info;
+  st.info_base = info_base;
+  st.sy = sy;
+  st.stage = stage;
+  st.wrow = wrow;
+  st.pval = pval;
+  st.best = best;
+  st.gaveup = &gaveup;
+
+  if (st.w == 0 && tid == 0) {
+    // ordered before step 0's publication, which every workgroup waits for before its
+    // final atomicAdd on the counter
+    __hip_atomic_store(sy.list_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_thread_fence(__ATOMIC_RELEASE);
+  }
+  // an earlier panel already hit a zero pivot: leave everything as it is (empty list)
+  if (__builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+    return;
+  if (tid == 0) {
+    gaveup = 0;
+    for (int k = 0; k < 4; ++k) best[k] = 0;
+  }
+
+  const int row = st.row;
+  const bool in = row < m;
+  st.lbl = in ? row : INT_MAX;
+  st.lprev = 0.f;
+  float* prow = P + (int64_t)(in ? row : 0) * ld;
+  const bool vec = nb == NB_MAX && (ld & 3) == 0 && (reinterpret_cast<uintptr_t>(P) & 15) == 0;
+  if (vec) {
+#pragma unroll
+    for (int q = 0; q < NB_MAX / 4; ++q) {
+      const f4v x = in ? reinterpret_cast<const f4v*>(prow)[q] : f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) st.blk[q >> 2][(q & 3) * 4 + e] = x[e];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NB_MAX; ++c) st.blk[c >> 4][c & 15] = (in && c < nb) ? prow[c] : 0.f;
+  }
+  st.cur = st.blk[0][0];
+  __syncthreads();  // the LDS words above are cleared before any wave uses them
+
+  // uniform early exits between the blocks: a zero pivot or a wait that gave up
+  bool stop = st.template block<0>();
+  if (!stop && nb > 16) stop = st.template block<1>();
+  if (!stop && nb > 32) stop = st.template block<2>();
+  if (!stop && nb > 48) stop = st.template block<3>();
+  if (!stop && nb > 64) stop = st.template block<4>();
+  if (!stop && nb > 80) stop = st.template block<5>();
+  if (!stop && nb > 96) stop = st.template block<6>();
+  if (!stop && nb > 112) stop = st.template block<7>();
+
+  // every row to the position of its label, and the row movement list (dst <- src)
+  const int lb = st.lbl;
+  if (lb != INT_MAX) {
+    float* dst = P + (int64_t)lb * ld;
+    if (vec) {
+#pragma unroll
+      for (int q = 0; q < NB_MAX / 4; ++q) {
+        f4v x;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[e] = st.blk[q >> 2][(q & 3) * 4 + e];
+        reinterpret_cast<f4v*>(dst)[q] = x;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB_MAX; ++c)
+        if (c < nb) dst[c] = st.blk[c >> 4][c & 15];
+    }
+    if (lb != row) {
+      const int slot = atomicAdd(sy.list_count, 1);
+      sy.list[2 * slot] = lb;
+      sy.list[2 * slot + 1] = row;
+    }
+  }
+}
+
 // Applies the (dst <- src) row list of a panel to columns [0, ncols) of A (rows relative to
 // row0), skipping columns [skip_lo, skip_hi).  The moved rows form a permutation among
 // themselves, so each workgroup owns LW columns outright: it reads every source row of its
@@ -673,6 +1101,42 @@ int launch_panel_r(int m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
   return 0;
 }
 
+// The FP32 row-per-lane panel: W = ceil(m / 512) workgroups of 8 waves, one per CU; wave v
+// reads slots v, v + 8, ..., so S (a power of two >= W / 8) slots per wave and W <= 64.
+constexpr int ROWS_NW = 8;
+constexpr int ROWS_SMAX = 8;
+
+template <int S>
+int launch_rows_s(int W, int m, int nb, float* P, int64_t ld, int32_t* piv, int32_t* info,
+                  const PanelSync& sy, unsigned base, int info_base, hipStream_t s) {
+  hipLaunchKernelGGL((getf2_rows_kernel<ROWS_NW, S>), dim3(W), dim3(64 * ROWS_NW), 0, s, m, nb, P,
+                     ld, piv, info, sy, base, info_base);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_rows_panel(int m, int nb, float* P, int64_t ld, int32_t* piv, int32_t* info,
+                      const PanelSync& sy, unsigned base, int info_base, hipStream_t s) {
+  const int W = ceil_div(m, 64 * ROWS_NW);
+  const int S = ceil_div(W, ROWS_NW);
+  if (S <= 1) return launch_rows_s<1>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if (S <= 2) return launch_rows_s<2>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if (S <= 4) return launch_rows_s<4>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  return launch_rows_s<8>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+}
+
+// The row-per-lane panel takes FP32 panels of up to 64 workgroups (m <= 32768) that fit on
+// the device at once; DDLA_FP32_PANEL=labels selects the column-per-lane kernel instead.
+bool rows_panel_fits(int64_t m) {
+  static int mode = -1;
+  if (mode < 0) {
+    const char* e = getenv("DDLA_FP32_PANEL");
+    mode = (e && strcmp(e, "labels") == 0) ? 0 : 1;
+  }
+  const int64_t W = (m + 64 * ROWS_NW - 1) / (64 * ROWS_NW);
+  return mode == 1 && W <= ROWS_NW * ROWS_SMAX && W <= ctx().num_cus;
+}
+
 template <typename T, int R>
 int max_coresident() {
   static int cached = -1;
@@ -749,6 +1213,10 @@ int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int
   }
   ProfScope prof(sizeof(T) == 8 ? "getf2_panel_f64" : "getf2_panel_f32",
                  (double)m * nb * nb - (double)nb * nb * nb / 3.0, 2.0 * sizeof(T) * m * nb, s);
+  if constexpr (sizeof(T) == 4) {
+    if (rows_panel_fits(m))
+      return launch_rows_panel((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
+  }
   // fewest workgroups that hold the panel (fewer slots to read per step)
   if (fits<T, 32>(m, 64)) return launch_panel_r<T, 32>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
   if (fits<T, 64>(m, 128)) return launch_panel_r<T, 64>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);

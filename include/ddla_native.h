@@ -399,6 +399,13 @@ int ddla_dposv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc
  * counterpart (a preconditioner for callers running their own refinement). */
 int ddla_sgetrf_mp_dev(ddla_dim_t n, float* a, ddla_inc_t lda, int32_t prec, int32_t pivoting,
                        int32_t* perm);
+/* The panel of ddla_sgetrf_mp_dev with partial pivoting on its own: PLUDecUnblock
+ * (DDLA/Factorizations/LU.cs:344-374, GetAndApplyPivUnblock's AMax rule) in FP32 on a
+ * row-major m x nb device panel in place, 1 <= nb <= min(m, 128), m <= 32768; relative
+ * pivots (nb entries, device) as ddla_dgetrf_dev.  Returns > 0 (first zero pivot column + 1)
+ * where the reference's panel stops.  The reference has no FP32 path; the panel is exposed for
+ * callers that build their own blocked FP32 LU and for parity tests. */
+int ddla_sgetf2_dev(ddla_dim_t m, ddla_dim_t nb, float* p, ddla_inc_t ldp, int32_t* piv);
 
 /* ------------------------------------------------ batched uneven factorizations ----- */
 /* No counterpart in the reference (DDLA factors one matrix per call: LU.PLUDecompose,

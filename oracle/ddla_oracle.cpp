@@ -636,6 +636,36 @@ int64_t orc_dgetf2(int64_t m, int64_t nb, double* a, int64_t rs, int64_t cs, int
   return plu_unblock(A, piv);
 }
 
+// PLUDecUnblock (LU.cs:344-374) in FP32 on an m x nb panel (element (i, j) at a[i*rs + j*cs]):
+// the same steps as plu_unblock in single precision -- AMax's first index of the largest |x|
+// (NaN below every number), a physical row swap, the column scaled by the reciprocal of the
+// pivot (InvScal, 1v.cs:102), then GeR as one fused multiply-add per element.  The FP32 panel
+// of the mixed-precision LU has no counterpart in the reference; this pins its arithmetic.
+int64_t orc_sgetf2(int64_t m, int64_t nb, float* a, int64_t rs, int64_t cs, int32_t* piv) {
+  auto at = [&](int64_t i, int64_t j) -> float& { return a[i * rs + j * cs]; };
+  const int64_t len = std::min(m, nb);
+  for (int64_t i = 0; i < len; ++i) {
+    int64_t ip = i;
+    float bv = -1.0f;
+    for (int64_t r = i; r < m; ++r) {
+      const float v = std::fabs(at(r, i));
+      if (v > bv) { bv = v; ip = r; }
+    }
+    if (ip != i)
+      for (int64_t c = 0; c < nb; ++c) std::swap(at(i, c), at(ip, c));
+    piv[i] = static_cast<int32_t>(ip - i);
+    const float a11 = at(i, i);
+    if (a11 == 0.0f) return i + 1;
+    const float r = 1.0f / a11;
+    for (int64_t rr = i + 1; rr < m; ++rr) at(rr, i) *= r;
+    for (int64_t rr = i + 1; rr < m; ++rr) {
+      const float l = at(rr, i);
+      for (int64_t c = i + 1; c < nb; ++c) at(rr, c) = std::fmaf(-l, at(i, c), at(rr, c));
+    }
+  }
+  return 0;
+}
+
 // LUDecompose without pivoting (LU.cs:133-142): blocked iff n > block.
 int64_t orc_dgetrf_nopiv(int64_t n, double* a, int64_t rs, int64_t cs) {
   MV A{a, n, n, rs, cs};

@@ -54,6 +54,8 @@ def lib():
         L.orc_dgetf2.restype = i64
         fp = ctypes.POINTER(ctypes.c_float)
         L.orc_stslu.argtypes = [i64, fp, i64, ip, fp, ip, ip]
+        L.orc_sgetf2.argtypes = [i64, i64, fp, i64, i64, ip]
+        L.orc_sgetf2.restype = i64
         L.orc_stslu.restype = i
         L.orc_dgetrf_nopiv.argtypes = [i64, dp, i64, i64]
         L.orc_dgetrf_nopiv.restype = i64
@@ -219,6 +221,18 @@ def getf2(P):
     m, nb = P.shape
     piv = np.zeros(min(m, nb), dtype=np.int32)
     info = lib().orc_dgetf2(m, nb, _dp(P), *_strides(P), _ip(piv))
+    return int(info), piv
+
+
+def sgetf2(P):
+    """PLUDecUnblock (LU.cs:344-374) in FP32 on an m x nb float32 panel in place (the mixed
+    LU's panel; the reference has no FP32 path).  Returns (info, relative pivots)."""
+    assert P.dtype == np.float32
+    m, nb = P.shape
+    piv = np.zeros(min(m, nb), dtype=np.int32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    info = lib().orc_sgetf2(m, nb, P.ctypes.data_as(fp), P.strides[0] // 4, P.strides[1] // 4,
+                            _ip(piv))
     return int(info), piv
 
 
