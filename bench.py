@@ -103,6 +103,35 @@ def time_gemm(ddla, n, reps):
     return {"n": n, "ms": dt * 1e3, "tflops": tf / 1e12, "frac_fp64_peak": tf / PEAK["f64"]}
 
 
+def vendor_reference(torch, A_t, b_t, n):
+    """The vendor libraries on the same system, through torch (reference points, not the product
+    path): torch.linalg.solve (rocSOLVER getrf + getrs, FP64) on A, and the FP64 GEMM behind
+    torch.matmul at 1024^3 and 8192^3, timed with HIP events."""
+    def timed(f, reps):
+        f()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            f()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps
+    out = {"library": f"torch {torch.__version__} (rocSOLVER, hipBLASLt / rocBLAS)"}
+    out["linalg_solve_ms"] = timed(lambda: torch.linalg.solve(A_t, b_t), 2)
+    gem = []
+    for m in (1024, 8192):
+        a = torch.randn(m, m, dtype=torch.float64, device="cuda")
+        b = torch.randn(m, m, dtype=torch.float64, device="cuda")
+        ms = timed(lambda: a @ b, 20 if m == 1024 else 3)
+        tf = 2.0 * m ** 3 / ms / 1e9
+        gem.append({"n": m, "ms": ms, "tflops": tf, "frac_fp64_peak": tf * 1e12 / PEAK["f64"]})
+        del a, b
+    out["dgemm"] = gem
+    torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(name, n):
     """HBM bytes per launch of a kernel group from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, scaled against a device copy), when it covers this workload."""
@@ -577,6 +606,8 @@ def main():
                          "(--warmup + --steps times; for profiler runs) and print the timing")
     ap.add_argument("--breakdown", action="store_true",
                     help="with --solve-only: also time one solve per kernel (event profiling)")
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip the vendor-library reference (torch.linalg.solve, torch DGEMM)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other mixed configurations (BF16, 64 right-hand sides, "
                          "Cholesky), one timed solve each")
@@ -742,6 +773,11 @@ def main():
         log(f"kappa={kap:g}: {ts * 1e3:.1f} ms iters={si.iters} berr={si.backward_error:.2e}")
 
     # ---- the plain FP64 LU + solve configuration (n = 8192) on the GPU
+    vendor = None if args.no_vendor else vendor_reference(torch, A_t, b_t, n)
+    if vendor:
+        vendor["speedup_of_mixed_vs_linalg_solve"] = vendor["linalg_solve_ms"] / (t_mixed * 1e3)
+        vendor["speedup_of_fp64_lu_vs_linalg_solve"] = vendor["linalg_solve_ms"] / (t_fp64 * 1e3)
+        log("vendor", vendor)
     c2, piv_c2 = bench_c2(ddla, 8192)
     log("fp64 LU n=8192", c2)
     ldlt_qr = None if args.no_extras else bench_ldlt_qr(ddla, 8192,
@@ -795,6 +831,7 @@ def main():
         "speedup_vs_fp64_lu": t_fp64 / t_mixed,
         "gemm_fp64": gemm,
         "gemm_fp64_cpu_oracle": cpu_c1,
+        "vendor_fp64": vendor,
         "fp64_lu_n8192": c2,
         "ldlt_qr_n8192": ldlt_qr,
         "kernel_breakdown": breakdown,

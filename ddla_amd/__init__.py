@@ -327,6 +327,19 @@ def _zero_pivot(info: int):
     raise ZeroDivisionError(f"zero pivot at step {info - 1}")
 
 
+def PLUDecUnblock(A: "DeviceMatrix"):
+    """PLUDecUnblock (LU.cs:344-374) on a device panel in place: A is m x nb with
+    1 <= nb <= min(m, 128).  Returns the relative pivots (int32[nb]); raises on a zero pivot
+    like the reference (the panel stops there)."""
+    m, nb = A.rows, A.cols
+    dpiv = _DevInts(nb)
+    info = check(_lib().ddla_dgetf2_dev(m, nb, A.ptr, A.ld, dpiv.ptr))
+    piv = dpiv.to_host()
+    if info:
+        _zero_pivot(info)
+    return piv
+
+
 def PLUDecompose(A, pivs=None):
     """Blocked PLU in place (LU.cs:251-264). A: numpy (any strides) or DeviceMatrix.
 

@@ -241,6 +241,9 @@ public static class HipNative
     [DllImport(Lib, EntryPoint = "ddla_sgetf2_dev")]
     public static extern int ddla_sgetf2_dev(long m, long nb, IntPtr p, long ldp, IntPtr piv);
 
+    [DllImport(Lib, EntryPoint = "ddla_dgetf2_dev")]
+    public static extern int ddla_dgetf2_dev(long m, long nb, IntPtr p, long ldp, IntPtr piv);
+
     [DllImport(Lib, EntryPoint = "ddla_batch_create")]
     public static extern int ddla_batch_create(out IntPtr plan, int prec, long count, IntPtr n, IntPtr kind, IntPtr a, IntPtr piv, IntPtr info, int tune);
 

@@ -581,7 +581,7 @@ __device__ __forceinline__ int lanes_winner(unsigned key, unsigned tie, int lane
   return __ffsll(__ballot(lane < NL && key == kmax && tie == t)) - 1;
 }
 
-template <int NW, int S>
+template <int NW, int S, bool TWO>
 struct RowsPanel {
   // uniform over the workgroup
   int m, nb, W, w, wave, lane, row;
@@ -696,91 +696,182 @@ struct RowsPanel {
     bool gave = false;
     // 2. publish (several workgroups) or stage (one): the candidate row with its pending
     //    update, from the wave that holds it
-    const int nv = W > 1 && wave < W ? (W - wave + NW - 1) / NW : 0;  // slots this wave reads
-    const bool rl = lane >= 8 * JB;  // lanes that hold columns from block JB on
-    const u64* area = sy.cand + (size_t)par * WMAX * RS_WORDS;
-    u64 kw0 = 0, kw1 = 0, va[S], vb[S];
-    // every lane reads its pair of every slot (lanes < 8 JB too: their words are not checked)
-    auto issue = [&]() {
-      const u64* ks = area + (size_t)(wave + NW * min(lane, nv - 1)) * RS_WORDS;
-      kw0 = get(ks);
-      kw1 = get(ks + 1);
-#pragma unroll
-      for (int j = 0; j < S; ++j) {
-        const u64* vs = area + (size_t)(wave + NW * min(j, nv - 1)) * RS_WORDS + 8 + 2 * lane;
-        va[j] = get(vs);
-        vb[j] = get(vs + 1);
-      }
-    };
-    if (W > 1) {
-      if (holder || (!has && wave == 0)) {
-        const float2 x = staged_pair<JB>(i, has, labwg);
-        u64* slot = sy.cand + ((size_t)par * WMAX + w) * RS_WORDS;
-        if (rl) {
-          put(slot + 8 + 2 * lane, __builtin_bit_cast(unsigned, x.x), tag);
-          put(slot + 9 + 2 * lane, __builtin_bit_cast(unsigned, x.y), tag);
-        }
-        if (lane == 0) {
-          put(slot, kwg, tag);
-          put(slot + 1, static_cast<unsigned>(has ? labwg * 256 + w : INT_MAX), tag);
-        }
-      }
-      // 3. the first reads of this wave's slots go out before the pending update
-      if (nv > 0) issue();
-    } else {
-      if (holder) {
-        const float2 x = staged_pair<JB>(i, true, labwg);
-        if (rl) {
-          const int c0 = 2 * lane;
-          *reinterpret_cast<float2*>(&wrow[par][0][c0]) =
-              make_float2(c0 > i ? x.x : 0.f, c0 + 1 > i ? x.y : 0.f);
-          if (c0 == i) pval[par][0] = x.x;
-          if (c0 + 1 == i) pval[par][0] = x.y;
-        }
-      }
-      if (has && wave == 0 && lane == 0)
-        best[2 + par] = (static_cast<u64>(kwg) << 32) |
-                        (0xFFFFFFFFu - static_cast<unsigned>(labwg) * 256u);
-    }
-    apply_prev<JB>(i);
-    if (nv > 0) {
-      for (;;) {
-        bool ok = lane >= nv || (static_cast<unsigned>(kw0) == tag &&
-                                 static_cast<unsigned>(kw1) == tag);
-#pragma unroll
-        for (int j = 0; j < S; ++j)
-          ok = ok && (!rl || j >= nv ||
-                      (static_cast<unsigned>(va[j]) == tag && static_cast<unsigned>(vb[j]) == tag));
-        if (__all(ok)) break;  // wave-uniform loop: the round counter stays scalar
-        if (!keep_waiting(sy.keys, base + 1u, rounds)) {
-          gave = true;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        issue();
-      }
-      if (!gave) {
-        unsigned kb;
-        const unsigned kj = lane < nv ? static_cast<unsigned>(kw0 >> 32) : 0u;
-        const unsigned pj = lane < nv ? static_cast<unsigned>(kw1 >> 32) : 0xFFFFFFFFu;
-        const int jb = lanes_winner<S>(kj, pj, lane, kb);
-        if (jb >= 0) {
-          const unsigned pk = __builtin_amdgcn_readlane(pj, jb);
+    if constexpr (TWO) {
+      // 3. (two-phase exchange) wave 0 reads every slot's key (lane j: slot j), then the
+      //    winner's row into wrow slot 0
+      const bool poller = W > 1 && wave == 0;
+      const bool rl = lane >= 8 * JB;        // lanes that publish columns from block JB on
+      const bool rf = 2 * lane + 1 >= i;     // lanes that fetch column i or a later one
+      const u64* area = sy.cand + (size_t)par * WMAX * RS_WORDS;
+      u64 kw0 = 0, kw1 = 0;
+      auto issue = [&]() {
+        const u64* ks = area + (size_t)min(lane, W - 1) * RS_WORDS;
+        kw0 = get(ks);
+        kw1 = get(ks + 1);
+      };
+      if (W > 1) {
+        if (holder || (!has && wave == 0)) {
+          const float2 x = staged_pair<JB>(i, has, labwg);
+          u64* slot = sy.cand + ((size_t)par * WMAX + w) * RS_WORDS;
           if (rl) {
-            const float x0 = __builtin_bit_cast(float, static_cast<unsigned>(reg_get(va, jb) >> 32));
-            const float x1 = __builtin_bit_cast(float, static_cast<unsigned>(reg_get(vb, jb) >> 32));
-            const int c0 = 2 * lane;
-            *reinterpret_cast<float2*>(&wrow[par][wave][c0]) =
-                make_float2(c0 > i ? x0 : 0.f, c0 + 1 > i ? x1 : 0.f);
-            if (c0 == i) pval[par][wave] = x0;
-            if (c0 + 1 == i) pval[par][wave] = x1;
+            put(slot + 8 + 2 * lane, __builtin_bit_cast(unsigned, x.x), tag);
+            put(slot + 9 + 2 * lane, __builtin_bit_cast(unsigned, x.y), tag);
           }
-          if (lane == 0)
-            __hip_atomic_fetch_max(&best[2 + par], (static_cast<u64>(kb) << 32) | (0xFFFFFFFFu - pk),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (lane == 0) {
+            put(slot, kwg, tag);
+            put(slot + 1, static_cast<unsigned>(has ? labwg * 256 + w : INT_MAX), tag);
+          }
         }
-      } else if (lane == 0) {
-        *gaveup = 1;
+        if (poller) issue();
+      } else {
+        if (holder) {
+          const float2 x = staged_pair<JB>(i, true, labwg);
+          if (rl) {
+            const int c0 = 2 * lane;
+            *reinterpret_cast<float2*>(&wrow[par][0][c0]) =
+                make_float2(c0 > i ? x.x : 0.f, c0 + 1 > i ? x.y : 0.f);
+            if (c0 == i) pval[par][0] = x.x;
+            if (c0 + 1 == i) pval[par][0] = x.y;
+          }
+        }
+        if (has && wave == 0 && lane == 0)
+          best[2 + par] = (static_cast<u64>(kwg) << 32) |
+                          (0xFFFFFFFFu - static_cast<unsigned>(labwg) * 256u);
+      }
+      apply_prev<JB>(i);
+      if (poller) {
+        while (!__all(lane >= W || (static_cast<unsigned>(kw0) == tag &&
+                                    static_cast<unsigned>(kw1) == tag))) {
+          if (!keep_waiting(sy.keys, base + 1u, rounds)) {
+            gave = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          issue();
+        }
+        if (!gave) {
+          unsigned kb;
+          const unsigned kj = lane < W ? static_cast<unsigned>(kw0 >> 32) : 0u;
+          const unsigned pj = lane < W ? static_cast<unsigned>(kw1 >> 32) : 0xFFFFFFFFu;
+          const int jb = lanes_winner<64>(kj, pj, lane, kb);
+          if (jb >= 0) {
+            const unsigned pk = __builtin_amdgcn_readlane(pj, jb);
+            const u64* sl = area + (size_t)(pk & 255u) * RS_WORDS;
+            const u64* vs = rf ? sl + 8 + 2 * lane : sl;
+            u64 a0 = get(vs), a1 = get(vs + 1);
+            while (!__all(!rf || (static_cast<unsigned>(a0) == tag &&
+                                  static_cast<unsigned>(a1) == tag))) {
+              if (!keep_waiting(sy.keys, base + 1u, rounds)) {
+                gave = true;
+                break;
+              }
+              __builtin_amdgcn_s_sleep(1);
+              a0 = get(vs);
+              a1 = get(vs + 1);
+            }
+            if (!gave) {
+              if (rf) {
+                const float x0 = __builtin_bit_cast(float, static_cast<unsigned>(a0 >> 32));
+                const float x1 = __builtin_bit_cast(float, static_cast<unsigned>(a1 >> 32));
+                const int c0 = 2 * lane;
+                *reinterpret_cast<float2*>(&wrow[par][0][c0]) =
+                    make_float2(c0 > i ? x0 : 0.f, c0 + 1 > i ? x1 : 0.f);
+                if (c0 == i) pval[par][0] = x0;
+                if (c0 + 1 == i) pval[par][0] = x1;
+              }
+              if (lane == 0) best[2 + par] = (static_cast<u64>(kb) << 32) | (0xFFFFFFFFu - pk);
+            }
+          }
+        }
+        if (gave && lane == 0) *gaveup = 1;
+      }
+    } else {
+      const int nv = W > 1 && wave < W ? (W - wave + NW - 1) / NW : 0;  // slots this wave reads
+      const bool rl = lane >= 8 * JB;  // lanes that hold columns from block JB on
+      const u64* area = sy.cand + (size_t)par * WMAX * RS_WORDS;
+      u64 kw0 = 0, kw1 = 0, va[S], vb[S];
+      // every lane reads its pair of every slot (lanes < 8 JB too: their words are not checked)
+      auto issue = [&]() {
+        const u64* ks = area + (size_t)(wave + NW * min(lane, nv - 1)) * RS_WORDS;
+        kw0 = get(ks);
+        kw1 = get(ks + 1);
+  #pragma unroll
+        for (int j = 0; j < S; ++j) {
+          const u64* vs = area + (size_t)(wave + NW * min(j, nv - 1)) * RS_WORDS + 8 + 2 * lane;
+          va[j] = get(vs);
+          vb[j] = get(vs + 1);
+        }
+      };
+      if (W > 1) {
+        if (holder || (!has && wave == 0)) {
+          const float2 x = staged_pair<JB>(i, has, labwg);
+          u64* slot = sy.cand + ((size_t)par * WMAX + w) * RS_WORDS;
+          if (rl) {
+            put(slot + 8 + 2 * lane, __builtin_bit_cast(unsigned, x.x), tag);
+            put(slot + 9 + 2 * lane, __builtin_bit_cast(unsigned, x.y), tag);
+          }
+          if (lane == 0) {
+            put(slot, kwg, tag);
+            put(slot + 1, static_cast<unsigned>(has ? labwg * 256 + w : INT_MAX), tag);
+          }
+        }
+        // 3. the first reads of this wave's slots go out before the pending update
+        if (nv > 0) issue();
+      } else {
+        if (holder) {
+          const float2 x = staged_pair<JB>(i, true, labwg);
+          if (rl) {
+            const int c0 = 2 * lane;
+            *reinterpret_cast<float2*>(&wrow[par][0][c0]) =
+                make_float2(c0 > i ? x.x : 0.f, c0 + 1 > i ? x.y : 0.f);
+            if (c0 == i) pval[par][0] = x.x;
+            if (c0 + 1 == i) pval[par][0] = x.y;
+          }
+        }
+        if (has && wave == 0 && lane == 0)
+          best[2 + par] = (static_cast<u64>(kwg) << 32) |
+                          (0xFFFFFFFFu - static_cast<unsigned>(labwg) * 256u);
+      }
+      apply_prev<JB>(i);
+      if (nv > 0) {
+        for (;;) {
+          bool ok = lane >= nv || (static_cast<unsigned>(kw0) == tag &&
+                                   static_cast<unsigned>(kw1) == tag);
+  #pragma unroll
+          for (int j = 0; j < S; ++j)
+            ok = ok && (!rl || j >= nv ||
+                        (static_cast<unsigned>(va[j]) == tag && static_cast<unsigned>(vb[j]) == tag));
+          if (__all(ok)) break;  // wave-uniform loop: the round counter stays scalar
+          if (!keep_waiting(sy.keys, base + 1u, rounds)) {
+            gave = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          issue();
+        }
+        if (!gave) {
+          unsigned kb;
+          const unsigned kj = lane < nv ? static_cast<unsigned>(kw0 >> 32) : 0u;
+          const unsigned pj = lane < nv ? static_cast<unsigned>(kw1 >> 32) : 0xFFFFFFFFu;
+          const int jb = lanes_winner<S>(kj, pj, lane, kb);
+          if (jb >= 0) {
+            const unsigned pk = __builtin_amdgcn_readlane(pj, jb);
+            if (rl) {
+              const float x0 = __builtin_bit_cast(float, static_cast<unsigned>(reg_get(va, jb) >> 32));
+              const float x1 = __builtin_bit_cast(float, static_cast<unsigned>(reg_get(vb, jb) >> 32));
+              const int c0 = 2 * lane;
+              *reinterpret_cast<float2*>(&wrow[par][wave][c0]) =
+                  make_float2(c0 > i ? x0 : 0.f, c0 + 1 > i ? x1 : 0.f);
+              if (c0 == i) pval[par][wave] = x0;
+              if (c0 + 1 == i) pval[par][wave] = x1;
+            }
+            if (lane == 0)
+              __hip_atomic_fetch_max(&best[2 + par], (static_cast<u64>(kb) << 32) | (0xFFFFFFFFu - pk),
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else if (lane == 0) {
+          *gaveup = 1;
+        }
       }
     }
     __syncthreads();
@@ -790,7 +881,9 @@ struct RowsPanel {
     if (static_cast<unsigned>(gb >> 32) == 0u) return true;  // no candidate (m >= nb: never)
     const unsigned gp = 0xFFFFFFFFu - static_cast<unsigned>(gb);
     const int p = static_cast<int>(gp >> 8);  // label of the pivot row
-    const int qw = static_cast<int>(gp & 255u) % NW;
+    // wrow slot of the pivot row: the wave that read the owner's slot, or wave 0 (two-phase
+    // exchange, and one workgroup, whose owner is 0)
+    const int qw = TWO ? 0 : static_cast<int>(gp & 255u) % NW;
     const float pv = pval[par][qw];
     if (w == 0 && wave == 0 && lane == 0) piv[i] = p - i;
     const bool stop = pv == 0.f;
@@ -830,7 +923,7 @@ struct RowsPanel {
   }
 };
 
-template <int NW, int S>
+template <int NW, int S, bool TWO>
 __global__ __launch_bounds__(NW * 64, 1) void getf2_rows_kernel(int m, int nb,
                                                                 float* __restrict__ P, int64_t ld,
                                                                 int32_t* __restrict__ piv,
@@ -843,7 +936,7 @@ __global__ __launch_bounds__(NW * 64, 1) void getf2_rows_kernel(int m, int nb,
   __shared__ unsigned long long best[4];
   __shared__ int gaveup;
 
-  RowsPanel<NW, S> st;
+  RowsPanel<NW, S, TWO> st;
   st.m = m;
   st.nb = nb;
   st.W = gridDim.x;
@@ -928,6 +1021,420 @@ __global__ __launch_bounds__(NW * 64, 1) void getf2_rows_kernel(int m, int nb,
         if (c < nb) dst[c] = st.blk[c >> 4][c & 15];
     }
     if (lb != row) {
+      const int slot = atomicAdd(sy.list_count, 1);
+      sy.list[2 * slot] = lb;
+      sy.list[2 * slot + 1] = row;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 panel with two lanes per row (getf2_rows64_kernel): the design of getf2_rows_kernel for
+// FP64, whose 128-column row would not fit one lane's registers.  Lanes 2p and 2p + 1 of wave v
+// of workgroup w hold panel row w * 32 NW + 32 v + p, the even and the odd columns of it
+// (lane parity h: columns 16 B + 2 e + h, 64 values).  Column i lives in the lanes of parity
+// i & 1; the multiplier formed there reaches the partner lane by a DPP swap, and column i + 1,
+// the next arg-max's column, is always in the partner.  Keys are 64 bits (the bits of |x| + 2,
+// NaN 1, no candidate 0), so equal keys are equal magnitudes and ties go to the smallest label
+// exactly as AMax's first index; the waves' and the slots' winners are reduced from small LDS
+// tables.  A slot holds the key, the packed label and the row's 128 values as two tagged words
+// each; the best row a wave reads goes to LDS split by column parity, so a lane's reads of the
+// pivot row are contiguous.  With a single workgroup (m <= 32 NW) nothing is published.
+typedef double d8v __attribute__((ext_vector_type(8)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// slot: word 0 key (high half), 1 key (low half), 2 packed label, 8 + 2c + {0, 1}: value c
+// (high, low); 272 words keep slots on 128-byte lines.  At most 64 workgroups, two parities.
+constexpr int RS64_WORDS = 272;
+constexpr int W64MAX = 64;
+static_assert(2 * (size_t)W64MAX * RS64_WORDS * sizeof(u64) <= PANEL_CAND_BYTES,
+              "FP64 row-panel slots exceed the candidate area");
+
+__device__ __forceinline__ u64 arg_key64(double x, bool cand) {
+  const u64 b = __builtin_bit_cast(u64, x) & 0x7FFFFFFFFFFFFFFFull;
+  return !cand ? 0ull : (b > 0x7FF0000000000000ull ? 1ull : b + 2ull);
+}
+
+// The winner over lanes [0, NL) of 64-bit keys: the largest key (its high half by one DPP max,
+// the low half among the lanes that share it by a second), the smallest tie value among equal
+// keys; returns the lane (-1 when every key is 0) and the key.
+template <int NL>
+__device__ __forceinline__ int lanes_winner64(u64 key, unsigned tie, int lane, u64& kmax) {
+  const unsigned hi = static_cast<unsigned>(key >> 32), lo = static_cast<unsigned>(key);
+  const unsigned mh = lanes_reduce_u32<NL, true>(lane < NL ? hi : 0u);
+  const unsigned ml = lanes_reduce_u32<NL, true>(lane < NL && hi == mh ? lo : 0u);
+  kmax = (static_cast<u64>(mh) << 32) | ml;
+  if (kmax == 0ull) return -1;
+  const unsigned long long bm = __ballot(lane < NL && key == kmax);
+  if (__popcll(bm) == 1) return __ffsll(bm) - 1;
+  const unsigned t = lanes_reduce_u32<NL, false>(lane < NL && key == kmax ? tie : 0xFFFFFFFFu);
+  return __ffsll(__ballot(lane < NL && key == kmax && tie == t)) - 1;
+}
+
+// the value of the partner lane (lane ^ 1): DPP quad_perm [1, 0, 3, 2]
+__device__ __forceinline__ double partner(double x) {
+  const u64 b = __builtin_bit_cast(u64, x);
+  const unsigned lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), 0xB1, 0xf, 0xf, true);
+  const unsigned hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), 0xB1, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, (static_cast<u64>(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ void put64(u64* p, double v, unsigned tag) {
+  const u64 b = __builtin_bit_cast(u64, v);
+  put(p, static_cast<unsigned>(b >> 32), tag);
+  put(p + 1, static_cast<unsigned>(b), tag);
+}
+__device__ __forceinline__ double join64(u64 hi, u64 lo) {
+  return __builtin_bit_cast(double, ((hi >> 32) << 32) | (lo >> 32));
+}
+
+template <int NW, int S>
+struct RowsPanel64 {
+  // uniform over the workgroup (h: per lane)
+  int m, nb, W, w, wave, lane, h, row;
+  unsigned base, rounds;
+  int qwp;  // wrow slot of the previous step's pivot row
+  int32_t* piv;
+  int32_t* info;
+  int info_base;
+  PanelSync sy;
+  double (*stage)[NB_MAX + 2];      // per wave: the raw candidate row, its pending multiplier
+  double (*wrow)[NW][2][NB_MAX / 2];  // [parity][wave][column parity][64]: best row read
+  double (*pval)[NW];               // [parity][wave]: its column-i value
+  u64 (*tkey)[NW];                  // [0]: waves' candidates, [1]: waves' best slots
+  unsigned (*tlab)[NW];
+  int* gaveup;
+  // per lane: its half row (blk[B][e] = column 16 B + 2 e + h), label, column-i value (lanes of
+  // parity i & 1) and the pending multiplier of the previous step
+  d8v blk[NB_MAX / 16];
+  int lbl;
+  double cur, lprev;
+
+  template <int JB>
+  __device__ __forceinline__ void apply_prev(int i) {
+    if (i == 0 || !(lbl >= i && lbl != INT_MAX)) return;
+    const double nl = -lprev;
+    const double* u = wrow[(i - 1) & 1][qwp][h];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const d2v uu = *reinterpret_cast<const d2v*>(u + 8 * JB + 2 * q);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int k = 2 * q + e;
+        const double y = fma(nl, uu[e], blk[JB][k]);
+        blk[JB][k] = 16 * JB + 2 * k + h > i ? y : blk[JB][k];
+      }
+    }
+#pragma unroll
+    for (int B = JB + 1; B < NB_MAX / 16; ++B) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const d2v uu = *reinterpret_cast<const d2v*>(u + 8 * B + 2 * q);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) blk[B][2 * q + e] = fma(nl, uu[e], blk[B][2 * q + e]);
+      }
+    }
+  }
+
+  // The candidate row (label lab, held by a lane pair of this wave) with its pending update:
+  // lane l ends with columns 2l and 2l + 1 (lanes >= 8 JB; zeros when has is false).
+  template <int JB>
+  __device__ __forceinline__ d2v staged_pair(int i, bool has, int lab) {
+    if (has && lbl == lab) {
+#pragma unroll
+      for (int B = JB; B < NB_MAX / 16; ++B)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) stage[wave][16 * B + 2 * e + h] = blk[B][e];
+      if (h == 0) stage[wave][NB_MAX] = lprev;
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the staged row
+    __builtin_amdgcn_wave_barrier();
+    d2v x = {0.0, 0.0};
+    if (has && lane >= 8 * JB) {
+      const int c0 = 2 * lane;
+      x = *reinterpret_cast<const d2v*>(&stage[wave][c0]);
+      if (i > 0) {
+        const double nl = -stage[wave][NB_MAX];
+        const double u0 = wrow[(i - 1) & 1][qwp][0][lane], u1 = wrow[(i - 1) & 1][qwp][1][lane];
+        if (c0 > i) x[0] = fma(nl, u0, x[0]);
+        if (c0 + 1 > i) x[1] = fma(nl, u1, x[1]);
+      }
+    }
+    return x;
+  }
+
+  // the best row of this wave to LDS (columns <= i zeroed) and its column-i value
+  __device__ __forceinline__ void keep_row(int i, int par, int slotw, d2v x) {
+    const int c0 = 2 * lane;
+    wrow[par][slotw][0][lane] = c0 > i ? x[0] : 0.0;
+    wrow[par][slotw][1][lane] = c0 + 1 > i ? x[1] : 0.0;
+    if (c0 == i) pval[par][slotw] = x[0];
+    if (c0 + 1 == i) pval[par][slotw] = x[1];
+  }
+
+  template <int JB>
+  __device__ __forceinline__ bool step(int ii) {
+    const int i = 16 * JB + ii;
+    const int par = i & 1;
+    const unsigned tag = base + static_cast<unsigned>(i) + 1u;
+    const int hi_ = i & 1;  // parity of the lanes holding column i
+    // 1. the workgroup's candidate: the wave's winner, then the waves' table
+    {
+      const bool cand = h == hi_ && lbl >= i && lbl != INT_MAX;
+      u64 kw;
+      const int wl = lanes_winner64<64>(arg_key64(cur, cand), static_cast<unsigned>(lbl), lane, kw);
+      if (lane == 0) {
+        tkey[0][wave] = kw;
+        tlab[0][wave] = wl >= 0 ? static_cast<unsigned>(__builtin_amdgcn_readlane(lbl, wl))
+                                : 0xFFFFFFFFu;
+      }
+    }
+    __syncthreads();
+    u64 kwg;
+    int labwg;
+    {
+      const u64 k = lane < NW ? tkey[0][lane] : 0ull;
+      const unsigned t = lane < NW ? tlab[0][lane] : 0xFFFFFFFFu;
+      const int wq = lanes_winner64<NW>(k, t, lane, kwg);
+      labwg = wq >= 0 ? static_cast<int>(__builtin_amdgcn_readlane(t, wq)) : INT_MAX;
+    }
+    const bool has = kwg != 0ull;
+    const bool holder = __ballot(has && lbl == labwg) != 0;
+    bool gave = false;
+    const int nv = W > 1 && wave < W ? (W - wave + NW - 1) / NW : 0;
+    const bool rl = lane >= 8 * JB;
+    const u64* area = sy.cand + (size_t)par * W64MAX * RS64_WORDS;
+    u64 kw0 = 0, kw1 = 0, kw2 = 0, va[S][4];
+    auto issue = [&]() {
+      const u64* ks = area + (size_t)(wave + NW * min(lane, nv - 1)) * RS64_WORDS;
+      kw0 = get(ks);
+      kw1 = get(ks + 1);
+      kw2 = get(ks + 2);
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const u64* sl = area + (size_t)(wave + NW * min(j, nv - 1)) * RS64_WORDS;
+        const u64* vs = rl ? sl + 8 + 4 * lane : sl;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) va[j][e] = get(vs + (rl ? e : 0));
+      }
+    };
+    // 2. publish (several workgroups) or stage (one)
+    if (W > 1) {
+      if (holder || (!has && wave == 0)) {
+        const d2v x = staged_pair<JB>(i, has, labwg);
+        u64* slot = sy.cand + ((size_t)par * W64MAX + w) * RS64_WORDS;
+        if (rl) {
+          put64(slot + 8 + 4 * lane, x[0], tag);
+          put64(slot + 10 + 4 * lane, x[1], tag);
+        }
+        if (lane == 0) {
+          put(slot, static_cast<unsigned>(kwg >> 32), tag);
+          put(slot + 1, static_cast<unsigned>(kwg), tag);
+          put(slot + 2, static_cast<unsigned>(has ? labwg * 256 + w : INT_MAX), tag);
+        }
+      }
+      // 3. the first reads of this wave's slots go out before the pending update
+      if (nv > 0) issue();
+    } else {
+      if (holder) keep_row(i, par, 0, staged_pair<JB>(i, true, labwg));
+      if (wave == 0 && lane == 0) {
+        tkey[1][0] = kwg;
+        tlab[1][0] = has ? static_cast<unsigned>(labwg) * 256u : 0xFFFFFFFFu;
+        for (int v = 1; v < NW; ++v) tkey[1][v] = 0ull;
+      }
+    }
+    apply_prev<JB>(i);
+    if (nv > 0) {
+      for (;;) {
+        bool ok = lane >= nv || (static_cast<unsigned>(kw0) == tag &&
+                                 static_cast<unsigned>(kw1) == tag &&
+                                 static_cast<unsigned>(kw2) == tag);
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            ok = ok && (!rl || j >= nv || static_cast<unsigned>(va[j][e]) == tag);
+        if (__all(ok)) break;  // wave-uniform loop: the round counter stays scalar
+        if (!keep_waiting(sy.keys, base + 1u, rounds)) {
+          gave = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        issue();
+      }
+      u64 kb = 0ull;
+      unsigned pk = 0xFFFFFFFFu;
+      if (!gave) {
+        const u64 kj = lane < nv ? (((kw0 >> 32) << 32) | (kw1 >> 32)) : 0ull;
+        const unsigned pj = lane < nv ? static_cast<unsigned>(kw2 >> 32) : 0xFFFFFFFFu;
+        const int jb = lanes_winner64<S>(kj, pj, lane, kb);
+        if (jb >= 0) {
+          pk = __builtin_amdgcn_readlane(pj, jb);
+          if (rl) {
+            u64 a[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              a[e] = va[0][e];
+#pragma unroll
+              for (int j = 1; j < S; ++j) a[e] = j == jb ? va[j][e] : a[e];
+            }
+            keep_row(i, par, wave, d2v{join64(a[0], a[1]), join64(a[2], a[3])});
+          }
+        }
+      } else if (lane == 0) {
+        *gaveup = 1;
+      }
+      if (lane == 0) {
+        tkey[1][wave] = kb;
+        tlab[1][wave] = pk;
+      }
+    } else if (W > 1 && lane == 0) {
+      tkey[1][wave] = 0ull;
+      tlab[1][wave] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    // 4. the winner, the same in every workgroup
+    if (*gaveup) return true;
+    u64 gk;
+    const int qw = lanes_winner64<NW>(lane < NW ? tkey[1][lane] : 0ull,
+                                      lane < NW ? tlab[1][lane] : 0xFFFFFFFFu, lane, gk);
+    if (qw < 0) return true;  // no candidate (m >= nb: never)
+    const unsigned gp = tlab[1][qw];
+    const int p = static_cast<int>(gp >> 8);  // label of the pivot row
+    const double pv = pval[par][qw];
+    if (w == 0 && wave == 0 && lane == 0) piv[i] = p - i;
+    const bool stop = pv == 0.0;
+    if (stop && w == 0 && wave == 0 && lane == 0) atomicCAS(info, 0, info_base + i + 1);
+    lbl = lbl == i ? p : (lbl == p ? i : lbl);
+    qwp = qw;
+    if (stop) return true;
+    // 5. multiplier (lanes of parity i & 1, shared with the partner), and column i + 1
+    constexpr int JN = JB + 1 < NB_MAX / 16 ? JB + 1 : JB;
+    const bool act = lbl > i && lbl != INT_MAX;
+    const double lown = act && h == hi_ ? cur * (1.0 / pv) : 0.0;
+    // the swap runs with every lane active (its source is the other lane of the pair; a swap
+    // moved under the branch of the receiving lanes would read inactive lanes as 0)
+    const double lpart = opaque(partner(lown));
+    const double l = h == hi_ ? lown : lpart;
+    if (act) {
+      const int ei = ii >> 1;  // column i: element ei of block JB in the lanes of parity hi_
+      if (h == hi_) blk[JB][ei] = l;
+      lprev = l;
+      if (i + 1 < nb && h != hi_) {
+        const double un = wrow[par][qw][h][(i + 1) >> 1];
+        if (ii < 15) {
+          const int en = (ii + 1) >> 1;
+          const double y = fma(-l, un, blk[JB][en]);
+          blk[JB][en] = y;
+          cur = y;
+        } else {
+          blk[JN][0] = fma(-l, un, blk[JN][0]);
+          cur = blk[JN][0];
+        }
+      }
+    } else if (h != hi_) {
+      cur = ii < 15 ? blk[JB][(ii + 1) >> 1] : blk[JN][0];
+    }
+    return false;
+  }
+
+  template <int JB>
+  __device__ __forceinline__ bool block() {
+    const int ie = min(16, nb - 16 * JB);
+    for (int ii = 0; ii < ie; ++ii)
+      if (step<JB>(ii)) return true;
+    return false;
+  }
+};
+
+template <int NW, int S>
+__global__ __launch_bounds__(NW * 64, 1) void getf2_rows64_kernel(int m, int nb,
+                                                                  double* __restrict__ P,
+                                                                  int64_t ld,
+                                                                  int32_t* __restrict__ piv,
+                                                                  int32_t* __restrict__ info,
+                                                                  PanelSync sy, unsigned base,
+                                                                  int info_base) {
+  __shared__ __attribute__((aligned(16))) double stage[NW][NB_MAX + 2];
+  __shared__ __attribute__((aligned(16))) double wrow[2][NW][2][NB_MAX / 2];
+  __shared__ double pval[2][NW];
+  __shared__ u64 tkey[2][NW];
+  __shared__ unsigned tlab[2][NW];
+  __shared__ int gaveup;
+
+  RowsPanel64<NW, S> st;
+  st.m = m;
+  st.nb = nb;
+  st.W = gridDim.x;
+  st.w = blockIdx.x;
+  const int tid = threadIdx.x;
+  st.lane = tid & 63;
+  st.h = st.lane & 1;
+  st.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  st.row = st.w * 32 * NW + 32 * st.wave + (st.lane >> 1);
+  st.base = base;
+  st.rounds = 0;
+  st.qwp = 0;
+  st.piv = piv;
+  st.info = info;
+  st.info_base = info_base;
+  st.sy = sy;
+  st.stage = stage;
+  st.wrow = wrow;
+  st.pval = pval;
+  st.tkey = tkey;
+  st.tlab = tlab;
+  st.gaveup = &gaveup;
+
+  if (st.w == 0 && tid == 0) {
+    // ordered before step 0's publication, which every workgroup waits for before its
+    // final atomicAdd on the counter
+    __hip_atomic_store(sy.list_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_thread_fence(__ATOMIC_RELEASE);
+  }
+  // an earlier panel already hit a zero pivot: leave everything as it is (empty list)
+  if (__builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+    return;
+  if (tid == 0) gaveup = 0;
+
+  const int row = st.row, h = st.h;
+  const bool in = row < m;
+  st.lbl = in ? row : INT_MAX;
+  st.lprev = 0.0;
+  const double* prow = P + (int64_t)(in ? row : 0) * ld;
+#pragma unroll
+  for (int B = 0; B < NB_MAX / 16; ++B)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int c = 16 * B + 2 * e + h;
+      st.blk[B][e] = (in && c < nb) ? prow[c] : 0.0;
+    }
+  st.cur = st.blk[0][0];  // column 0 (lanes of parity 0)
+  __syncthreads();        // gaveup is cleared before any wave uses it
+
+  bool stop = st.template block<0>();
+  if (!stop && nb > 16) stop = st.template block<1>();
+  if (!stop && nb > 32) stop = st.template block<2>();
+  if (!stop && nb > 48) stop = st.template block<3>();
+  if (!stop && nb > 64) stop = st.template block<4>();
+  if (!stop && nb > 80) stop = st.template block<5>();
+  if (!stop && nb > 96) stop = st.template block<6>();
+  if (!stop && nb > 112) stop = st.template block<7>();
+
+  // every half row to the position of its label, and the row movement list (dst <- src)
+  const int lb = st.lbl;
+  if (lb != INT_MAX) {
+    double* dst = P + (int64_t)lb * ld;
+#pragma unroll
+    for (int B = 0; B < NB_MAX / 16; ++B)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int c = 16 * B + 2 * e + h;
+        if (c < nb) dst[c] = st.blk[B][e];
+      }
+    if (h == 0 && lb != row) {
       const int slot = atomicAdd(sy.list_count, 1);
       sy.list[2 * slot] = lb;
       sy.list[2 * slot + 1] = row;
@@ -1105,36 +1612,84 @@ int launch_panel_r(int m, int nb, T* P, int64_t ld, int32_t* piv, int32_t* info,
 // reads slots v, v + 8, ..., so S (a power of two >= W / 8) slots per wave and W <= 64.
 constexpr int ROWS_NW = 8;
 constexpr int ROWS_SMAX = 8;
+// from this many workgroups on, the two-phase exchange (keys, then the winner's row) beats
+// reading every slot's row with its key
+constexpr int ROWS_TWO_W = 17;
 
-template <int S>
+template <int S, bool TWO>
 int launch_rows_s(int W, int m, int nb, float* P, int64_t ld, int32_t* piv, int32_t* info,
                   const PanelSync& sy, unsigned base, int info_base, hipStream_t s) {
-  hipLaunchKernelGGL((getf2_rows_kernel<ROWS_NW, S>), dim3(W), dim3(64 * ROWS_NW), 0, s, m, nb, P,
-                     ld, piv, info, sy, base, info_base);
+  hipLaunchKernelGGL((getf2_rows_kernel<ROWS_NW, S, TWO>), dim3(W), dim3(64 * ROWS_NW), 0, s, m,
+                     nb, P, ld, piv, info, sy, base, info_base);
   DDLA_HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+// DDLA_FP32_PANEL: "labels" the column-per-lane kernel; "rows1" / "rows2" the row-per-lane
+// kernel with the one-phase / two-phase exchange at every width; unset: the row-per-lane
+// kernel, two-phase exchange from ROWS_TWO_W workgroups on.
+int fp32_panel_mode() {
+  static int mode = -1;
+  if (mode < 0) {
+    const char* e = getenv("DDLA_FP32_PANEL");
+    mode = !e ? 1 : strcmp(e, "labels") == 0 ? 0 : strcmp(e, "rows1") == 0 ? 2
+                  : strcmp(e, "rows2") == 0 ? 3 : 1;
+  }
+  return mode;
 }
 
 int launch_rows_panel(int m, int nb, float* P, int64_t ld, int32_t* piv, int32_t* info,
                       const PanelSync& sy, unsigned base, int info_base, hipStream_t s) {
   const int W = ceil_div(m, 64 * ROWS_NW);
+  const int mode = fp32_panel_mode();
+  const bool two = W > 1 && (mode == 3 || (mode == 1 && W >= ROWS_TWO_W));
+  if (two) return launch_rows_s<1, true>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
   const int S = ceil_div(W, ROWS_NW);
-  if (S <= 1) return launch_rows_s<1>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
-  if (S <= 2) return launch_rows_s<2>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
-  if (S <= 4) return launch_rows_s<4>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
-  return launch_rows_s<8>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if (S <= 1) return launch_rows_s<1, false>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if (S <= 2) return launch_rows_s<2, false>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if (S <= 4) return launch_rows_s<4, false>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  return launch_rows_s<8, false>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
 }
 
 // The row-per-lane panel takes FP32 panels of up to 64 workgroups (m <= 32768) that fit on
-// the device at once; DDLA_FP32_PANEL=labels selects the column-per-lane kernel instead.
+// the device at once.
 bool rows_panel_fits(int64_t m) {
+  const int64_t W = (m + 64 * ROWS_NW - 1) / (64 * ROWS_NW);
+  return fp32_panel_mode() != 0 && W <= ROWS_NW * ROWS_SMAX && W <= ctx().num_cus;
+}
+
+// The FP64 panel with two lanes per row: W = ceil(m / 256) workgroups of 8 waves.  It takes
+// only panels of one workgroup (m <= 256), where it is faster than the column-per-lane kernel
+// (312 vs 404 us at m = 256); with more workgroups its 2 KB slots double the exchange traffic
+// of the FP32 form and it is slower (m = 1024: 439 vs 413 us; 8192: 1033 vs 515 us).  The
+// launch handles up to 32 workgroups (4 slots per wave) for measurements.
+// DDLA_FP64_PANEL=labels selects the column-per-lane kernel everywhere.
+template <int S>
+int launch_rows64_s(int W, int m, int nb, double* P, int64_t ld, int32_t* piv, int32_t* info,
+                    const PanelSync& sy, unsigned base, int info_base, hipStream_t s) {
+  hipLaunchKernelGGL((getf2_rows64_kernel<ROWS_NW, S>), dim3(W), dim3(64 * ROWS_NW), 0, s, m, nb,
+                     P, ld, piv, info, sy, base, info_base);
+  DDLA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_rows64_panel(int m, int nb, double* P, int64_t ld, int32_t* piv, int32_t* info,
+                        const PanelSync& sy, unsigned base, int info_base, hipStream_t s) {
+  const int W = ceil_div(m, 32 * ROWS_NW);
+  const int S = ceil_div(W, ROWS_NW);
+  if (S <= 1) return launch_rows64_s<1>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  if (S <= 2) return launch_rows64_s<2>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+  return launch_rows64_s<4>(W, m, nb, P, ld, piv, info, sy, base, info_base, s);
+}
+
+bool rows64_panel_fits(int64_t m) {
   static int mode = -1;
   if (mode < 0) {
-    const char* e = getenv("DDLA_FP32_PANEL");
+    const char* e = getenv("DDLA_FP64_PANEL");
     mode = (e && strcmp(e, "labels") == 0) ? 0 : 1;
   }
-  const int64_t W = (m + 64 * ROWS_NW - 1) / (64 * ROWS_NW);
-  return mode == 1 && W <= ROWS_NW * ROWS_SMAX && W <= ctx().num_cus;
+  const int64_t W = (m + 32 * ROWS_NW - 1) / (32 * ROWS_NW);
+  return mode == 1 && W <= 1;
 }
 
 template <typename T, int R>
@@ -1216,6 +1771,9 @@ int launch_getf2_panel_ex(int64_t m, int nb, T* P, int64_t ld, int32_t* piv, int
   if constexpr (sizeof(T) == 4) {
     if (rows_panel_fits(m))
       return launch_rows_panel((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
+  } else {
+    if (rows64_panel_fits(m))
+      return launch_rows64_panel((int)m, nb, P, ld, piv, info, sy, base, info_base, s);
   }
   // fewest workgroups that hold the panel (fewer slots to read per step)
   if (fits<T, 32>(m, 64)) return launch_panel_r<T, 32>((int)m, nb, P, ld, piv, info, sy, base, info_base, s);

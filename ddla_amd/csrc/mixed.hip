@@ -1385,6 +1385,25 @@ int ddla_sgetf2_dev(ddla_dim_t m, ddla_dim_t nb, float* p, ddla_inc_t ldp, int32
   return h;
 }
 
+int ddla_dgetf2_dev(ddla_dim_t m, ddla_dim_t nb, double* p, ddla_inc_t ldp, int32_t* piv) {
+  DDLA_RETURN_IF(ensure_init());
+  DDLA_ARG_CHECK(nb >= 1 && nb <= NBLK && m >= nb && ldp >= nb && p && piv,
+                 "bad dgetf2 arguments (m %lld, nb %lld, ld %lld)", (long long)m, (long long)nb,
+                 (long long)ldp);
+  DDLA_ARG_CHECK(m <= 32768, "m = %lld above the supported 32768", (long long)m);
+  hipStream_t s = stream();
+  int32_t* dinfo;
+  int *counts, *lists;
+  DDLA_RETURN_IF(lu_bookkeeping(m, &dinfo, &counts, &lists));
+  DDLA_RETURN_IF(launch_fill_u32(dinfo, 1, 0u, s));
+  DDLA_RETURN_IF(
+      launch_getf2_panel_ex<double>(m, (int)nb, p, ldp, piv, dinfo, 0, lists, counts, s));
+  int32_t h = 0;
+  DDLA_HIP_CHECK(hipMemcpyAsync(&h, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DDLA_RETURN_IF(check_abort(panel_abort_word(nullptr), s, "FP64 panel"));
+  return h;
+}
+
 int ddla_dposv_mpir_dev(ddla_dim_t n, ddla_dim_t nrhs, const double* a, ddla_inc_t lda,
                         const double* b, ddla_inc_t ldb, double* x, ddla_inc_t ldx,
                         const ddla_mpir_opts* opts, ddla_mpir_info* info) {

@@ -117,6 +117,7 @@ SIGNATURES = {
                             ctypes.POINTER(MpirInfo)],
     "ddla_sgetrf_mp_dev": [i64, vp, i64, cint, cint, vp],
     "ddla_sgetf2_dev": [i64, i64, vp, i64, vp],
+    "ddla_dgetf2_dev": [i64, i64, vp, i64, vp],
     "ddla_batch_create": [ctypes.POINTER(vp), cint, i64, vp, vp, vp, vp, vp, cint],
     "ddla_batch_execute": [vp],
     "ddla_batch_describe": [vp, ctypes.c_char_p, ctypes.c_size_t],

@@ -406,6 +406,11 @@ int ddla_sgetrf_mp_dev(ddla_dim_t n, float* a, ddla_inc_t lda, int32_t prec, int
  * where the reference's panel stops.  The reference has no FP32 path; the panel is exposed for
  * callers that build their own blocked FP32 LU and for parity tests. */
 int ddla_sgetf2_dev(ddla_dim_t m, ddla_dim_t nb, float* p, ddla_inc_t ldp, int32_t* piv);
+/* PLUDecUnblock (DDLA/Factorizations/LU.cs:344-374) on its own: the FP64 LU panel of
+ * ddla_dgetrf_dev on a row-major m x nb device panel in place, 1 <= nb <= min(m, 128),
+ * m <= 32768; relative pivots (nb entries, device).  Returns > 0 (first zero pivot column + 1)
+ * where the reference's panel stops. */
+int ddla_dgetf2_dev(ddla_dim_t m, ddla_dim_t nb, double* p, ddla_inc_t ldp, int32_t* piv);
 
 /* ------------------------------------------------ batched uneven factorizations ----- */
 /* No counterpart in the reference (DDLA factors one matrix per call: LU.PLUDecompose,

@@ -94,3 +94,18 @@ def test_first_panel_of_the_fp32_factor(ddla, oracle, n):
     inv = np.empty(n, dtype=np.int64)
     inv[perm] = np.arange(n)
     np.testing.assert_array_equal(LU[inv[idx[nb:]], :nb], P[nb:])
+
+
+# The FP64 panel (PLUDecUnblock, LU.cs:344-374) through the FP64 LU: the pivot sequence of the
+# first panel and its factored columns bit for bit against the oracle's restatement, for one
+# workgroup up to 32 (m <= 8192) and beyond (the column-per-lane kernel).
+@pytest.mark.parametrize("n", [128, 129, 255, 256, 257, 1000, 2049, 4100, 8192, 8320])
+def test_fp64_first_panel_matches_oracle_bitwise(ddla, oracle, n):
+    A = np.random.default_rng(n + 17).uniform(-1, 1, (n, min(n, 128)))
+    P = A.copy()
+    info_o, piv_o = oracle.getf2(P)
+    assert info_o == 0
+    dP = ddla.DeviceMatrix.from_host(np.ascontiguousarray(A))
+    piv = ddla.PLUDecUnblock(dP)
+    np.testing.assert_array_equal(piv, piv_o)
+    np.testing.assert_array_equal(dP.to_host(), P)
