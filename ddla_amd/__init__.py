@@ -62,6 +62,11 @@ def profile_enable(on: bool = True):
     check(_lib().ddla_profile_enable(1 if on else 0))
 
 
+def profile_only(name=None):
+    """Restricts profiling to one kernel name (None: every kernel)."""
+    check(_lib().ddla_profile_only(name.encode() if name else None))
+
+
 def profile_reset():
     check(_lib().ddla_profile_reset())
 

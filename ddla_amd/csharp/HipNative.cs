@@ -82,6 +82,9 @@ public static class HipNative
     [DllImport(Lib, EntryPoint = "ddla_profile_reset")]
     public static extern int ddla_profile_reset();
 
+    [DllImport(Lib, EntryPoint = "ddla_profile_only")]
+    public static extern int ddla_profile_only(string name);
+
     [DllImport(Lib, EntryPoint = "ddla_profile_read")]
     public static extern int ddla_profile_read(int idx, StringBuilder name, nuint len, out long count, out double total_ms, out double flops, out double bytes);
 

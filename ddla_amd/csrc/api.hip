@@ -274,6 +274,12 @@ int ddla_profile_reset(void) {
   return 0;
 }
 
+int ddla_profile_only(const char* name) {
+  API_INIT();
+  profile_only(name);
+  return 0;
+}
+
 int ddla_profile_read(int idx, char* name, size_t len, int64_t* count, double* total_ms,
                       double* flops, double* bytes) {
   API_INIT();

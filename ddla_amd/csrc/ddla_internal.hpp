@@ -139,6 +139,7 @@ class ProfScope {
 };
 void profile_enable(bool on);
 void profile_reset();
+void profile_only(const char* name);
 int profile_read(int idx, char* name, size_t len, int64_t* count, double* ms, double* flops,
                  double* bytes);
 

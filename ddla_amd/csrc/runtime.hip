@@ -158,6 +158,7 @@ struct PendingEvent {
   double flops, bytes;
 };
 bool g_prof_on = false;
+std::string g_prof_only;  // empty: every kernel
 std::vector<ProfEntry> g_prof;
 std::vector<PendingEvent> g_pending;
 std::vector<hipEvent_t> g_event_pool;
@@ -195,7 +196,7 @@ void resolve_pending() {
 bool profiling() { return g_prof_on; }
 
 ProfScope::ProfScope(const char* name, double flops, double bytes, hipStream_t s) : s_(s) {
-  if (!g_prof_on) return;
+  if (!g_prof_on || (!g_prof_only.empty() && g_prof_only != name)) return;
   int idx = -1;
   for (size_t i = 0; i < g_prof.size(); ++i)
     if (g_prof[i].name == name) idx = (int)i;
@@ -222,6 +223,11 @@ ProfScope::~ProfScope() {
 void profile_enable(bool on) {
   resolve_pending();
   g_prof_on = on;
+}
+
+void profile_only(const char* name) {
+  resolve_pending();
+  g_prof_only = name ? name : "";
 }
 
 void profile_reset() {

@@ -52,6 +52,7 @@ SIGNATURES = {
     "ddla_set_wait_limit": [u64],
     "ddla_profile_enable": [cint],
     "ddla_profile_reset": [],
+    "ddla_profile_only": [ctypes.c_char_p],
     "ddla_profile_read": [cint, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(i64),
                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double)],

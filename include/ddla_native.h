@@ -99,6 +99,9 @@ int ddla_profile_enable(int on);
 int ddla_profile_reset(void);
 int ddla_profile_read(int idx, char* name, size_t len, int64_t* count, double* total_ms,
                       double* flops, double* bytes);
+/* Restricts the timing to the launches of one kernel name (null or "": every kernel), so that
+ * a timed region can carry the events of the kernel it is judged by and nothing else. */
+int ddla_profile_only(const char* name);
 
 /* ------------------------------------------------------- device buffers ---------- */
 /* Device-resident matrices (the boundary's replacement for pinned managed arrays). */
