@@ -740,14 +740,31 @@ struct RowsPanel {
       }
       apply_prev<JB>(i);
       if (poller) {
-        while (!__all(lane >= W || (static_cast<unsigned>(kw0) == tag &&
-                                    static_cast<unsigned>(kw1) == tag))) {
+        // While keys are missing, the row of the best key seen so far is read with the next
+        // round of keys: when that candidate wins, its row is already in registers.
+        int qs = -1;  // slot whose row a0 / a1 hold (read in the same round as the keys)
+        u64 a0 = 0, a1 = 0;
+        auto row_of = [&](int q) { return rf ? area + (size_t)q * RS_WORDS + 8 + 2 * lane
+                                             : area + (size_t)q * RS_WORDS; };
+        for (;;) {
+          const bool have = lane >= W || (static_cast<unsigned>(kw0) == tag &&
+                                          static_cast<unsigned>(kw1) == tag);
+          if (__all(have)) break;
           if (!keep_waiting(sy.keys, base + 1u, rounds)) {
             gave = true;
             break;
           }
-          __builtin_amdgcn_s_sleep(1);
+          unsigned kq;
+          const unsigned kj = lane < W && have ? static_cast<unsigned>(kw0 >> 32) : 0u;
+          const unsigned pj = lane < W && have ? static_cast<unsigned>(kw1 >> 32) : 0xFFFFFFFFu;
+          const int jq = lanes_winner<64>(kj, pj, lane, kq);
+          qs = jq >= 0 ? static_cast<int>(__builtin_amdgcn_readlane(pj, jq) & 255u) : -1;
           issue();
+          if (qs >= 0) {
+            const u64* vs = row_of(qs);
+            a0 = get(vs);
+            a1 = get(vs + 1);
+          }
         }
         if (!gave) {
           unsigned kb;
@@ -756,9 +773,12 @@ struct RowsPanel {
           const int jb = lanes_winner<64>(kj, pj, lane, kb);
           if (jb >= 0) {
             const unsigned pk = __builtin_amdgcn_readlane(pj, jb);
-            const u64* sl = area + (size_t)(pk & 255u) * RS_WORDS;
-            const u64* vs = rf ? sl + 8 + 2 * lane : sl;
-            u64 a0 = get(vs), a1 = get(vs + 1);
+            const int q = static_cast<int>(pk & 255u);
+            const u64* vs = row_of(q);
+            if (q != qs) {
+              a0 = get(vs);
+              a1 = get(vs + 1);
+            }
             while (!__all(!rf || (static_cast<unsigned>(a0) == tag &&
                                   static_cast<unsigned>(a1) == tag))) {
               if (!keep_waiting(sy.keys, base + 1u, rounds)) {
