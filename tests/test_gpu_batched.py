@@ -177,3 +177,22 @@ def test_batched_tuning_cache(ddla, oracle, tmp_path, monkeypatch):
     monkeypatch.setenv("DDLA_TUNE_CACHE", "/proc/ddla_no_such_dir/tune.json")
     d4 = ddla.BatchPlan(n, kinds, ddla.PREC_FP64, a.ptr, piv.ptr, info.ptr, tune=1).describe()
     assert d4["from_cache"] is False
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batched_fp32_lu_matches_fp32_oracle_bitwise(ddla, oracle, tune):
+    # The batched LU takes every element through the unblocked elimination's operations in the
+    # same order (substitution and update as fma chains in k order), so in FP32 it reproduces
+    # PLUDecUnblock in FP32 (oracle.sgetf2 on the whole n x n matrix) bit for bit, pivots
+    # included, at every order and panel width.
+    rng = np.random.default_rng(31 + int(tune))
+    sizes = np.array([1, 2, 8, 15, 16, 17, 31, 32, 33, 64, 100, 127, 128, 129, 200, 255, 256])
+    mats = [rng.uniform(-1, 1, (n, n)).astype(np.float32) for n in sizes]
+    kinds = [ddla.BATCH_LU] * len(mats)
+    F, P, info = ddla.batch_factor(mats, kinds, ddla.PREC_FP32, tune=tune)
+    for i, A in enumerate(mats):
+        Q = A.copy()
+        info_o, piv_o = oracle.sgetf2(Q)
+        assert info[i] == info_o == 0
+        np.testing.assert_array_equal(P[i], piv_o, err_msg=f"n={A.shape[0]}")
+        np.testing.assert_array_equal(F[i], Q, err_msg=f"n={A.shape[0]}")
