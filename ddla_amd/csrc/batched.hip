@@ -68,13 +68,13 @@ struct BatchArgs {
 
 template <typename T, int NB>
 struct Smem {
-  T sa[4];
+  unsigned long long sk[4];  // each wave's best key
   int sr[4];
   T prow[NB];            // pivot row of the step (the new row i)
   T srow[NB];            // old row i (moves to the pivot row's place)
-  T dsh;                 // Cholesky: diagonal of the next step
   int stop;
   int orig[NT];          // panel-local source row of each panel row after the interchanges
+  T dsh;                 // Cholesky: diagonal of the next step
   T lcol[NT];            // Cholesky: column i of the panel's rows
   union {
     T Pn[NT][NB + 1];    // Cholesky: the factored panel (row t of the panel in row t)
@@ -85,27 +85,46 @@ struct Smem {
   };
 };
 
-// block-wide arg-max of (a, r) over the 4 waves: largest a, smallest r among equal a
+// Block-wide arg-max of column entries x over the candidate rows (cand): the largest |x|,
+// the smallest row among equal |x|, NaN below every number (AMax picks the first maximum).
+// Integer keys (0 not a candidate, 1 NaN, bits(|x|) + 2) reduce by one DPP max per step; ties
+// are resolved only when several lanes share the maximum.  Returns the key (<= 2: no nonzero
+// candidate) and the row r.
 template <typename T, int NB>
-__device__ __forceinline__ void block_argmax(T& a, int& r, Smem<T, NB>& sm) {
-  wave_argmax_dpp(a, r);
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sm.sa[wave] = a;
-    sm.sr[wave] = r;
+__device__ __forceinline__ unsigned long long block_argmax(T x, bool cand, int& r,
+                                                           Smem<T, NB>& sm) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long kw = 0;
+  int wl;
+  if constexpr (sizeof(T) == 8) {
+    wl = lanes_winner64<64>(arg_key64(x, cand), (unsigned)threadIdx.x, lane, kw);
+  } else {
+    unsigned k = 0;
+    wl = lanes_winner<64>(arg_key(x, cand), (unsigned)threadIdx.x, lane, k);
+    kw = k;
+  }
+  if (lane == 0) {
+    sm.sk[wave] = kw;
+    sm.sr[wave] = 64 * wave + max(wl, 0);
   }
   __syncthreads();
-  a = sm.sa[0];
-  r = sm.sr[0];
+  unsigned long long k[4];
+  int rr[4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {  // the four entries read at once
+    k[w] = sm.sk[w];
+    rr[w] = sm.sr[w];
+  }
+  // waves hold ascending rows: the first wave with the largest key has the smallest row
+  unsigned long long kb = k[0];
+  r = rr[0];
 #pragma unroll
   for (int w = 1; w < 4; ++w) {
-    const T a2 = sm.sa[w];
-    const int r2 = sm.sr[w];
-    if (a2 > a || (a2 == a && r2 < r)) {
-      a = a2;
-      r = r2;
-    }
+    const bool better = k[w] > kb;
+    kb = better ? k[w] : kb;
+    r = better ? rr[w] : r;
   }
+  return kb;
 }
 
 // LU with partial pivoting of one n x n row-major problem (ld = n) at M; relative pivots at
@@ -129,10 +148,9 @@ __device__ __forceinline__ void lu_problem(int n, T* __restrict__ M, int32_t* __
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       if (i < jb && !stop) {
-        T a = (own && tid >= i) ? fmax(fabs(v[i]), T(-1)) : T(-2);
         int p = tid;
-        block_argmax(a, p, sm);
-        if (!(a > T(0))) {  // every candidate is zero (or NaN): zero pivot
+        const unsigned long long kmax = block_argmax(v[i], own && tid >= i, p, sm);
+        if (kmax <= 2ull) {  // every candidate is zero (or NaN): zero pivot
           if (tid == 0) {
             *info = j0 + i + 1;
             sm.stop = 1;
@@ -167,9 +185,11 @@ __device__ __forceinline__ void lu_problem(int n, T* __restrict__ M, int32_t* __
           if (own && tid > i) {
             const T l = v[i] * rinv;
             v[i] = l;
+            // no per-column guard: the columns past jb hold zeros in every row (the pivot
+            // row's too) and are never stored, and a guard would serialize one LDS read per
+            // column behind its own branch
 #pragma unroll
-            for (int c = i + 1; c < NB; ++c)
-              if (c < jb) v[c] = fma(-l, sm.prow[c], v[c]);
+            for (int c = i + 1; c < NB; ++c) v[c] = fma(-l, sm.prow[c], v[c]);
           }
         }
       }
@@ -308,9 +328,10 @@ __device__ __forceinline__ void chol_problem(int n, T* __restrict__ M, int64_t l
           __syncthreads();
           if (own && tid > i) {
             const T xr = -v[i];
+            // unguarded for the same reason as in lu_problem: the entries past jb and above
+            // the diagonal (c > tid) are never stored or read again
 #pragma unroll
-            for (int c = i + 1; c < NB; ++c)
-              if (c < jb && c <= tid) v[c] = fma(sm.lcol[c], xr, v[c]);
+            for (int c = i + 1; c < NB; ++c) v[c] = fma(sm.lcol[c], xr, v[c]);
           }
           if (i + 1 < NB && tid == i + 1) sm.dsh = v[min(i + 1, NB - 1)];
           __syncthreads();

@@ -69,4 +69,58 @@ __device__ __forceinline__ void wave_argmax_dpp(T& a, int& k) {
   lanes_argmax_dpp<64>(a, k);
 }
 
+// max (MAX) or min of v over lanes [0, NL) by DPP, the result in every lane (uniform)
+template <int NL, bool MAX>
+__device__ __forceinline__ unsigned lanes_reduce_u32(unsigned v) {
+  constexpr unsigned id = MAX ? 0u : 0xFFFFFFFFu;
+  auto f = [](unsigned a, unsigned b) { return MAX ? max(a, b) : min(a, b); };
+  if constexpr (NL > 1) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x111, 0xf, 0xf, false));
+  if constexpr (NL > 2) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x112, 0xf, 0xf, false));
+  if constexpr (NL > 4) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x114, 0xf, 0xf, false));
+  if constexpr (NL > 8) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x118, 0xf, 0xf, false));
+  if constexpr (NL > 16) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
+  if constexpr (NL > 32) v = f(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, NL - 1);
+}
+
+// key of a column entry: 0 for rows that are not candidates, 1 for NaN, bits(|x|) + 2
+__device__ __forceinline__ unsigned arg_key(float x, bool cand) {
+  const unsigned b = __builtin_bit_cast(unsigned, x) & 0x7FFFFFFFu;
+  return !cand ? 0u : (b > 0x7F800000u ? 1u : b + 2u);
+}
+
+// The winner over lanes [0, NL): the largest key, among equal keys the smallest tie value
+// (label); returns the winning lane (-1 when every key is 0) and the key.
+template <int NL>
+__device__ __forceinline__ int lanes_winner(unsigned key, unsigned tie, int lane, unsigned& kmax) {
+  kmax = lanes_reduce_u32<NL, true>(key);
+  if (kmax == 0u) return -1;
+  const unsigned long long bm = __ballot(lane < NL && key == kmax);
+  if (__popcll(bm) == 1) return __ffsll(bm) - 1;
+  const unsigned t = lanes_reduce_u32<NL, false>(lane < NL && key == kmax ? tie : 0xFFFFFFFFu);
+  return __ffsll(__ballot(lane < NL && key == kmax && tie == t)) - 1;
+}
+
+__device__ __forceinline__ unsigned long long arg_key64(double x, bool cand) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x) & 0x7FFFFFFFFFFFFFFFull;
+  return !cand ? 0ull : (b > 0x7FF0000000000000ull ? 1ull : b + 2ull);
+}
+
+// The winner over lanes [0, NL) of 64-bit keys: the largest key (its high half by one DPP max,
+// the low half among the lanes that share it by a second), the smallest tie value among equal
+// keys; returns the lane (-1 when every key is 0) and the key.
+template <int NL>
+__device__ __forceinline__ int lanes_winner64(unsigned long long key, unsigned tie, int lane,
+                                              unsigned long long& kmax) {
+  const unsigned hi = static_cast<unsigned>(key >> 32), lo = static_cast<unsigned>(key);
+  const unsigned mh = lanes_reduce_u32<NL, true>(lane < NL ? hi : 0u);
+  const unsigned ml = lanes_reduce_u32<NL, true>(lane < NL && hi == mh ? lo : 0u);
+  kmax = (static_cast<unsigned long long>(mh) << 32) | ml;
+  if (kmax == 0ull) return -1;
+  const unsigned long long bm = __ballot(lane < NL && key == kmax);
+  if (__popcll(bm) == 1) return __ffsll(bm) - 1;
+  const unsigned t = lanes_reduce_u32<NL, false>(lane < NL && key == kmax ? tie : 0xFFFFFFFFu);
+  return __ffsll(__ballot(lane < NL && key == kmax && tie == t)) - 1;
+}
+
 }  // namespace ddla
