@@ -429,24 +429,33 @@ int grid_for(int64_t total) {
 // Measured at 1024^3: 51 us against 57.7 us for the general 64 x 64 kernel.  (Accumulators must
 // stay in VGPRs: __launch_bounds__(256, 2) caps the registers; FP64 MFMAs accumulating in AGPRs
 // issue at less than half the rate on gfx950.)
-template <bool BETA0>
-__global__ __launch_bounds__(256, 2) void dgemm_long_k_kernel(int M, int N, int K, double alpha,
-                                                           const double* __restrict__ A,
-                                                           int64_t lda,
-                                                           const double* __restrict__ B,
-                                                           int64_t ldb, double beta,
-                                                           double* __restrict__ Cm, int64_t ldc,
-                                                           int tiles_m, int tiles_n) {
+//
+// KS = 2 (a grid of no more tiles than CUs, as at 1024^3): 512 threads per tile, waves 0-3
+// take the first half of K and waves 4-7 the second, each half with its own LDS buffers, so
+// every SIMD runs two waves; at the end the second half's accumulators pass through LDS and
+// C = alpha (P0 + P1) (+ beta C), the halves always summed in that order.
+template <bool BETA0, int KS>
+__global__ __launch_bounds__(256 * KS, KS == 1 ? 2 : 1) void dgemm_long_k_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int64_t lda,
+    const double* __restrict__ B, int64_t ldb, double beta, double* __restrict__ Cm,
+    int64_t ldc, int tiles_m, int tiles_n) {
   typedef double v2 __attribute__((ext_vector_type(2)));
   typedef double v4 __attribute__((ext_vector_type(4)));
   constexpr int BM = 64, BN = 64, BK = 32, P = BK + 2;  // [row][k] slices, rows of BK + 2
   constexpr int LQ = BM * BK / 2 / 256;         // 16-byte loads per thread and operand
-  __shared__ __attribute__((aligned(16))) double As[2][BM * P];
-  __shared__ __attribute__((aligned(16))) double Bs[2][BN * P];
+  static_assert(KS == 1 || KS == 2, "one or two K halves");
+  __shared__ __attribute__((aligned(16))) double Ash[KS][2][BM * P];
+  __shared__ __attribute__((aligned(16))) double Bsh[KS][2][BN * P];
+  const int half = KS == 1 ? 0 : (int)(threadIdx.x >> 8);
+  double (&As)[2][BM * P] = Ash[half];
+  double (&Bs)[2][BN * P] = Bsh[half];
   int tmi, tni;
   gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
   const int m0 = tmi * BM, n0 = tni * BN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x & 255, lane = tid & 63, wave = tid >> 6;
+  const int nk = K / BK / KS;                 // K steps of each half
+  A += (int64_t)half * nk * BK;               // this half's K range
+  B += (int64_t)half * nk * BK * ldb;
   const int wm = wave >> 1, wn = wave & 1;
   const int l16 = lane & 15, lk = lane >> 4;
   // loaders: A row ar, k pair ak (v2 along k); B row bk (a k), n pair bn (v2 along n)
@@ -485,7 +494,6 @@ __global__ __launch_bounds__(256, 2) void dgemm_long_k_kernel(int M, int N, int 
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4{0.0, 0.0, 0.0, 0.0};
-  const int nk = K / BK;
   gload(0);
   lstore(0);
   __syncthreads();
@@ -536,6 +544,28 @@ __global__ __launch_bounds__(256, 2) void dgemm_long_k_kernel(int M, int N, int 
     lds_pair(cur ^ 1, 0, 0);
   }
   kstep((nk - 1) & 1, -1);
+  if constexpr (KS == 2) {
+    // the second half's 16 accumulators per thread to LDS (the first half's buffers, free
+    // now), the first half adds them to its own: P0 + P1
+    __syncthreads();
+    double* xch = &Ash[0][0][0];
+    if (half == 1) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) xch[((i * 2 + j) * 4 + r) * 256 + tid] = acc[i][j][r];
+    }
+    __syncthreads();
+    if (half == 1) return;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[i][j][r] += xch[((i * 2 + j) * 4 + r) * 256 + tid];
+  }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -602,12 +632,17 @@ int gemm_any(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, 
         K >= 512 && lda % 2 == 0 && ldb % 2 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
         (reinterpret_cast<uintptr_t>(B) & 15) == 0) {
       const int tm = M / 64, tn = N / 64;
-      if (beta == T(0))
-        hipLaunchKernelGGL(dgemm_long_k_kernel<true>, dim3(tm * tn), dim3(256), 0, s, M, N, K,
-                           alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
-      else
-        hipLaunchKernelGGL(dgemm_long_k_kernel<false>, dim3(tm * tn), dim3(256), 0, s, M, N, K,
-                           alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+      // no more tiles than CUs: two K halves per tile, so each SIMD runs two waves
+      const bool two = (int64_t)tm * tn <= cus && K % 64 == 0;
+#define DLK(B0, KS)                                                                          \
+  hipLaunchKernelGGL((dgemm_long_k_kernel<B0, KS>), dim3(tm * tn), dim3(256 * KS), 0, s, M, N, \
+                     K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn)
+      if (beta == T(0)) {
+        if (two) DLK(true, 2); else DLK(true, 1);
+      } else {
+        if (two) DLK(false, 2); else DLK(false, 1);
+      }
+#undef DLK
       DDLA_HIP_CHECK(hipGetLastError());
       return 0;
     }
