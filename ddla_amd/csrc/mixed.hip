@@ -363,6 +363,27 @@ __global__ __launch_bounds__(256) void row_abs_sum_kernel(int64_t n, const doubl
   if (lane == 0) out[row] = s;
 }
 
+// dst = (float) A (as d2s_kernel) and out[i] = sum_j |A[i][j]| (as row_abs_sum_kernel, same
+// order of the sums) in one pass over A: one wave per row
+__global__ __launch_bounds__(256) void d2s_rowsum_kernel(int64_t n, const double* __restrict__ A,
+                                                         int64_t lda, float* __restrict__ dst,
+                                                         int64_t ldd, double* __restrict__ out) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  const double* a = A + row * lda;
+  float* d = dst + row * ldd;
+#pragma unroll 4
+  for (int64_t j = lane; j < n; j += 64) {
+    const double v = a[j];
+    d[j] = static_cast<float>(v);
+    s += fabs(v);
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) out[row] = s;
+}
+
 // out[0] = max_i v[i] over n values (single workgroup)
 __global__ __launch_bounds__(1024) void max_kernel(int64_t n, const double* __restrict__ v,
                                                    double* __restrict__ out) {
@@ -1241,7 +1262,10 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
   // default split: FP16 x2 (22 bits) and BF16 x3 (24 bits) reach FP32 product accuracy
   w.terms = (opts && opts->terms > 0) ? std::min(opts->terms, 3)
                                       : (prec == DDLA_PREC_BF16 ? 3 : 2);
-  DDLA_RETURN_IF(norm_inf(n, a, lda, w));
+  // ||A||_inf: in the same pass as the FP32 copy of A, except for the butterfly LU (whose
+  // factor does not start from that copy)
+  const bool fuse_norm = kind != LU_BUTTERFLY;
+  if (!fuse_norm) DDLA_RETURN_IF(norm_inf(n, a, lda, w));
   Timer t;
   double factor_ms = 0.0, refine_ms = 0.0;
   int iters = 0;
@@ -1259,7 +1283,14 @@ int mpir(bool lu, int64_t n, int64_t nrhs, const double* a, int64_t lda, const d
       DDLA_HIP_CHECK(hipGetLastError());
       DDLA_RETURN_IF(sgetrf_nopiv_mixed(np, prec, w, &finfo));
     } else {
-      hipLaunchKernelGGL(d2s_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, n, n, a, lda, w.W, n);
+      if (attempt == 0 && fuse_norm) {
+        hipLaunchKernelGGL(d2s_rowsum_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, s, n, a, lda,
+                           w.W, n, w.rowsum);
+        hipLaunchKernelGGL(max_kernel, dim3(1), dim3(1024), 0, s, n, w.rowsum, w.normA);
+      } else {
+        hipLaunchKernelGGL(d2s_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, n, n, a, lda, w.W,
+                           n);
+      }
       DDLA_HIP_CHECK(hipGetLastError());
       if (kind == LU_PIVOT || kind == LU_TOURNAMENT) {
         DDLA_RETURN_IF(sgetrf_mixed(n, prec, kind == LU_TOURNAMENT, w.W, n, w, &finfo));

@@ -160,7 +160,39 @@ __host__ __device__ inline int64_t inv64_elems(int64_t n) {
 template <int BS>
 __host__ __device__ inline int64_t blocks_elems(int64_t n) { return blocks_of(n, BS) * BS * BS; }
 
-// Builds Dfull and M (see above) for one block row per workgroup, in LDS.
+// out (BS x BS, row-major) = S1 S2 for two BS x BS LDS blocks.  Thread (tr, tc) =
+// (tid / 16, tid % 16) owns rows tr + 16 i and columns tc + 16 j (an RB x RB register tile),
+// so that a k step reads RB values of each operand for RB * RB fmas; every output is still
+// one fma chain over k = 0, 1, ..., BS - 1 from zero.
+template <typename T, int BS>
+__device__ __forceinline__ void block_product(const T (*S1)[BS + 1], const T (*S2)[BS + 1],
+                                              T* __restrict__ out) {
+  constexpr int RB = BS / 16;
+  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+  T acc[RB][RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < RB; ++j) acc[i][j] = T(0);
+  for (int k = 0; k < BS; ++k) {
+    T a[RB], b[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) a[i] = S1[tr + 16 * i][k];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) b[j] = S2[k][tc + 16 * j];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int j = 0; j < RB; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+  }
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < RB; ++j) out[(tr + 16 * i) * BS + tc + 16 * j] = acc[i][j];
+}
+
+// Builds Dfull and M (blockIdx.y == 0) or N (blockIdx.y == 1), see above, for one block row
+// per workgroup, in LDS.  Both halves build Dfull; the first stores it.
 template <typename T, int BS, bool LOWER, bool TRANS>
 __global__ __launch_bounds__(256) void trsv_combine_kernel(int64_t n, const T* __restrict__ A,
                                                            int64_t lda, T* __restrict__ Dinv) {
@@ -169,6 +201,11 @@ __global__ __launch_bounds__(256) void trsv_combine_kernel(int64_t n, const T* _
   __shared__ T S2[BS][BS + 1];  // operand blocks
   const int nblk = (int)blocks_of(n, BS);
   const int kb = blockIdx.x;
+  const bool second = blockIdx.y == 1;
+  {
+    const int kneed = second ? (LOWER ? kb - 2 : kb + 2) : kb;
+    if (kneed < 0 || kneed >= nblk) return;  // uniform: no N block for this row
+  }
   const int64_t r0 = (int64_t)kb * BS;
   const int tid = threadIdx.x;
   T* Dfull = Dinv + inv64_elems(n) + (int64_t)kb * BS * BS;
@@ -216,37 +253,19 @@ __global__ __launch_bounds__(256) void trsv_combine_kernel(int64_t n, const T* _
     }
   }
   __syncthreads();
-  for (int idx = tid; idx < BS * BS; idx += 256) Dfull[idx] = S1[idx / BS][idx % BS];
-  const int kprev = LOWER ? kb - 1 : kb + 1;
+  if (!second)
+    for (int idx = tid; idx < BS * BS; idx += 256) Dfull[idx] = S1[idx / BS][idx % BS];
+  // M: the block solved just before kb; N: the one before that
+  const int kprev = second ? (LOWER ? kb - 2 : kb + 2) : (LOWER ? kb - 1 : kb + 1);
   if (kprev < 0 || kprev >= nblk) return;  // uniform
   const int64_t c0 = (int64_t)kprev * BS;
+  __syncthreads();  // S2 is rewritten
   for (int idx = tid; idx < BS * BS; idx += 256) {
     const int r = idx / BS, c = idx % BS;
     S2[r][c] = opT(r0 + r, c0 + c);
   }
   __syncthreads();
-  for (int idx = tid; idx < BS * BS; idx += 256) {
-    const int r = idx / BS, c = idx % BS;
-    T acc = T(0);
-    for (int k = 0; k < BS; ++k) acc = fma(S1[r][k], S2[k][c], acc);
-    Mb[idx] = acc;
-  }
-  const int kprev2 = LOWER ? kb - 2 : kb + 2;
-  if (kprev2 < 0 || kprev2 >= nblk) return;  // uniform
-  T* Nb = Mb + blocks_elems<BS>(n);
-  const int64_t c2 = (int64_t)kprev2 * BS;
-  __syncthreads();  // S2 is rewritten
-  for (int idx = tid; idx < BS * BS; idx += 256) {
-    const int r = idx / BS, c = idx % BS;
-    S2[r][c] = opT(r0 + r, c2 + c);
-  }
-  __syncthreads();
-  for (int idx = tid; idx < BS * BS; idx += 256) {
-    const int r = idx / BS, c = idx % BS;
-    T acc = T(0);
-    for (int k = 0; k < BS; ++k) acc = fma(S1[r][k], S2[k][c], acc);
-    Nb[idx] = acc;
-  }
+  block_product<T, BS>(S1, S2, second ? Mb + blocks_elems<BS>(n) : Mb);
 }
 
 template <typename T, int BS, int NR, bool LOWER, bool TRANS>
@@ -653,7 +672,8 @@ int trsv_prepare(bool lower, bool unit, bool trans, int64_t n, const T* A, int64
   constexpr int BS = solve_bs<T>();
   const int nbb = ceil_div(n, BS);
 #define TC(L, TR) \
-  hipLaunchKernelGGL((trsv_combine_kernel<T, BS, L, TR>), dim3(nbb), dim3(256), 0, s, n, A, lda, Dinv)
+  hipLaunchKernelGGL((trsv_combine_kernel<T, BS, L, TR>), dim3(nbb, 2), dim3(256), 0, s, n, A, \
+                     lda, Dinv)
   if (lower) {
     if (trans) TC(true, true); else TC(true, false);
   } else {
