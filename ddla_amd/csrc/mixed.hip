@@ -399,37 +399,6 @@ __global__ __launch_bounds__(1024) void max_kernel(int64_t n, const double* __re
   }
 }
 
-// per column j of the n x nrhs matrices: out[2j] = ||R_j||_inf, out[2j+1] = ||X_j||_inf
-// out[2j] = max_i |R[i][j]|, out[2j+1] = max_i |X[i][j]| (NaN counts as infinity) as the bit
-// patterns of non-negative doubles, which order like the values: column j = blockIdx.y,
-// a wave-level maximum first, then one atomicMax per wave.
-__global__ __launch_bounds__(256) void col_inf_norms_kernel(int64_t n, int64_t nrhs,
-                                                            const double* __restrict__ R,
-                                                            const double* __restrict__ X,
-                                                            int64_t ldx,
-                                                            unsigned long long* __restrict__ out,
-                                                            const int* __restrict__ skip) {
-  if (skipped(skip)) return;
-  const int64_t j = blockIdx.y;
-  double rm = 0.0, xm = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double r = fabs(R[i * nrhs + j]), x = fabs(X[i * ldx + j]);
-    if (r != r) r = INFINITY;
-    if (x != x) x = INFINITY;
-    rm = fmax(rm, r);
-    xm = fmax(xm, x);
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    rm = fmax(rm, __shfl_xor(rm, off));
-    xm = fmax(xm, __shfl_xor(xm, off));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(out + 2 * j, __double_as_longlong(rm));
-    atomicMax(out + 2 * j + 1, __double_as_longlong(xm));
-  }
-}
-
 // R[i][c] -= sum_k A[i][k] X[k][c] for NR right-hand sides: one wave per RPW rows of the
 // FP64 matrix, lanes along the rows (coalesced 512-byte reads; each X value serves RPW rows),
 // a butterfly sum at the end.  The refinement's residual reads A once per step, so this runs
@@ -492,33 +461,33 @@ struct IrState {
   int done;     // 1: converged, stagnated, not finite or out of steps (later steps do nothing)
   int iters;    // steps taken
   int slow;     // consecutive steps that lowered the backward error by less than 10%
-  int pad;
+  unsigned arrived;  // workgroups of the current step's norm launch that have finished
   double berr;  // backward error after the last step
   double prev;  // backward error before it
 };
 
-__global__ void ir_state_init_kernel(IrState* st) {
+__global__ void ir_state_init_kernel(IrState* st, unsigned long long* norms, int nwords) {
+  for (int i = 0; i < nwords; ++i) norms[i] = 0ull;
   st->done = 0;
   st->iters = 0;
   st->slow = 0;
-  st->pad = 0;
+  st->arrived = 0;
   st->berr = INFINITY;
   st->prev = INFINITY;
 }
 
 // The convergence test of one refinement step, in FP64 on the GPU: berr = max_j ||r_j||_inf /
-// (||A||_inf ||x_j||_inf) from the norm words of col_inf_norms_kernel; done when berr <= tol,
-// after max_iters steps, when the error is not finite, or when three steps in a row lower it
-// by less than 10% (stagnation).
-__global__ void ir_check_kernel(int nrhs, const unsigned long long* __restrict__ norms,
-                                const double* __restrict__ normA, double tol, int max_iters,
-                                IrState* __restrict__ st) {
-  if (threadIdx.x != 0 || st->done) return;
-  const double na = *normA;
+// (||A||_inf ||x_j||_inf) from the norm words (see ir_norms_check_kernel); done when
+// berr <= tol, after max_iters steps, when the error is not finite, or when three steps in a
+// row lower it by less than 10% (stagnation).
+__device__ void ir_check(int nrhs, const unsigned long long* norms, double na, double tol,
+                         int max_iters, IrState* st) {
   double worst = 0.0;
   for (int j = 0; j < nrhs; ++j) {
-    const double r = __longlong_as_double((long long)norms[2 * j]);
-    const double x = __longlong_as_double((long long)norms[2 * j + 1]);
+    const double r = __longlong_as_double((long long)__hip_atomic_load(
+        norms + 2 * j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const double x = __longlong_as_double((long long)__hip_atomic_load(
+        norms + 2 * j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     double e = (x > 0.0 && na > 0.0) ? r / (na * x) : (r == 0.0 ? 0.0 : INFINITY);
     if (!isfinite(r) || !isfinite(x) || isnan(e)) e = INFINITY;  // NaN / Inf: failed
     worst = fmax(worst, e);
@@ -532,6 +501,54 @@ __global__ void ir_check_kernel(int nrhs, const unsigned long long* __restrict__
   st->slow = (worst > 0.9 * st->prev) ? st->slow + 1 : 0;
   if (st->slow >= 3) st->done = 1;
   st->prev = worst;
+}
+
+// One launch per refinement step for the norms and the test: norms[2j] = max_i |R[i][j]|,
+// norms[2j+1] = max_i |X[i][j]| (bit patterns of non-negative doubles, NaN as infinity;
+// atomic max, so the words do not depend on the order in which workgroups arrive), column
+// j = blockIdx.y.  The last workgroup to arrive runs ir_check, then clears the words and the
+// arrival count for the next step (ir_state_init_kernel clears them before the first).  A
+// finished refinement (st->done, set only by a last workgroup, after every workgroup of its
+// launch has read it) makes the launch a no-op.
+__global__ __launch_bounds__(256) void ir_norms_check_kernel(
+    int64_t n, int64_t nrhs, const double* __restrict__ R, const double* __restrict__ X,
+    int64_t ldx, unsigned long long* __restrict__ norms, const double* __restrict__ normA,
+    double tol, int max_iters, IrState* __restrict__ st) {
+  if (st->done) return;
+  const int64_t j = blockIdx.y;
+  double rm = 0.0, xm = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double r = fabs(R[i * nrhs + j]), x = fabs(X[i * ldx + j]);
+    if (r != r) r = INFINITY;
+    if (x != x) x = INFINITY;
+    rm = fmax(rm, r);
+    xm = fmax(xm, x);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    rm = fmax(rm, __shfl_xor(rm, off));
+    xm = fmax(xm, __shfl_xor(xm, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    __hip_atomic_fetch_max(norms + 2 * j, (unsigned long long)__double_as_longlong(rm),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(norms + 2 * j + 1, (unsigned long long)__double_as_longlong(xm),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();  // this wave's maxima are performed before the barrier
+  }
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned total = gridDim.x * gridDim.y;
+    last = __hip_atomic_fetch_add(&st->arrived, 1u, __ATOMIC_ACQ_REL,
+                                  __HIP_MEMORY_SCOPE_AGENT) == total - 1;
+    if (last) {
+      ir_check((int)nrhs, norms, *normA, tol, max_iters, st);
+      for (int w = 0; w < 2 * (int)nrhs; ++w)
+        __hip_atomic_store(norms + w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      st->arrived = 0;
+    }
+  }
 }
 
 // out[2j] = max_i |X[i][j] - Xr[i][j]|, out[2j+1] = max_i |Xr[i][j]| (bit patterns of
@@ -1128,7 +1145,7 @@ enum FactorKind { LU_PIVOT = 0, LU_BUTTERFLY = 1, CHOLESKY = 2, LU_TOURNAMENT = 
 // Classic refinement: X += M^-1 R, R = B - A X, with M the FP32 factor of kind `kind`.
 // Stops at backward error <= tol, after max_iters steps, or when three steps in a row fail
 // to lower the backward error by 10% (stagnation).  The test runs on the GPU after every step
-// (ir_check_kernel); the host queues STEPS steps at a time and reads the state once per group
+// (ir_norms_check_kernel); the host queues STEPS steps at a time and reads the state once per group
 // (steps queued after convergence do nothing).  With more than 8 right-hand sides the
 // triangular solves and the residual are library GEMMs without the skip word, so the host
 // reads the state after every step there.
@@ -1147,7 +1164,8 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
   // X = 0, R = B
   DDLA_RETURN_IF(launch_dscale2d(n, nrhs, 0.0, X, ldx, 1, 0, s));
   DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, B, ldb, 1, w.R, nrhs, s));
-  hipLaunchKernelGGL(ir_state_init_kernel, dim3(1), dim3(1), 0, s, w.state);
+  hipLaunchKernelGGL(ir_state_init_kernel, dim3(1), dim3(1), 0, s, w.state, w.norms,
+                     (int)(2 * nrhs));
   DDLA_HIP_CHECK(hipGetLastError());
   IrState st{};
   for (int queued = 0; queued < max_iters;) {
@@ -1195,12 +1213,10 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
         DDLA_RETURN_IF(launch_dgemm(n, nrhs, n, -1.0, A, lda, true, X, ldx, true, 1.0, w.R, nrhs,
                                     0, s));
       }
-      hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, s, 2 * nrhs, w.norms, skip);
-      hipLaunchKernelGGL(col_inf_norms_kernel,
+      hipLaunchKernelGGL(ir_norms_check_kernel,
                          dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 64), (unsigned)nrhs),
-                         dim3(256), 0, s, n, nrhs, w.R, X, ldx, w.norms, skip);
-      hipLaunchKernelGGL(ir_check_kernel, dim3(1), dim3(64), 0, s, (int)nrhs, w.norms, w.normA,
-                         tol, max_iters, w.state);
+                         dim3(256), 0, s, n, nrhs, w.R, X, ldx, w.norms, w.normA, tol, max_iters,
+                         w.state);
       DDLA_HIP_CHECK(hipGetLastError());
     }
     queued += group;
