@@ -80,6 +80,8 @@ struct Context {
   // default priority like the main one; events order the two
   hipStream_t aux_stream = nullptr;
   hipEvent_t ev[4] = {};
+  // small pinned host buffer (asynchronous device-to-host copies of status words)
+  void* pinned = nullptr;
   // growable scratch buffers (device memory kept for the life of the context)
   void* ws[WS_COUNT] = {};
   size_t ws_bytes[WS_COUNT] = {};
@@ -103,6 +105,9 @@ int noalloc_setup(int device, hipStream_t s);
 hipStream_t stream();
 // the auxiliary stream and its events (created on first use)
 int aux_stream(hipStream_t* s, hipEvent_t** ev);
+// the context's pinned host buffer (allocated on first use, kept for the process)
+constexpr size_t PINNED_BYTES = 4096;
+int pinned_host(void** p);
 // Returns a device scratch buffer `slot` of at least `bytes` (contents undefined).
 int workspace(int slot, size_t bytes, void** out);
 // The same, with the first `bytes` bytes cleared at least once since the buffer was

@@ -1145,10 +1145,10 @@ enum FactorKind { LU_PIVOT = 0, LU_BUTTERFLY = 1, CHOLESKY = 2, LU_TOURNAMENT = 
 // Classic refinement: X += M^-1 R, R = B - A X, with M the FP32 factor of kind `kind`.
 // Stops at backward error <= tol, after max_iters steps, or when three steps in a row fail
 // to lower the backward error by 10% (stagnation).  The test runs on the GPU after every step
-// (ir_norms_check_kernel); the host queues STEPS steps at a time and reads the state once per group
-// (steps queued after convergence do nothing).  With more than 8 right-hand sides the
+// (ir_norms_check_kernel); the host queues STEPS steps at a time, one group ahead of the state
+// it reads (steps queued after convergence do nothing).  With more than 8 right-hand sides the
 // triangular solves and the residual are library GEMMs without the skip word, so the host
-// reads the state after every step there.
+// reads the state after every step there before it queues the next.
 int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
            int64_t ldb, double* X, int64_t ldx, MixedWs& w, double tol, int max_iters, int* iters,
            double* berr_out) {
@@ -1160,15 +1160,31 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
   const float* RV = w.rbt + 2 * np;
   const bool few = nrhs <= 8;
   const int* skip = few ? &w.state->done : nullptr;
-  const int STEPS = few ? 4 : 1;
+  // Steps are queued in groups.  With few right-hand sides every kernel of a step is a no-op
+  // once the device-side test has converged, so the next group is queued before the host reads
+  // the state of the current one (the GPU does not wait for the host between groups; at most
+  // one group of no-op steps follows convergence).  State and the solves' abort word reach
+  // the host by asynchronous copies into pinned memory, one slot per group parity.
+  const int STEPS = few ? 2 : 1;
   // X = 0, R = B
   DDLA_RETURN_IF(launch_dscale2d(n, nrhs, 0.0, X, ldx, 1, 0, s));
   DDLA_RETURN_IF(launch_dcopy2d(n, nrhs, B, ldb, 1, w.R, nrhs, s));
   hipLaunchKernelGGL(ir_state_init_kernel, dim3(1), dim3(1), 0, s, w.state, w.norms,
                      (int)(2 * nrhs));
   DDLA_HIP_CHECK(hipGetLastError());
-  IrState st{};
-  for (int queued = 0; queued < max_iters;) {
+  struct Slot {
+    IrState st;
+    unsigned abort;
+  };
+  static_assert(2 * sizeof(Slot) <= PINNED_BYTES, "pinned slots");
+  void* pin = nullptr;
+  DDLA_RETURN_IF(pinned_host(&pin));
+  Slot* slots = static_cast<Slot*>(pin);
+  hipStream_t s2;
+  hipEvent_t* ev;
+  DDLA_RETURN_IF(aux_stream(&s2, &ev));  // ev[2], ev[3]: the group copies' completion
+  int queued = 0;
+  auto queue_group = [&](int gi) -> int {
     const int group = std::min(STEPS, max_iters - queued);
     for (int q = 0; q < group; ++q) {
       // D = M^-1 R in FP32
@@ -1220,10 +1236,28 @@ int refine(FactorKind kind, int64_t n, int64_t nrhs, const double* A, int64_t ld
       DDLA_HIP_CHECK(hipGetLastError());
     }
     queued += group;
-    DDLA_HIP_CHECK(hipMemcpyAsync(&st, w.state, sizeof(IrState), hipMemcpyDeviceToHost, s));
-    DDLA_HIP_CHECK(hipStreamSynchronize(s));
-    if (few) DDLA_RETURN_IF(check_abort(trsv_abort_word(), s, "refinement triangular solve"));
+    Slot* sl = slots + (gi & 1);
+    DDLA_HIP_CHECK(hipMemcpyAsync(&sl->st, w.state, sizeof(IrState), hipMemcpyDeviceToHost, s));
+    if (few)
+      DDLA_HIP_CHECK(hipMemcpyAsync(&sl->abort, trsv_abort_word(), sizeof(unsigned),
+                                    hipMemcpyDeviceToHost, s));
+    DDLA_HIP_CHECK(hipEventRecord(ev[2 + (gi & 1)], s));
+    return 0;
+  };
+  IrState st{};
+  DDLA_RETURN_IF(queue_group(0));
+  int last = 0;  // the last group queued
+  for (int g = 0;; ++g) {
+    if (few && last == g && queued < max_iters) DDLA_RETURN_IF(queue_group(last = g + 1));
+    DDLA_HIP_CHECK(hipEventSynchronize(ev[2 + (g & 1)]));
+    st = slots[g & 1].st;
+    if (few && slots[g & 1].abort)
+      DDLA_RETURN_IF(check_abort(trsv_abort_word(), s, "refinement triangular solve"));
     if (st.done) break;
+    if (last == g) {  // nothing queued beyond this group
+      if (queued >= max_iters) break;
+      DDLA_RETURN_IF(queue_group(last = g + 1));
+    }
   }
   *iters = st.iters;
   *berr_out = st.berr;

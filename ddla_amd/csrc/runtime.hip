@@ -65,6 +65,12 @@ int aux_stream(hipStream_t* s, hipEvent_t** ev) {
   return 0;
 }
 
+int pinned_host(void** p) {
+  if (!g_ctx.pinned) DDLA_HIP_CHECK(hipHostMalloc(&g_ctx.pinned, PINNED_BYTES, hipHostMallocDefault));
+  *p = g_ctx.pinned;
+  return 0;
+}
+
 int ensure_init() {
   if (g_ctx.ready) {
     DDLA_HIP_CHECK(hipSetDevice(g_ctx.device));
