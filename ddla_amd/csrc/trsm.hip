@@ -48,11 +48,28 @@ __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   auto row_of = [&](int q) { return LOWER ? q : bs - 1 - q; };
   const int64_t c0 = ((int64_t)blockIdx.x * 4 + wave) * CW;
+  const bool kfast = cs == 1;  // consecutive threads read consecutive addresses
+  constexpr int PER = TB * TB / 256, BATCH = sizeof(T) == 4 ? 32 : 16;
   int cnt = 0;
+  T tv[PIV ? PER : 1];  // PIV: this thread's part of the triangle, in flight during the list work
   if constexpr (PIV) {
-    cnt = min(*count, 2 * TB);
+    // The count, the whole list area (4 TB words; the entries past 2 cnt are not used) and the
+    // triangle are requested together: one memory round trip before the list is used, not
+    // three in a row.
+    static_assert(4 * TB == 2 * 256, "list area = two words per thread");
+    const int c = *count;
+    const int l0 = list[tid], l1 = list[tid + 256];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int p = i * 256 + tid;
+      const int q = kfast ? p / TB : p % TB, k = kfast ? p % TB : p / TB;
+      const bool in = q < bs && k <= q;
+      tv[i] = A[in ? (int64_t)row_of(q) * rs + (int64_t)row_of(k) * cs : 0];
+    }
+    cnt = min(c, 2 * TB);
+    lst[tid] = l0;
+    lst[tid + 256] = l1;
     for (int t = tid; t < TB; t += 256) srcq[t] = t;
-    for (int e = tid; e < 2 * cnt; e += 256) lst[e] = list[e];
     __syncthreads();
     for (int e = tid; e < cnt; e += 256)
       if (lst[2 * e] < bs) srcq[lst[2 * e]] = lst[2 * e + 1];
@@ -68,12 +85,21 @@ __global__ __launch_bounds__(256) void trsm_slab_kernel(int bs, int64_t ncols,
     x0[c] = B[i0 ? (int64_t)s0 * ldb + c0 + c : 0];
     x1[c] = B[i1 ? (int64_t)s1 * ldb + c0 + c : 0];
   }
-  const bool kfast = cs == 1;  // consecutive threads read consecutive addresses
-  // Staging in batches of 32 (FP64: 16) reads per thread, issued back to back (branch-free: an
-  // out-of-range element reads A[0] and is replaced by zero), then written to LDS.
-  constexpr int PER = TB * TB / 256, BATCH = sizeof(T) == 4 ? 32 : 16;
+  if constexpr (PIV) {
 #pragma unroll
-  for (int b0 = 0; b0 < PER; b0 += BATCH) {
+    for (int i = 0; i < PER; ++i) {
+      const int p = i * 256 + tid;
+      const int q = kfast ? p / TB : p % TB, k = kfast ? p % TB : p / TB;
+      const T x = (q < bs && k <= q) ? tv[i] : T(0);
+      if (k == q) dg[q] = x;
+      Mt[q][k] = k == q ? T(0) : x;
+    }
+  }
+  // Otherwise staging in batches of 32 (FP64: 16) reads per thread, issued back to back
+  // (branch-free: an out-of-range element reads A[0] and is replaced by zero), then written to
+  // LDS.
+#pragma unroll
+  for (int b0 = 0; b0 < (PIV ? 0 : PER); b0 += BATCH) {
     T v[BATCH];
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
