@@ -67,7 +67,8 @@ enum WorkspaceSlot {
   WS_EIN_A = 21,   // tensor contraction: gathered operands and product
   WS_EIN_B = 22,
   WS_EIN_C = 23,
-  WS_COUNT = 24,
+  WS_PIVLIST = 24, // relative pivots as per-block row-move lists (launch_piv_to_perm)
+  WS_COUNT = 25,
 };
 
 struct Context {

@@ -1463,45 +1463,95 @@ __global__ __launch_bounds__(256) void perm_compose_kernel(int n, int npanels, i
                                                            int* __restrict__ perm_out) {
   extern __shared__ int perm[];
   for (int i = threadIdx.x; i < n; i += 256) perm[i] = i;
+  // The lists are complete before the launch: panel k + 1's count and entries are requested
+  // while panel k is applied (entry e < 2 nb lies inside the panel's 4 nb-word list area).
+  int ncnt = 0, nd[2] = {0, 0}, ns[2] = {0, 0};
+  auto fetch = [&](int k) {
+    ncnt = counts[k];
+    const int* L = lists + (size_t)k * 4 * nb;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      if (e < 2 * nb) {
+        nd[q] = L[2 * e];
+        ns[q] = L[2 * e + 1];
+      }
+    }
+  };
+  if (npanels > 0) fetch(0);
   __syncthreads();
   for (int k = 0; k < npanels; ++k) {
     const int row0 = k * nb;
-    const int cnt = counts[k];
-    const int* L = lists + (size_t)k * 4 * nb;
+    const int cnt = ncnt;
+    const int d[2] = {nd[0], nd[1]}, sr[2] = {ns[0], ns[1]};
+    if (k + 1 < npanels) fetch(k + 1);
     int v[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int e = threadIdx.x + 256 * q;
-      v[q] = (e < cnt) ? perm[row0 + L[2 * e + 1]] : 0;
+      v[q] = (e < cnt) ? perm[row0 + sr[q]] : 0;
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int e = threadIdx.x + 256 * q;
-      if (e < cnt) perm[row0 + L[2 * e]] = v[q];
+      if (e < cnt) perm[row0 + d[q]] = v[q];
     }
     __syncthreads();
   }
   for (int i = threadIdx.x; i < n; i += 256) perm_out[i] = perm[i];
 }
 
-// Sequential application of relative pivots to an index vector (one lane, LDS).
-__global__ void piv_to_perm_kernel(int n, const int32_t* __restrict__ piv, int* __restrict__ out) {
-  extern __shared__ int perm[];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = i;
+// The row moves of one block of PB consecutive relative pivots (rows i and i + piv[i]
+// exchanged in order of i; piv[i] <= 0, or a partner past n, exchanges nothing) in the list form
+// perm_compose_kernel applies: one wave replays the block's exchanges on a table of the
+// positions they touch -- the block's own rows directly, later rows in an extra list searched
+// by a ballot -- and writes (dst, src) pairs relative to the block's first row: after the
+// block, position dst holds what position src held before it.
+constexpr int PB = 128;
+__global__ __launch_bounds__(64) void piv_block_lists_kernel(int n, const int32_t* __restrict__ piv,
+                                                             int* __restrict__ counts,
+                                                             int* __restrict__ lists) {
+  __shared__ int md[PB];             // content of the block's rows
+  __shared__ int ep[PB], ev[PB];     // later rows touched: position, content
+  const int k = blockIdx.x, b0 = k * PB, rows = min(PB, n - b0), lane = threadIdx.x;
+  for (int t = lane; t < PB; t += 64) md[t] = b0 + t;
+  const int p0 = lane < rows ? piv[b0 + lane] : 0;
+  const int p1 = lane + 64 < rows ? piv[b0 + lane + 64] : 0;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < n; ++i) {
-      const int p = piv[i];
-      if (p > 0) {
-        const int a = perm[i], b = perm[i + p];
-        perm[i] = b;
-        perm[i + p] = a;
+  int m = 0;  // extra rows so far (the same in every lane)
+  for (int r = 0; r < rows; ++r) {
+    const int p = __shfl(r < 64 ? p0 : p1, r & 63);
+    const int t = b0 + r + p;
+    if (p <= 0 || t >= n) continue;  // uniform
+    if (t < b0 + rows) {
+      if (lane == 0) {
+        const int a = md[r];
+        md[r] = md[t - b0];
+        md[t - b0] = a;
       }
+    } else {
+      const unsigned long long h0 = __ballot(lane < m && ep[lane] == t);
+      const unsigned long long h1 = __ballot(lane + 64 < m && ep[lane + 64] == t);
+      const int j = h0 ? __ffsll((long long)h0) - 1 : h1 ? 64 + __ffsll((long long)h1) - 1 : m;
+      if (lane == 0) {
+        if (j == m) ev[j] = t;
+        ep[j] = t;
+        const int a = md[r];
+        md[r] = ev[j];
+        ev[j] = a;
+      }
+      m += j == m;
     }
+    __syncthreads();
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = perm[i];
+  int* L = lists + (size_t)k * 4 * PB;
+  for (int e = lane; e < rows + m; e += 64) {
+    const bool own = e < rows;
+    L[2 * e] = own ? e : ep[e - rows] - b0;
+    L[2 * e + 1] = (own ? md[e] : ev[e - rows]) - b0;
+  }
+  if (lane == 0) counts[k] = rows + m;
 }
 
 // *bad = 1 + the first i with piv[i] > 0 and i + piv[i] >= n (a swap outside the matrix, where
@@ -1791,21 +1841,24 @@ int launch_perm_compose(int n, int npanels, int nb, const int* lists, const int*
   return 0;
 }
 
+// perm[i] = the row that relative pivots piv[0..n) bring to position i: the moves of each block
+// of PB pivots in parallel (piv_block_lists_kernel), then the blocks in order
+// (perm_compose_kernel), instead of n exchanges in one lane.
 int launch_piv_to_perm(int64_t n, const int32_t* piv, int32_t* perm, hipStream_t s) {
-  const size_t sh = (size_t)n * sizeof(int);
-  if (sh > 160 * 1024) {
+  if (n <= 0) return 0;
+  if ((size_t)n * sizeof(int) > 160 * 1024) {
     set_error("permutation of %lld rows does not fit in LDS", (long long)n);
     return DDLA_ERR_UNSUPPORTED;
   }
-  static bool attr = false;
-  if (!attr) {
-    DDLA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(piv_to_perm_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr = true;
-  }
-  hipLaunchKernelGGL(piv_to_perm_kernel, dim3(1), dim3(256), sh, s, (int)n, piv, perm);
+  const int nblk = ceil_div(n, PB);
+  void* ws = nullptr;
+  DDLA_RETURN_IF(workspace(WS_PIVLIST, (size_t)nblk * (1 + 4 * PB) * sizeof(int), &ws));
+  int* counts = static_cast<int*>(ws);
+  int* lists = counts + nblk;
+  hipLaunchKernelGGL(piv_block_lists_kernel, dim3(nblk), dim3(64), 0, s, (int)n, piv, counts,
+                     lists);
   DDLA_HIP_CHECK(hipGetLastError());
-  return 0;
+  return launch_perm_compose((int)n, nblk, PB, lists, counts, perm, s);
 }
 
 __global__ void fill_u32_kernel(unsigned* __restrict__ p, int64_t count, unsigned v) {

@@ -135,6 +135,27 @@ def test_laswp_device(ddla, oracle):
     np.testing.assert_array_equal(d.to_host(), oracle.laswp(Y0.copy(), piv, side=oracle.RIGHT))
 
 
+def test_laswp_device_pivots_across_blocks(ddla, oracle):
+    # the pivots are turned into row moves per block of 128 and then composed: runs of rows
+    # that all exchange with one far row, partners several blocks ahead, a short last block
+    n = 1000
+    rng = np.random.default_rng(7)
+    piv = np.zeros(n, dtype=np.int32)
+    for i in range(n):
+        r = rng.random()
+        if r < 0.3:
+            piv[i] = max(0, 900 - i)          # many rows with the same partner
+        elif r < 0.6:
+            piv[i] = rng.integers(0, n - i)   # anywhere below
+        elif r < 0.8:
+            piv[i] = min(n - 1 - i, 1)        # the next row
+    X0 = oracle.uniform_matrix(n, 8, 6)
+    for forward in (True, False):
+        d = ddla.DeviceMatrix.from_host(X0)
+        ddla.ApplyPiv(d, piv, forward=forward)
+        np.testing.assert_array_equal(d.to_host(), oracle.laswp(X0.copy(), piv, forward=forward))
+
+
 @pytest.mark.slow
 def test_lu_8192_pivot_sequence_and_backward_error(ddla, oracle):
     # config: n=8192, A uniform[-1,1], b = A x_true; pivot sequence must match exactly
