@@ -169,6 +169,37 @@ __device__ __forceinline__ void gemm_tile_of(int pid, int tiles_m, int tiles_n, 
   tn = in_group / gm;
 }
 
+// Tiles (i, j) with j <= i (lower triangle of C, square tiles, C's diagonal through tile (0, 0))
+// of a tiles_m x tiles_n grid: rows i < D = min(tiles_m, tiles_n) hold i + 1 of them, later
+// rows all tiles_n.
+__host__ __device__ inline int64_t lower_tile_count(int tiles_m, int tiles_n) {
+  const int64_t d = tiles_m < tiles_n ? tiles_m : tiles_n;
+  return d * (d + 1) / 2 + (int64_t)(tiles_m - d) * tiles_n;
+}
+// The pid-th of those tiles, spread over the XCDs as gemm_tile_of spreads a full grid: XCD x
+// (pid % 8 == x) takes a contiguous range of the row-major enumeration, so every XCD gets the
+// same number of tiles (a full-grid order with the upper tiles skipped leaves the XCD of the
+// last rows with most of the work).
+__device__ __forceinline__ void lower_tile_of(int pid, int tiles_m, int tiles_n, int& tm,
+                                              int& tn) {
+  const int T = (int)lower_tile_count(tiles_m, tiles_n);
+  const int T8 = (T / 8) * 8;
+  const int t = pid < T8 ? (pid % 8) * (T8 / 8) + pid / 8 : pid;
+  const int d = tiles_m < tiles_n ? tiles_m : tiles_n;
+  const int t1 = d * (d + 1) / 2;
+  if (t < t1) {
+    int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    while (i * (i + 1) / 2 > t) --i;
+    tm = i;
+    tn = t - i * (i + 1) / 2;
+  } else {
+    const int u = t - t1;
+    tm = d + u / tiles_n;
+    tn = u % tiles_n;
+  }
+}
+
 // ------------------------------------------------------------ kernels (launchers) --------
 // FP64 GEMM on row-major C: C = alpha*A*B + beta*C, A m x k, B k x n.
 //   a_kcontig: A(i,k) at A[i*lda + k], else A[i + k*lda]

@@ -192,7 +192,10 @@ __global__ __launch_bounds__(256) void hgemm_kernel(int M, int N, int K, float a
   __shared__ __attribute__((aligned(16))) E Bs[TERMS][BN * PK];
 
   int tmi, tni;
-  gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
+  if (TRI == 1)  // launched over the lower tiles only
+    lower_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
+  else
+    gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
   const int m0 = tmi * BM, n0 = tni * BN;
   if (TRI == 1 && n0 > m0 + BM - 1) return;
   const float scA = sa[tmi], scB = sb[tni];
@@ -765,13 +768,14 @@ int launch_hgemm_parts(bool bf16, int terms, int tri, int64_t m, int64_t n, int6
   const int tm = ceil_div(m, 128), tn = ceil_div(n, 128);
   const int Kp = ((int)k + 31) / 32 * 32;
   const double mn = (tri ? 0.5 : 1.0) * (double)m * n;
+  const int64_t tiles = tri ? lower_tile_count(tm, tn) : (int64_t)tm * tn;
   ProfScope prof(bf16 ? "hgemm_mfma_bf16" : "hgemm_mfma_f16",
                  2.0 * mn * k * std::max(1, terms * (terms + 1) / 2),
                  4.0 * 2.0 * mn + 2.0 * terms * ((double)m + n) * k, s);
 #define HG(BF, TM, T)                                                                             \
   do {                                                                                            \
     typedef typename LowP<BF>::e E;                                                               \
-    hipLaunchKernelGGL((hgemm_kernel<BF, TM, T>), dim3((unsigned)((int64_t)tm * tn)), dim3(256),  \
+    hipLaunchKernelGGL((hgemm_kernel<BF, TM, T>), dim3((unsigned)tiles), dim3(256),               \
                        0, s, (int)m, (int)n, Kp, alpha, static_cast<const E*>(a.parts),           \
                        static_cast<const E*>(b.parts), C, ldc, a.scales, b.scales, tm, tn,        \
                        a.plane, b.plane);                                                         \
