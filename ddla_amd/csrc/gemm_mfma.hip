@@ -193,7 +193,13 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, typen
   }
 
   int tmi, tni;
-  gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
+  static_assert(TRI == 0 || C::BM == C::BN, "triangular launches need square tiles");
+  if (TRI == 1)  // launched over the tiles of the lower triangle only
+    lower_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
+  else if (TRI == 2)  // upper: the lower tiles of the transposed grid
+    lower_tile_of(blockIdx.x, tiles_n, tiles_m, tni, tmi);
+  else
+    gemm_tile_of(blockIdx.x, tiles_m, tiles_n, tmi, tni);
   const int m0 = tmi * C::BM, n0 = tni * C::BN;
   if (TRI == 1 && n0 > m0 + C::BM - 1) return;  // tile strictly above the diagonal
   if (TRI == 2 && m0 > n0 + C::BN - 1) return;  // tile strictly below the diagonal
@@ -313,9 +319,11 @@ int launch_cfg(int M, int N, int K, typename C::E alpha, const typename C::E* A,
   const int64_t bz = BNC ? (int64_t)kchunk * ldb : (int64_t)kchunk;
   const int64_t cz = splits > 1 ? (int64_t)M * ldc : 0;
   const int tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
-  const int64_t tiles = (int64_t)tm * tn;
-  if (tiles > INT32_MAX) {
-    set_error("gemm: problem too large (%lld tiles)", (long long)tiles);
+  const int64_t tiles = tri == 1   ? lower_tile_count(tm, tn)
+                        : tri == 2 ? lower_tile_count(tn, tm)
+                                   : (int64_t)tm * tn;
+  if ((int64_t)tm * tn > INT32_MAX) {
+    set_error("gemm: problem too large (%lld tiles)", (long long)tm * tn);
     return DDLA_ERR_INVALID_ARG;
   }
   const bool veca = (reinterpret_cast<uintptr_t>(A) % (2 * sizeof(typename C::E)) == 0) && (lda % 2 == 0);
