@@ -203,10 +203,14 @@ __global__ __launch_bounds__(256) void potf2_rec_kernel(int nb, T* __restrict__ 
       for (int c = 0; c < CB; ++c) S[i][j0 + c] = x[c];
     }
     __syncthreads();
-    // (c) trailing lower triangle: a_ic -= sum_p l_ip l_cp, p over the block in order
-    for (int e = tid; e < m2 * m2; e += 256) {
-      const int ii = e / m2, cc = e % m2;
-      if (cc > ii) continue;
+    // (c) trailing lower triangle: a_ic -= sum_p l_ip l_cp, p over the block in order; the
+    // m2 (m2 + 1) / 2 elements with cc <= ii enumerated row by row (no idle lanes for the
+    // upper half)
+    for (int e = tid; e < m2 * (m2 + 1) / 2; e += 256) {
+      int ii = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+      while ((ii + 1) * (ii + 2) / 2 <= e) ++ii;
+      while (ii * (ii + 1) / 2 > e) --ii;
+      const int cc = e - ii * (ii + 1) / 2;
       const int i = i0 + ii, c = i0 + cc;
       T x = S[i][c];
 #pragma unroll
