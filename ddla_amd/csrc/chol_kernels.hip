@@ -1,8 +1,10 @@
 // Kernels of the blocked Cholesky (lower).  Reference: CholeskyLowerBlock /
 // CholeskyLowerUnblock / Sqrt (DDLA/Factorizations/Cholesky.cs:127-160,228-236).
 //
-//   potf2_rec_kernel  the nb x nb diagonal block in LDS, in 32-column blocks: the diagonal
-//                     32 x 32 in one wave's registers, substitution and rank-32 update by all
+//   potf2_rec_kernel  the nb x nb diagonal block in LDS, in 16-column blocks: the diagonal
+//                     16 x 16 in one wave's registers, substitution and rank-16 update by all
+//                     (16 measured faster than 32 and 8: the single-wave diagonal step grows
+//                     with the block width, the number of barriers with its inverse)
 //   trsm_rlt_kernel   A21 <- A21 * L11^-T (or * U11^-1), four lanes per row of A21 with the
 //                     row in registers, the triangle in LDS
 //   transpose_kernel  in-place transpose of a square matrix (the upper factorization is the
@@ -113,7 +115,7 @@ __device__ __forceinline__ T rl(T x, int lane) {
 template <typename T>
 __global__ __launch_bounds__(256) void potf2_rec_kernel(int nb, T* __restrict__ A, int64_t lda,
                                                         int32_t* __restrict__ info, int info_base) {
-  constexpr int CB = 32;
+  constexpr int CB = 16;
   __shared__ T S[NB][NB + 1];
   __shared__ T piv_s[NB], piv_r[NB];  // sqrt(a_kk) and its reciprocal
   __shared__ int stop;
